@@ -1,0 +1,183 @@
+/*
+ * tchll.h — C ABI of the MI355X (gfx950) triangle-counting hot path.
+ *
+ * Plain pointers and sizes only.  Every array argument is a DEVICE pointer
+ * unless its comment says "host".  `stream` is a hipStream_t passed as void*
+ * (NULL = the null stream).  Calls are asynchronous on `stream` unless noted;
+ * results written to device memory are valid once the stream is synchronised.
+ *
+ * Every function returns TC_OK (0) or a TC_ERR_* code; tc_last_error() gives
+ * the message of the last failure on the calling thread.
+ *
+ * Each declaration names the reference interface it replaces
+ * (ChitandaEruQ/triangle_counting_hll, file:line).
+ */
+#ifndef TCHLL_H
+#define TCHLL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TC_OK 0
+#define TC_ERR_INVALID 1     /* bad argument        -> Python ValueError   */
+#define TC_ERR_HIP 2         /* HIP runtime failure -> Python RuntimeError */
+#define TC_ERR_OOM 3         /* allocation failure  -> Python MemoryError  */
+#define TC_ERR_UNSUPPORTED 4 /* unsupported device  -> Python RuntimeError */
+
+#define TC_ABI_VERSION 1
+
+/* ---- library ----------------------------------------------------------- */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *tc_last_error(void);
+/* TC_ABI_VERSION of the loaded library. */
+int tc_abi_version(void);
+/* Device facts (host outputs).  Replaces the compute-capability probe of
+ * src/gpu_hybrid.py:27-39 (_warn_if_float64_atomic_slow). */
+int tc_device_info(int *n_cu, int *wave_size, int64_t *lds_per_block,
+                   int64_t *total_mem, char *name, int name_len);
+
+/* Per-kernel timing for measurement.  While enabled, the pipelines record a
+ * hipEvent pair on the call's stream around each launch of the timed stages
+ * (TC_STAGE_*).  tc_timing_read synchronises the recorded events, writes
+ * the summed milliseconds and the launch count of every stage into ms[] and
+ * launches[] (host, TC_N_STAGES entries each) and clears the log.
+ * No reference counterpart (the reference times host wall-clock around
+ * cuda.synchronize(), src/gpu_hybrid.py:346-352). */
+#define TC_STAGE_CLASSIFY 0   /* light-edge merge + center histogram  */
+#define TC_STAGE_HEAVY 1      /* LDS-hash intersection kernel          */
+#define TC_STAGE_HLL_BUILD 2  /* sketch build                          */
+#define TC_STAGE_HLL_EDGES 3  /* per-edge union estimates              */
+#define TC_N_STAGES 4
+int tc_timing_enable(int on);
+int tc_timing_read(double *ms, int64_t *launches);
+
+/* ---- (a) CSR build and orientation ------------------------------------- */
+
+/* Opaque state between the two phases of the device CSR build. */
+typedef struct tc_csr_build tc_csr_build;
+
+/* Phase 1 of the device CSR build from raw undirected pairs (int32 [m][2],
+ * ids >= 0).  Drops self-loops, dedups canonical (min,max) pairs, relabels
+ * densely by sorted unique endpoint, computes undirected degrees and orients
+ * u->v iff (deg u, u) < (deg v, v).  Host outputs: node count, oriented edge
+ * count.  Synchronises `stream`.
+ * Replaces the vectorised loader experiment/exp_gpu.py:264-389 (_load_graph)
+ * and src/graph.py:29-61 (orient_graph). */
+int tc_csr_build_begin(const int32_t *pairs, int64_t m, void *stream,
+                       tc_csr_build **out, int64_t *n_nodes, int64_t *n_edges);
+
+/* Phase 2: writes the forward CSR into caller-owned arrays:
+ *   row_ptr  [n+1]  int32 (row_ptr_64 = 0) or int64 (row_ptr_64 = 1)
+ *   col_idx  [E]    int32, rows sorted ascending
+ *   degrees  [n]    int32 forward degree
+ *   node_ids [n]    int64 original id of each dense id (sorted)
+ *   edge_u, edge_v [E] int32 COO of the CSR (u-major)
+ *   undirected_deg [n] int32 (may be NULL)
+ * Same output as src/gpu_graph.py:6-41 (to_csr_numpy) on the oriented dict. */
+int tc_csr_build_finish(tc_csr_build *b, void *row_ptr, int row_ptr_64,
+                        int32_t *col_idx, int32_t *degrees, int64_t *node_ids,
+                        int32_t *edge_u, int32_t *edge_v, int32_t *undirected_deg,
+                        void *stream);
+
+/* Releases the phase-1 state (safe on NULL). */
+void tc_csr_build_free(tc_csr_build *b);
+
+/* Degree-bucket sort: vertices grouped by ceil(log2(deg+ + 1)) (33 buckets),
+ * ascending id inside a bucket.  bucket_start [34] int64, perm [n] int32.
+ * No reference counterpart (the reference sorts edges by cost with argsort,
+ * experiment1_gpu/gpu_experiment_fixed.py:406-417). */
+int tc_degree_buckets(const void *row_ptr, int row_ptr_64, int64_t n_nodes,
+                      int64_t *bucket_start, int32_t *perm, void *stream);
+
+/* ---- (b) exact counting ------------------------------------------------- */
+
+/* Tuning knobs of the exact pipeline (0 = library default). */
+typedef struct tc_exact_params {
+  int light_max;     /* edges with deg+u + deg+v <= light_max use a lane merge */
+  int chunk;         /* others per (center, chunk) work item                   */
+  int waves_per_cu;  /* persistent waves per CU for the heavy kernel           */
+  int reserved;
+} tc_exact_params;
+
+/* count[0] += sum over the given oriented edges (edge_u[e] -> edge_v[e]) of
+ * |N+(u) ∩ N+(v)|, with N+(x) = col_idx[row_ptr[x] : row_ptr[x+1]] sorted.
+ * `count` is a device uint64.  edge_u/edge_v may both be NULL: the edge list
+ * is then the CSR's own COO.  params may be NULL.
+ * Replaces src/gpu_hybrid.py:96-130 (exact_count_kernel) and :332-354
+ * (gpu_exact_triangle_count). */
+int tc_exact_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                   int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                   int64_t n_edges, uint64_t *count, const tc_exact_params *params,
+                   void *stream);
+
+/* Per-edge supports: out[e] = |N+(edge_u[e]) ∩ N+(edge_v[e])| (int32).
+ * Replaces the per-edge records of src/exact.py:87-91. */
+int tc_edge_supports(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                     const int32_t *edge_u, const int32_t *edge_v, int64_t n_edges,
+                     int32_t *out, void *stream);
+
+/* ---- (c) HLL sketch build ----------------------------------------------- */
+
+/* out[i] = splitmix64 finaliser of x[i] (uint64).  Exposes the device hash of
+ * src/gpu_hll.py:12-17 (= src/hll.py:13-19) for testing. */
+int tc_hash64(const uint64_t *x, uint64_t *out, int64_t n, void *stream);
+
+/* Builds HLL registers of N+(u) for every u with deg+(u) >= theta and sets
+ * has_sketch[u] (uint8, every u).  Row of u: regs + slot[u] * 2^p when `slot`
+ * (int32 [n], -1 = no row) is given, else regs + u * 2^p (dense layout of
+ * src/gpu_hll.py:69).  accumulate = 1 takes the max with the bytes already in
+ * the row (the in-place semantics of src/gpu_hll.py:80-81); accumulate = 0
+ * overwrites the row.  4 <= p <= 16.
+ * Replaces src/gpu_hll.py:47-81 (build_sketches_kernel). */
+int tc_hll_build(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                 int64_t n_nodes, int64_t theta, int p, uint8_t *regs,
+                 const int32_t *slot, uint8_t *has_sketch, int accumulate,
+                 void *stream);
+
+/* ---- (d) union / intersection estimation (FP64) -------------------------- */
+
+/* out[i] = FP64 HLL estimate of max(row(us[i]), row(vs[i])) register-wise
+ * (us[i] == vs[i] gives the single-sketch estimate).  Rows addressed as in
+ * tc_hll_build.  Replaces src/gpu_hll.py:88-121 (_union_estimate_device) with
+ * the FP64 estimator of src/hll.py:37-54, 112-120. */
+int tc_union_estimate(const uint8_t *regs, const int32_t *slot, int p,
+                      const int32_t *us, const int32_t *vs, int64_t k,
+                      double *out, void *stream);
+
+/* ---- (e) hybrid router and count ----------------------------------------- */
+
+#define TC_PART_BOTH 0   /* fused: every edge, routed    (hybrid_count_kernel)      */
+#define TC_PART_EXACT 1  /* exact-routed edges only      (hybrid_exact_part_kernel) */
+#define TC_PART_HLL 2    /* HLL-routed edges only        (hybrid_hll_part_kernel)   */
+
+/* Hybrid count over the given oriented edges.  deg(x) = degrees[x] when
+ * `degrees` (int32 [n_nodes]) is given, else the CSR row length (the
+ * reference passes the forward degrees).  Edge e wants HLL iff
+ * route[e] != 0 when `route` (uint8 [n_edges]) is given, else iff
+ * min(deg u, deg v) >= gamma; it takes the HLL path iff it wants HLL and
+ * has_sketch[u] == has_sketch[v] == 1, else the exact path.
+ * HLL support: s = deg u + deg v - union_estimate, clamped to [0, min].
+ * row_ptr and col_idx may be NULL for part == TC_PART_HLL (then `degrees` is
+ * required), and regs/p are ignored for part == TC_PART_EXACT.
+ * Accumulates (device): counts[0] += FP64 sum of the supports of the edges of
+ * `part` (deterministic order), stats[0] += n_approx, stats[1] += n_exact,
+ * stats[2] += n_clamped (int64), and, when non-NULL, fallback[0] += edges
+ * that wanted HLL but lacked a sketch.  per_edge (double [n_edges], may be
+ * NULL) receives each processed edge's support.
+ * Replaces src/gpu_hybrid.py:80-89, 137-325, 357-431 and src/hybrid.py:48-166. */
+int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                    int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                    int64_t n_edges, const int32_t *degrees, const uint8_t *regs,
+                    const int32_t *slot, const uint8_t *has_sketch, int64_t gamma, int p,
+                    const uint8_t *route, int part, double *counts, int64_t *stats,
+                    int64_t *fallback, double *per_edge, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TCHLL_H */

@@ -1,0 +1,410 @@
+"""CPU oracle for the triangle-counting hot path (test infrastructure only).
+
+This module restates, in numpy and plain Python, the reference algorithms the
+MI355X path must match.  It is the *checker*: only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it.  The product path (``triangle_counting_hll_amd``) never calls it.
+
+Every function cites the reference file:line it restates.  How each function
+is pinned (golden vectors, closed forms, library routines, brute force) is
+listed in ``tests/test_oracle.py``; nothing here is "parity unpinned" except
+the bit-exact *float order* of numpy's pairwise sum inside the estimator,
+which the tests bound at 1e-12 relative instead.
+"""
+
+from __future__ import annotations
+
+import math
+from collections import defaultdict
+from typing import Dict, List, Optional, Set, Tuple
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+# ---------------------------------------------------------------------------
+# Hash and HLL register update  (src/hll.py:13-19, 81-96; src/gpu_hll.py:12-17, 47-81)
+# ---------------------------------------------------------------------------
+
+
+def hash64_int(x: int) -> int:
+    """splitmix64 finaliser on a Python int (src/hll.py:13-19)."""
+    x &= MASK64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & MASK64
+    return (x ^ (x >> 31)) & MASK64
+
+
+def hash64(x: np.ndarray) -> np.ndarray:
+    """Vectorised splitmix64 finaliser; uint64 arithmetic wraps mod 2**64."""
+    x = np.asarray(x).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return x
+
+
+def hll_index_rank(h: np.ndarray, p: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Register index (top p bits) and rank (src/hll.py:82-87).
+
+    ``rho = (64 - p) - bit_length(w) + 1`` for the low ``64 - p`` bits ``w``,
+    and ``65 - p`` when ``w == 0``.  bit_length is computed exactly with
+    integer comparisons (no float log2).
+    """
+    h = np.asarray(h, dtype=np.uint64)
+    rem = 64 - p
+    j = (h >> np.uint64(rem)).astype(np.int64)
+    w = h & np.uint64((1 << rem) - 1)
+    bl = np.zeros(w.shape, dtype=np.int64)
+    for b in range(rem):
+        bl += (w >= np.uint64(1 << b)).astype(np.int64)
+    rho = rem - bl + 1
+    return j, rho.astype(np.uint8)
+
+
+def hll_add_scalar(registers: np.ndarray, x: int, p: int) -> None:
+    """One ``HLL.add`` call on a Python int (src/hll.py:81-91)."""
+    h = hash64_int(x)
+    rem = 64 - p
+    j = h >> rem
+    w = h & ((1 << rem) - 1)
+    rho = rem + 1 if w == 0 else rem - w.bit_length() + 1
+    if rho > registers[j]:
+        registers[j] = rho
+
+
+def build_registers(row_ptr: np.ndarray, col_idx: np.ndarray, p: int,
+                    theta: int = 0) -> Tuple[np.ndarray, np.ndarray]:
+    """Dense register matrix ``uint8[n, 2**p]`` and ``has_sketch uint8[n]``.
+
+    Restates ``build_sketches_kernel`` (src/gpu_hll.py:47-81): node u gets a
+    sketch iff ``deg+(u) >= theta``; its registers are the max of the ranks of
+    the hashed dense neighbour ids ``col_idx[row_ptr[u]:row_ptr[u+1]]``.
+    Same semantics as ``build_sketches`` + ``HLL.add_array``
+    (src/hybrid.py:15-41, src/hll.py:93-96) on the dense-id graph.
+    """
+    if not (4 <= p <= 16):
+        raise ValueError(f"p must be in [4, 16], got {p}")
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    n = len(row_ptr) - 1
+    m = 1 << p
+    deg = np.diff(row_ptr)
+    has = (deg >= theta).astype(np.uint8)
+    regs = np.zeros((n, m), dtype=np.uint8)
+    owner = np.repeat(np.arange(n, dtype=np.int64), deg)
+    keep = has[owner].astype(bool)
+    j, rho = hll_index_rank(hash64(np.asarray(col_idx, dtype=np.int64)[keep]), p)
+    flat = owner[keep] * m + j
+    out = regs.reshape(-1)
+    np.maximum.at(out, flat, rho)
+    return regs, has
+
+
+# ---------------------------------------------------------------------------
+# Estimator  (src/hll.py:26-54, 112-120)
+# ---------------------------------------------------------------------------
+
+
+def alpha(m: int) -> float:
+    """Bias-correction constant (src/hll.py:26-34)."""
+    if m == 16:
+        return 0.673
+    if m == 32:
+        return 0.697
+    if m == 64:
+        return 0.709
+    return 0.7213 / (1.0 + 1.079 / m)
+
+
+def estimate_from_registers(registers: np.ndarray, alpha_m: Optional[float] = None) -> float:
+    """FP64 HLL estimate of one register row (src/hll.py:37-54)."""
+    registers = np.asarray(registers)
+    m = len(registers)
+    if alpha_m is None:
+        alpha_m = alpha(m)
+    Z = float(np.sum(np.power(2.0, -registers.astype(np.float64))))
+    E = alpha_m * m * m / Z
+    if E <= 2.5 * m:
+        V = int(np.sum(registers == 0))
+        if V > 0:
+            E = m * np.log(m / V)
+    two64 = float(1 << 64)
+    if E > two64 / 30.0:
+        E = -two64 * np.log(1.0 - E / two64)
+    return float(E)
+
+
+def estimate_rows(regs: np.ndarray) -> np.ndarray:
+    """``estimate_from_registers`` applied to each row of ``uint8[k, m]``."""
+    regs = np.asarray(regs)
+    k, m = regs.shape
+    a = alpha(m)
+    Z = np.sum(np.power(2.0, -regs.astype(np.float64)), axis=1)
+    E = a * m * m / Z
+    V = np.sum(regs == 0, axis=1)
+    small = (E <= 2.5 * m) & (V > 0)
+    with np.errstate(divide="ignore"):
+        E = np.where(small, m * np.log(m / np.maximum(V, 1)), E)
+    two64 = float(1 << 64)
+    big = E > two64 / 30.0
+    if np.any(big):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            E = np.where(big, -two64 * np.log(1.0 - E / two64), E)
+    return E
+
+
+def union_estimate_rows(regs_a: np.ndarray, regs_b: np.ndarray) -> np.ndarray:
+    """Register-wise max then estimate, row by row (src/hll.py:112-120)."""
+    return estimate_rows(np.maximum(regs_a, regs_b))
+
+
+# ---------------------------------------------------------------------------
+# Orientation and CSR  (src/graph.py:29-61; src/gpu_graph.py:6-41;
+# experiment/exp_gpu.py:291-365)
+# ---------------------------------------------------------------------------
+
+
+def orient_graph(edges: Set[Tuple[int, int]]) -> Tuple[Dict[int, np.ndarray], Dict[int, int]]:
+    """Dict-based orientation by (degree, id) (src/graph.py:29-61).  Small graphs only."""
+    degree: Dict[int, int] = defaultdict(int)
+    for u, v in edges:
+        degree[u] += 1
+        degree[v] += 1
+    fwd: Dict[int, list] = defaultdict(list)
+    nodes: Set[int] = set()
+    for u, v in edges:
+        nodes.add(u)
+        nodes.add(v)
+        if (degree[u], u) < (degree[v], v):
+            fwd[u].append(v)
+        else:
+            fwd[v].append(u)
+    return {x: np.array(sorted(fwd[x]), dtype=np.int64) for x in nodes}, dict(degree)
+
+
+def to_csr_numpy(forward_adj: Dict[int, np.ndarray]):
+    """Dense-relabelled int32 CSR from a forward-adjacency dict (src/gpu_graph.py:6-41)."""
+    node_ids = np.array(sorted(forward_adj.keys()), dtype=np.int32)
+    n = len(node_ids)
+    index = {int(x): i for i, x in enumerate(node_ids)}
+    degrees = np.array([len(forward_adj[int(x)]) for x in node_ids], dtype=np.int32)
+    row_ptr = np.zeros(n + 1, dtype=np.int32)
+    row_ptr[1:] = np.cumsum(degrees)
+    E = int(row_ptr[-1])
+    col_idx = np.empty(E, dtype=np.int32)
+    edge_u = np.empty(E, dtype=np.int32)
+    edge_v = np.empty(E, dtype=np.int32)
+    for i, x in enumerate(node_ids):
+        s = row_ptr[i]
+        for k, v in enumerate(forward_adj[int(x)]):
+            j = index[int(v)]
+            col_idx[s + k] = j
+            edge_u[s + k] = i
+            edge_v[s + k] = j
+    return row_ptr, col_idx, degrees, node_ids, edge_u, edge_v
+
+
+def csr_from_pairs(pairs: np.ndarray):
+    """Vectorised loader restating ``_load_graph`` (experiment/exp_gpu.py:291-365).
+
+    Raw pairs -> drop self-loops -> canonical (min, max) -> unique -> dense
+    relabel by sorted unique endpoint -> undirected degree -> orient by
+    (degree, id) -> forward CSR with rows sorted by (src, dst).
+
+    Returns ``(row_ptr int64[n+1], col_idx int32[E], fwd_deg int32[n],
+    node_ids int64[n], edge_u int32[E], edge_v int32[E], undirected_deg int32[n])``.
+    """
+    pairs = np.asarray(pairs)
+    a = np.minimum(pairs[:, 0], pairs[:, 1]).astype(np.int64)
+    b = np.maximum(pairs[:, 0], pairs[:, 1]).astype(np.int64)
+    keep = a != b
+    key = np.unique((a[keep] << 32) | b[keep])
+    a = key >> 32
+    b = key & 0xFFFFFFFF
+    if len(key) == 0:
+        z32 = np.zeros(0, dtype=np.int32)
+        return (np.zeros(1, dtype=np.int64), z32, z32, np.zeros(0, dtype=np.int64), z32, z32, z32)
+    node_ids = np.union1d(a, b)
+    n = len(node_ids)
+    nidx = np.full(int(node_ids[-1]) + 1, -1, dtype=np.int64)
+    nidx[node_ids] = np.arange(n)
+    ra = nidx[a]
+    rb = nidx[b]
+    deg = (np.bincount(ra, minlength=n) + np.bincount(rb, minlength=n)).astype(np.int64)
+    first = (deg[ra] < deg[rb]) | ((deg[ra] == deg[rb]) & (ra < rb))
+    src = np.where(first, ra, rb)
+    dst = np.where(first, rb, ra)
+    fdeg = np.bincount(src, minlength=n)
+    row_ptr = np.zeros(n + 1, dtype=np.int64)
+    row_ptr[1:] = np.cumsum(fdeg)
+    order = np.lexsort((dst, src))
+    col = dst[order].astype(np.int32)
+    eu = src[order].astype(np.int32)
+    return (row_ptr, col, fdeg.astype(np.int32), node_ids, eu, col.copy(), deg.astype(np.int32))
+
+
+def forward_adj_from_csr(row_ptr, col_idx) -> Dict[int, np.ndarray]:
+    """``forward_adj`` dict over dense ids (experiment/exp_gpu.py:368-377)."""
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    return {i: np.asarray(col_idx[row_ptr[i]:row_ptr[i + 1]], dtype=np.int64)
+            for i in range(len(row_ptr) - 1)}
+
+
+# ---------------------------------------------------------------------------
+# Exact counting  (src/exact.py:12-98)
+# ---------------------------------------------------------------------------
+
+
+def intersect_adaptive(a: np.ndarray, b: np.ndarray, r: float = 8.0) -> int:
+    """Merge when size ratio <= r, else binary search (src/exact.py:29-41)."""
+    la, lb = len(a), len(b)
+    if la == 0 or lb == 0:
+        return 0
+    if la > lb:
+        a, b, la, lb = b, a, lb, la
+    if lb / la <= r:
+        return int(len(np.intersect1d(a, b, assume_unique=True)))
+    idx = np.searchsorted(b, a)
+    mask = idx < lb
+    return int(np.sum(b[idx[mask]] == a[mask]))
+
+
+def exact_triangle_count(forward_adj: Dict[int, np.ndarray], r: float = 8.0) -> int:
+    """Forward-exact count over the dict (src/exact.py:48-98).  Small graphs."""
+    total = 0
+    for u, nu in forward_adj.items():
+        if len(nu) == 0:
+            continue
+        for v in nu:
+            nv = forward_adj.get(int(v))
+            if nv is None or len(nv) == 0:
+                continue
+            total += intersect_adaptive(nu, nv, r)
+    return total
+
+
+def edge_supports(row_ptr, col_idx, edge_u, edge_v) -> np.ndarray:
+    """Per-edge ``|N+(u) ∩ N+(v)|`` for the given edge list (src/exact.py:79-95).
+
+    Vectorised: every (edge, element of N+(u)) pair is tested for membership in
+    N+(v) by ``searchsorted`` on the globally sorted keys ``row * 2**32 + col``.
+    Memory is O(sum over edges of deg+(u)); use for graphs up to ~1e7 of that.
+    """
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    col = np.asarray(col_idx, dtype=np.int64)
+    eu = np.asarray(edge_u, dtype=np.int64)
+    ev = np.asarray(edge_v, dtype=np.int64)
+    n = len(row_ptr) - 1
+    deg = np.diff(row_ptr)
+    owner = np.repeat(np.arange(n, dtype=np.int64), deg)
+    keys = (owner << 32) | col            # sorted because rows are sorted
+    du = deg[eu]
+    tot = int(du.sum())
+    out = np.zeros(len(eu), dtype=np.int64)
+    if tot == 0:
+        return out
+    e_rep = np.repeat(np.arange(len(eu), dtype=np.int64), du)
+    starts = np.repeat(row_ptr[eu], du)
+    offs = np.arange(tot, dtype=np.int64) - np.repeat(np.cumsum(du) - du, du)
+    w = col[starts + offs]
+    probe = (ev[e_rep] << 32) | w
+    pos = np.searchsorted(keys, probe)
+    hit = np.zeros(tot, dtype=bool)
+    ok = pos < len(keys)
+    hit[ok] = keys[pos[ok]] == probe[ok]
+    np.add.at(out, e_rep[hit], 1)
+    return out
+
+
+def exact_count_csr(row_ptr, col_idx, edge_u=None, edge_v=None) -> int:
+    """Exact count over the CSR's COO edge list (same sum as src/exact.py:79-95)."""
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    if edge_u is None:
+        deg = np.diff(row_ptr)
+        edge_u = np.repeat(np.arange(len(deg)), deg)
+        edge_v = col_idx
+    return int(edge_supports(row_ptr, col_idx, edge_u, edge_v).sum())
+
+
+# ---------------------------------------------------------------------------
+# Hybrid  (src/hybrid.py:48-166; src/gpu_hybrid.py:80-89, 137-211, 357-431)
+# ---------------------------------------------------------------------------
+
+
+def hybrid_count(row_ptr, col_idx, edge_u, edge_v, regs: np.ndarray, has: np.ndarray,
+                 gamma: int, route: Optional[np.ndarray] = None) -> dict:
+    """Per-edge hybrid estimate on dense arrays.
+
+    Routing (src/hybrid.py:107-112, src/gpu_hybrid.py:80-89): HLL iff
+    ``min(deg+u, deg+v) >= gamma`` and both endpoints have sketches; a given
+    boolean ``route`` array (True = wants HLL) replaces the degree rule.  HLL
+    support ``deg+u + deg+v - union_estimate`` clamped to ``[0, min]``
+    (src/hybrid.py:117-125).  Returns per-edge supports, the naive running
+    total (src/hybrid.py:125/134 order), an ``fsum`` total, and the stats keys
+    of src/hybrid.py:152-164.
+    """
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    eu = np.asarray(edge_u, dtype=np.int64)
+    ev = np.asarray(edge_v, dtype=np.int64)
+    deg = np.diff(row_ptr)
+    du = deg[eu]
+    dv = deg[ev]
+    proxy = np.minimum(du, dv)
+    wants = proxy >= gamma if route is None else np.asarray(route, dtype=bool)
+    both = (has[eu] == 1) & (has[ev] == 1)
+    use = wants & both
+    sup = edge_supports(row_ptr, col_idx, eu, ev).astype(np.float64)
+    s_hat = np.zeros(len(eu))
+    clamped = np.zeros(len(eu), dtype=bool)
+    idx = np.nonzero(use)[0]
+    if len(idx):
+        est = union_estimate_rows(regs[eu[idx]], regs[ev[idx]])
+        s = du[idx] + dv[idx] - est
+        up = proxy[idx].astype(np.float64)
+        clamped[idx] = (s < 0.0) | (s > up)
+        s_hat[idx] = s
+        sup[idx] = np.maximum(0.0, np.minimum(s, up))
+    n_edges = len(eu)
+    n_approx = int(use.sum())
+    n_fallback = int((wants & ~both).sum())
+    n_clamped = int(clamped.sum())
+    naive = 0.0
+    for x in sup.tolist():
+        naive += x
+    return {
+        "supports": sup,
+        "use_hll": use,
+        "s_hat": s_hat,
+        "total_naive": naive,
+        "total_fsum": math.fsum(sup.tolist()),
+        "exact_part": int(np.sum(sup[~use])),
+        "hll_part_fsum": math.fsum(sup[use].tolist()),
+        "stats": {
+            "n_edges": n_edges,
+            "n_approx": n_approx,
+            "n_exact": n_edges - n_approx,
+            "n_fallback": n_fallback,
+            "n_clamped": n_clamped,
+            "approx_coverage": n_approx / n_edges if n_edges else 0.0,
+            "fallback_ratio": (n_fallback / (n_approx + n_fallback)
+                               if (n_approx + n_fallback) else 0.0),
+            "clamp_ratio": n_clamped / n_approx if n_approx else 0.0,
+        },
+    }
+
+
+def relative_error(estimate: float, truth: int) -> float:
+    """``|est - truth| / truth`` with the zero conventions of src/utils.py:36-43."""
+    if truth == 0:
+        return float("inf") if estimate != 0 else 0.0
+    return abs(estimate - truth) / truth
+
+
+# ---------------------------------------------------------------------------
+# Reference known-answer graph (cpu_exp/triangle_hll_experiment.py:1054-1069)
+# ---------------------------------------------------------------------------
+
+SELF_TEST_EDGES = np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3], [3, 4], [4, 5]],
+                           dtype=np.int32)
+SELF_TEST_TRIANGLES = 4

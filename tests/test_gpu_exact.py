@@ -1,0 +1,134 @@
+"""Exact-count parity on the GPU against the CPU oracle (bit-exact integers)."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import cbind
+from oracle import tc_oracle as O
+from triangle_counting_hll_amd import gpu_graph, gpu_hybrid
+from triangle_counting_hll_amd.graphgen import config_pairs, erdos_renyi, random_geometric, rmat
+
+pytestmark = pytest.mark.gpu
+
+
+def _count(csr, **kw):
+    return gpu_hybrid.gpu_exact_triangle_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
+                                               **kw)[0]
+
+
+def _truth(pairs):
+    rp, col, *_ = O.csr_from_pairs(pairs)
+    return cbind.exact_count(rp, col)
+
+
+def test_reference_known_answer(device):
+    # cpu_exp/triangle_hll_experiment.py:1054-1077: K4 + pendant path = 4
+    csr = gpu_graph.build_csr_gpu(O.SELF_TEST_EDGES)
+    assert _count(csr) == 4
+
+
+@pytest.mark.parametrize("n", [3, 4, 10, 60])
+def test_complete_graph_closed_form(device, n):
+    iu = np.triu_indices(n, 1)
+    pairs = np.stack(iu, axis=1).astype(np.int32)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    assert _count(csr) == math.comb(n, 3)
+
+
+def test_clique_beyond_lds_table(device):
+    # K_1400: forward rows up to 1399 ids exceed the per-wave LDS table, so the
+    # global binary-search path runs; C(1400, 3) triangles.
+    n = 1400
+    pairs = np.stack(np.triu_indices(n, 1), axis=1).astype(np.int32)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    assert _count(csr) == math.comb(n, 3)
+
+
+def test_rmat14_config(device):
+    pairs = config_pairs("rmat14")
+    rp, col, fdeg, nid, eu, ev, _ = O.csr_from_pairs(pairs)
+    truth = O.exact_count_csr(rp, col)
+    assert truth == cbind.exact_count(rp, col)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    assert _count(csr) == truth
+    # the reference's own int32 CSR handed in from the host
+    got, secs = gpu_hybrid.gpu_exact_triangle_count(rp.astype(np.int32), col, eu, ev)
+    assert got == truth and secs > 0
+
+
+@pytest.mark.parametrize("light_max,chunk,wpc", [(0, 1, 4), (2, 7, 8), (16, 256, 16),
+                                                 (64, 32, 32), (10**6, 256, 16)])
+def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc):
+    pairs = rmat(12, 16, seed=7)
+    truth = _truth(pairs)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    params = {"light_max": light_max, "chunk": chunk, "waves_per_cu": wpc}
+    assert _count(csr, params=params) == truth
+
+
+def test_rgg_small(device):
+    pairs = random_geometric(1 << 15, 2.8, seed=1)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    assert _count(csr) == _truth(pairs)
+
+
+def test_erdos_renyi(device):
+    pairs = erdos_renyi(5000, 8 / 5000, seed=3)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    assert _count(csr) == _truth(pairs)
+
+
+def test_chunglu_downscaled(device):
+    pairs = config_pairs("chunglu", scale_down=6)
+    csr = gpu_graph.build_csr_gpu(pairs, row_ptr_dtype="int64")
+    assert csr.row_ptr.dtype.itemsize == 8
+    assert _count(csr) == _truth(pairs)
+
+
+def test_edge_subset_and_order(device):
+    import torch
+
+    pairs = rmat(11, 8, seed=2)
+    rp, col, fdeg, nid, eu, ev, _ = O.csr_from_pairs(pairs)
+    rng = np.random.default_rng(0)
+    sel = rng.permutation(len(eu))[: len(eu) // 3]
+    truth = cbind.exact_count(rp, col, eu[sel], ev[sel])
+    got, _ = gpu_hybrid.gpu_exact_triangle_count(rp, col, eu[sel], ev[sel])
+    assert got == truth
+    # reversed edge direction gives the same per-edge intersection
+    got_rev, _ = gpu_hybrid.gpu_exact_triangle_count(rp, col, ev[sel], eu[sel])
+    assert got_rev == truth
+    # empty edge list
+    z = torch.zeros(0, dtype=torch.int32, device="cuda")
+    assert gpu_hybrid.gpu_exact_triangle_count(rp, col, z, z)[0] == 0
+
+
+def test_coo_null_edges_and_launcher(device):
+    import torch
+
+    pairs = rmat(10, 8, seed=9)
+    rp, col, fdeg, nid, eu, ev, _ = O.csr_from_pairs(pairs)
+    truth = cbind.exact_count(rp, col)
+    cnt = gpu_hybrid.exact_count(rp, col)  # edge list NULL -> CSR's own COO
+    assert int(cnt.item()) == truth
+    counts = torch.full((1,), 5, dtype=torch.int64, device="cuda")
+    gpu_hybrid.exact_count_kernel[123, 256](rp.astype(np.int32), col, eu, ev, len(eu), counts)
+    torch.cuda.synchronize()
+    assert int(counts.item()) == truth + 5  # launch objects add into counts
+
+
+def test_edge_supports_match_oracle(device):
+    pairs = rmat(11, 16, seed=4)
+    rp, col, fdeg, nid, eu, ev, _ = O.csr_from_pairs(pairs)
+    sup = gpu_hybrid.edge_supports(rp, col, eu, ev).cpu().numpy()
+    assert np.array_equal(sup, cbind.edge_supports(rp, col, eu, ev))
+    assert np.array_equal(sup, O.edge_supports(rp, col, eu, ev))
+
+
+def test_repeat_is_deterministic(device):
+    pairs = rmat(13, 8, seed=1)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    a = [_count(csr) for _ in range(3)]
+    assert len(set(a)) == 1 and a[0] == _truth(pairs)

@@ -1,0 +1,248 @@
+// Library-wide state, error reporting and device-wide scan / compaction.
+#include <cstdarg>
+#include <mutex>
+#include <vector>
+
+#include "tc_common.hpp"
+
+namespace tc {
+
+namespace {
+thread_local char g_msg[1024] = "";
+thread_local int g_code = TC_OK;
+std::mutex g_init_mu;
+bool g_inited[64] = {};
+int g_cus[64] = {};
+}  // namespace
+
+namespace {
+std::mutex g_timing_mu;
+bool g_timing = false;
+struct TimedLaunch {
+  int stage;
+  hipEvent_t a, b;
+};
+std::vector<TimedLaunch> g_timed;
+}  // namespace
+
+bool timing_on() { return g_timing; }
+
+void timing_record(int stage, hipEvent_t a, hipEvent_t b) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  g_timed.push_back({stage, a, b});
+}
+
+void set_error(int code, const char *fmt, ...) {
+  g_code = code;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
+  va_end(ap);
+}
+
+int last_code() { return g_code; }
+
+int ensure_init() {
+  int dev = 0;
+  TC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    set_error(TC_ERR_UNSUPPORTED, "device index %d out of range", dev);
+    return TC_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (g_inited[dev]) return TC_OK;
+  hipDeviceProp_t prop;
+  TC_HIP(hipGetDeviceProperties(&prop, dev));
+  TC_CHECK(prop.warpSize == kWave, TC_ERR_UNSUPPORTED,
+           "device %d has wavefront size %d; these kernels need 64", dev, prop.warpSize);
+  g_cus[dev] = prop.multiProcessorCount;
+  hipMemPool_t pool;
+  TC_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+  uint64_t keep = UINT64_MAX;
+  TC_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  g_inited[dev] = true;
+  return TC_OK;
+}
+
+int num_cus() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_cus[dev] <= 0) return 256;
+  return g_cus[dev];
+}
+
+// ---------------------------------------------------------------------------
+// Reduce-then-scan.  A tile is kScanBlock threads x kScanItems items, each
+// thread owning kScanItems consecutive elements.
+// ---------------------------------------------------------------------------
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t *total) {
+  __shared__ int64_t wsum[kScanBlock / kWave];
+  const int lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  int64_t inc = wave_inclusive_scan(v);
+  if (lane == kWave - 1) wsum[w] = inc;
+  __syncthreads();
+  int64_t base = 0, all = 0;
+#pragma unroll
+  for (int i = 0; i < kScanBlock / kWave; ++i) {
+    if (i < w) base += wsum[i];
+    all += wsum[i];
+  }
+  __syncthreads();
+  if (total) *total = all;
+  return base + inc - v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kScanBlock)
+k_tile_sums(const T *__restrict__ in, int64_t n, int64_t *__restrict__ sums) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) s += (int64_t)in[base + i];
+  int64_t tot;
+  (void)block_exclusive_scan(s, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kScanBlock)
+k_tile_scan(const T *__restrict__ in, int64_t n, const int64_t *__restrict__ offs,
+            int64_t *__restrict__ out) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int64_t vals[kScanItems];
+  int64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    vals[i] = (base + i < n) ? (int64_t)in[base + i] : 0;
+    s += vals[i];
+  }
+  int64_t tot;
+  int64_t run = block_exclusive_scan(s, &tot) + (offs ? offs[blockIdx.x] : 0);
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) out[base + i] = run;
+    run += vals[i];
+  }
+  // out has n + 1 entries: the grand total goes to out[n].
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanBlock - 1) out[n] = run;
+}
+
+template <class T>
+int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t * /*unused*/, hipStream_t s) {
+  if (n <= 0) {
+    TC_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), s));
+    return TC_OK;
+  }
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles == 1) {
+    hipLaunchKernelGGL(k_tile_scan<T>, dim3(1), dim3(kScanBlock), 0, s, in, n,
+                       (const int64_t *)nullptr, out);
+    TC_HIP(hipGetLastError());
+    return TC_OK;
+  }
+  Scratch sums, offs;
+  TC_TRY(sums.alloc(sizeof(int64_t) * tiles, s));
+  TC_TRY(offs.alloc(sizeof(int64_t) * (tiles + 1), s));
+  hipLaunchKernelGGL(k_tile_sums<T>, dim3((unsigned)tiles), dim3(kScanBlock), 0, s, in, n,
+                     sums.as<int64_t>());
+  TC_HIP(hipGetLastError());
+  TC_TRY(exclusive_scan<int64_t>(sums.as<int64_t>(), offs.as<int64_t>(), tiles, nullptr, s));
+  hipLaunchKernelGGL(k_tile_scan<T>, dim3((unsigned)tiles), dim3(kScanBlock), 0, s, in, n,
+                     (const int64_t *)offs.as<int64_t>(), out);
+  TC_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+template int exclusive_scan<int32_t>(const int32_t *, int64_t *, int64_t, int64_t *, hipStream_t);
+template int exclusive_scan<int64_t>(const int64_t *, int64_t *, int64_t, int64_t *, hipStream_t);
+template int exclusive_scan<uint8_t>(const uint8_t *, int64_t *, int64_t, int64_t *, hipStream_t);
+template int exclusive_scan<uint32_t>(const uint32_t *, int64_t *, int64_t, int64_t *, hipStream_t);
+
+__global__ void k_scatter_flagged(const uint8_t *__restrict__ flag, const int64_t *__restrict__ pos,
+                                  int64_t n, int32_t *__restrict__ idx) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (flag[i]) idx[pos[i]] = (int32_t)i;
+}
+
+__global__ void k_copy_i64(const int64_t *src, int64_t *dst) { *dst = *src; }
+
+int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d_count,
+                    hipStream_t s) {
+  Scratch pos;
+  TC_TRY(pos.alloc(sizeof(int64_t) * (n + 1), s));
+  TC_TRY(exclusive_scan<uint8_t>(flag, pos.as<int64_t>(), n, nullptr, s));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter_flagged, dim3(grid_for(n)), dim3(kBlock), 0, s, flag,
+                       pos.as<int64_t>(), n, idx_out);
+    TC_HIP(hipGetLastError());
+  }
+  if (d_count) {
+    hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(1), 0, s, pos.as<int64_t>() + n, d_count);
+    TC_HIP(hipGetLastError());
+  }
+  return TC_OK;
+}
+
+}  // namespace tc
+
+// ---------------------------------------------------------------------------
+// extern "C" library entry points
+// ---------------------------------------------------------------------------
+extern "C" const char *tc_last_error(void) { return tc::g_msg; }
+
+extern "C" int tc_abi_version(void) { return TC_ABI_VERSION; }
+
+extern "C" int tc_timing_enable(int on) {
+  std::lock_guard<std::mutex> lk(tc::g_timing_mu);
+  tc::g_timing = on != 0;
+  return TC_OK;
+}
+
+extern "C" int tc_timing_read(double *ms, int64_t *launches) {
+  std::vector<tc::TimedLaunch> log;
+  {
+    std::lock_guard<std::mutex> lk(tc::g_timing_mu);
+    log.swap(tc::g_timed);
+  }
+  for (int i = 0; i < TC_N_STAGES; ++i) {
+    if (ms) ms[i] = 0.0;
+    if (launches) launches[i] = 0;
+  }
+  int rc = TC_OK;
+  for (auto &t : log) {
+    float x = 0.f;
+    if (hipEventSynchronize(t.b) != hipSuccess || hipEventElapsedTime(&x, t.a, t.b) != hipSuccess) {
+      tc::set_error(TC_ERR_HIP, "event timing failed");
+      rc = TC_ERR_HIP;
+    } else if (t.stage >= 0 && t.stage < TC_N_STAGES) {
+      if (ms) ms[t.stage] += x;
+      if (launches) launches[t.stage] += 1;
+    }
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  return rc;
+}
+
+extern "C" int tc_device_info(int *n_cu, int *wave_size, int64_t *lds_per_block,
+                              int64_t *total_mem, char *name, int name_len) {
+  TC_TRY(tc::ensure_init());
+  int dev = 0;
+  TC_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  TC_HIP(hipGetDeviceProperties(&prop, dev));
+  if (n_cu) *n_cu = prop.multiProcessorCount;
+  if (wave_size) *wave_size = prop.warpSize;
+  if (lds_per_block) *lds_per_block = (int64_t)prop.sharedMemPerBlock;
+  if (total_mem) *total_mem = (int64_t)prop.totalGlobalMem;
+  if (name && name_len > 0) {
+    snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  return TC_OK;
+}

@@ -1,0 +1,172 @@
+// Shared helpers for the gfx950 triangle-counting kernels: status handling,
+// wave64 primitives, stream-ordered scratch allocation and a device-wide scan.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "tchll.h"
+
+namespace tc {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kBlock = 256;        // default workgroup size (4 wavefronts)
+
+// ---------------------------------------------------------------------------
+// Status handling: every extern "C" entry point returns a TC_* code and leaves
+// a message retrievable through tc_last_error().
+// ---------------------------------------------------------------------------
+void set_error(int code, const char *fmt, ...);
+int last_code();
+
+#define TC_HIP(expr)                                                            \
+  do {                                                                          \
+    hipError_t _e = (expr);                                                     \
+    if (_e != hipSuccess) {                                                     \
+      ::tc::set_error(TC_ERR_HIP, "%s failed: %s (%s:%d)", #expr,              \
+                      hipGetErrorString(_e), __FILE__, __LINE__);               \
+      return TC_ERR_HIP;                                                        \
+    }                                                                           \
+  } while (0)
+
+#define TC_CHECK(cond, code, ...)                                               \
+  do {                                                                          \
+    if (!(cond)) {                                                              \
+      ::tc::set_error((code), __VA_ARGS__);                                     \
+      return (code);                                                            \
+    }                                                                           \
+  } while (0)
+
+#define TC_TRY(expr)                                                            \
+  do {                                                                          \
+    int _rc = (expr);                                                           \
+    if (_rc != TC_OK) return _rc;                                               \
+  } while (0)
+
+// Stream-ordered scratch buffer from the device's default memory pool.  The
+// pool keeps freed blocks (release threshold raised at init), so repeated
+// calls do not go back to the driver.
+struct Scratch {
+  void *ptr = nullptr;
+  hipStream_t stream = nullptr;
+  Scratch() = default;
+  Scratch(const Scratch &) = delete;
+  Scratch &operator=(const Scratch &) = delete;
+  ~Scratch() {
+    if (ptr) (void)hipFreeAsync(ptr, stream);
+  }
+  int alloc(size_t bytes, hipStream_t s) {
+    stream = s;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMallocAsync(&ptr, bytes, s);
+    if (e != hipSuccess) {
+      ptr = nullptr;
+      set_error(e == hipErrorOutOfMemory ? TC_ERR_OOM : TC_ERR_HIP,
+                "hipMallocAsync(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+      return e == hipErrorOutOfMemory ? TC_ERR_OOM : TC_ERR_HIP;
+    }
+    return TC_OK;
+  }
+  template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Stage timing (see tc_timing_enable): RAII pair of events around a launch.
+bool timing_on();
+void timing_record(int stage, hipEvent_t a, hipEvent_t b);
+struct StageTimer {
+  int stage;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  StageTimer(int st, hipStream_t str) : stage(st), s(str) {
+    if (timing_on() && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+      (void)hipEventRecord(a, s);
+    else
+      a = b = nullptr;
+  }
+  ~StageTimer() {
+    if (a && b) {
+      (void)hipEventRecord(b, s);
+      timing_record(stage, a, b);
+    }
+  }
+};
+
+int ensure_init();   // once per process: device pool threshold, device facts
+int num_cus();       // compute units of the current device
+
+inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 primitives
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Completes this wavefront's outstanding LDS traffic and keeps the compiler
+// from moving memory operations across the point.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <class T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+template <class T> __device__ __forceinline__ T wave_inclusive_scan(T v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    T t = __shfl_up(v, off, kWave);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+template <class T> __device__ __forceinline__ T wave_broadcast(T v, int src) {
+  return __shfl(v, src, kWave);
+}
+
+// Row pointer accessor for int32 or int64 CSR offsets.
+template <class I> struct Rows {
+  const I *p;
+  __device__ __forceinline__ int64_t begin(int64_t u) const { return (int64_t)p[u]; }
+  __device__ __forceinline__ int64_t end(int64_t u) const { return (int64_t)p[u + 1]; }
+  __device__ __forceinline__ int64_t deg(int64_t u) const { return (int64_t)p[u + 1] - (int64_t)p[u]; }
+};
+
+// Dispatch a functor templated on the row-pointer width.
+#define TC_DISPATCH_ROWS(is64, ROWT, ...)                                        \
+  do {                                                                          \
+    if (is64) {                                                                 \
+      using ROWT = int64_t;                                                     \
+      __VA_ARGS__;                                                              \
+    } else {                                                                    \
+      using ROWT = int32_t;                                                     \
+      __VA_ARGS__;                                                              \
+    }                                                                           \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan (reduce-then-scan), int64 output.
+// out[i] = sum_{j<i} in[i]; *total (device, optional) = sum of all.
+// ---------------------------------------------------------------------------
+template <class T>
+int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t *d_total, hipStream_t s);
+
+// Stream compaction by flag: idx_out receives the indices i with flag[i] != 0 in
+// increasing order; *d_count (device) receives their number.
+int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d_count,
+                    hipStream_t s);
+
+}  // namespace tc

@@ -1,0 +1,557 @@
+// (a) Device CSR build with (degree, id) orientation, and the degree-bucket sort.
+//
+// Pipeline (all on the device, no sort of the full edge list):
+//   canonical (min,max) keys -> hash-set dedup (64-bit CAS, open addressing)
+//   -> presence flags + scan = dense relabel by sorted unique endpoint
+//   -> undirected degrees -> orientation -> forward degrees -> scan = row_ptr
+//   -> scatter into rows -> per-row sort (wave bitonic / LDS bitonic / global
+//   bitonic by row length).
+// The output equals the lexsort-based loader of experiment/exp_gpu.py:301-365.
+#include <algorithm>
+#include <climits>
+#include <vector>
+
+#include "tc_common.hpp"
+
+namespace tc {
+
+constexpr uint64_t kEmptyKey = ~0ull;
+
+__device__ __forceinline__ uint64_t key_hash(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+// Canonical keys, self-loops dropped, then inserted into the hash set.  A key
+// seen for the first time is appended to `uniq` through an atomic cursor.
+__global__ void k_dedup_insert(const int32_t *__restrict__ pairs, int64_t m,
+                               unsigned long long *__restrict__ table, uint64_t tmask,
+                               uint64_t *__restrict__ uniq,
+                               unsigned long long *__restrict__ n_uniq,
+                               int *__restrict__ max_id, int *__restrict__ bad) {
+  int local_max = -1;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t x = pairs[2 * i], y = pairs[2 * i + 1];
+    if (x < 0 || y < 0) {
+      atomicOr(bad, 1);
+      continue;
+    }
+    if (x == y) continue;
+    uint32_t a = (uint32_t)min(x, y), b = (uint32_t)max(x, y);
+    local_max = max(local_max, (int)b);
+    uint64_t key = ((uint64_t)a << 32) | b;
+    uint64_t slot = key_hash(key) & tmask;
+    while (true) {
+      unsigned long long prev = atomicCAS(&table[slot], (unsigned long long)kEmptyKey,
+                                          (unsigned long long)key);
+      if (prev == kEmptyKey) {
+        uniq[atomicAdd(n_uniq, 1ull)] = key;
+        break;
+      }
+      if (prev == key) break;
+      slot = (slot + 1) & tmask;
+    }
+  }
+  // one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) local_max = max(local_max, __shfl_xor(local_max, off));
+  if (lane_id() == 0 && local_max >= 0) atomicMax(max_id, local_max);
+}
+
+__global__ void k_mark_present(const uint64_t *__restrict__ uniq, int64_t nu,
+                               uint8_t *__restrict__ present) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nu;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t k = uniq[i];
+    present[k >> 32] = 1;
+    present[k & 0xffffffffu] = 1;
+  }
+}
+
+// Relabel to dense ids in place (key -> dense a<<32 | dense b) and count
+// undirected degrees.
+__global__ void k_relabel_degree(uint64_t *__restrict__ uniq, int64_t nu,
+                                 const int64_t *__restrict__ nidx, int *__restrict__ udeg) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nu;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t k = uniq[i];
+    uint32_t a = (uint32_t)nidx[k >> 32], b = (uint32_t)nidx[k & 0xffffffffu];
+    uniq[i] = ((uint64_t)a << 32) | b;
+    atomicAdd(&udeg[a], 1);
+    atomicAdd(&udeg[b], 1);
+  }
+}
+
+// u -> v iff (deg u, u) < (deg v, v); keys become (src << 32 | dst).
+__global__ void k_orient(uint64_t *__restrict__ uniq, int64_t nu, const int *__restrict__ udeg,
+                         int *__restrict__ fdeg) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nu;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t k = uniq[i];
+    uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)(k & 0xffffffffu);
+    int da = udeg[a], db = udeg[b];
+    bool a_first = (da < db) || (da == db && a < b);
+    uint32_t s = a_first ? a : b, d = a_first ? b : a;
+    uniq[i] = ((uint64_t)s << 32) | d;
+    atomicAdd(&fdeg[s], 1);
+  }
+}
+
+__global__ void k_scatter_rows(const uint64_t *__restrict__ keys, int64_t nu,
+                               const int64_t *__restrict__ row_ptr, int *__restrict__ cursor,
+                               int32_t *__restrict__ col, int32_t *__restrict__ edge_u) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nu;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    uint32_t s = (uint32_t)(k >> 32), d = (uint32_t)(k & 0xffffffffu);
+    int64_t pos = row_ptr[s] + atomicAdd(&cursor[s], 1);
+    col[pos] = (int32_t)d;
+    if (edge_u) edge_u[pos] = (int32_t)s;
+  }
+}
+
+// Rows of length <= 64: one wavefront per row, bitonic sort across lanes.
+__global__ void k_sort_rows_wave(const int64_t *__restrict__ row_ptr, int64_t n,
+                                 int32_t *__restrict__ col) {
+  const int lane = lane_id();
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  for (int64_t r = wid; r < n; r += nw) {
+    const int64_t b = row_ptr[r], len = row_ptr[r + 1] - b;
+    if (len <= 1 || len > kWave) continue;
+    int v = lane < len ? col[b + lane] : INT_MAX;
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        int o = __shfl_xor(v, j);
+        bool up = ((lane & k) == 0);
+        bool lower = ((lane & j) == 0);
+        // keep min on the lower lane of an ascending pair, max on the upper
+        int lo = min(v, o), hi = max(v, o);
+        v = (lower == up) ? lo : hi;
+      }
+    }
+    if (lane < len) col[b + lane] = v;
+  }
+}
+
+// Rows of length in (64, kLdsSortMax]: one workgroup per row, bitonic in LDS.
+constexpr int kLdsSortMax = 8192;
+
+__global__ void __launch_bounds__(kBlock)
+k_sort_rows_lds(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ rows,
+                int64_t nrows, int32_t *__restrict__ col) {
+  __shared__ int buf[kLdsSortMax];
+  for (int64_t ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
+    const int64_t r = rows[ri];
+    const int64_t b = row_ptr[r];
+    const int len = (int)(row_ptr[r + 1] - b);
+    int P = 1;
+    while (P < len) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) buf[i] = i < len ? col[b + i] : INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+          int ix = i ^ j;
+          if (ix > i) {
+            int a = buf[i], c = buf[ix];
+            bool up = ((i & k) == 0);
+            if ((a > c) == up) {
+              buf[i] = c;
+              buf[ix] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < len; i += blockDim.x) col[b + i] = buf[i];
+    __syncthreads();
+  }
+}
+
+// Rows longer than kLdsSortMax: one workgroup per row, bitonic in a global
+// scratch buffer of P (power of two >= len) ints per row slot.
+__global__ void __launch_bounds__(kBlock)
+k_sort_rows_global(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ rows,
+                   int64_t nrows, int32_t *__restrict__ col, int *__restrict__ scratch,
+                   int64_t scratch_stride) {
+  int *buf = scratch + (int64_t)blockIdx.x * scratch_stride;
+  for (int64_t ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
+    const int64_t r = rows[ri];
+    const int64_t b = row_ptr[r];
+    const int64_t len = row_ptr[r + 1] - b;
+    int64_t P = 1;
+    while (P < len) P <<= 1;
+    for (int64_t i = threadIdx.x; i < P; i += blockDim.x) buf[i] = i < len ? col[b + i] : INT_MAX;
+    __syncthreads();
+    for (int64_t k = 2; k <= P; k <<= 1) {
+      for (int64_t j = k >> 1; j > 0; j >>= 1) {
+        for (int64_t i = threadIdx.x; i < P; i += blockDim.x) {
+          int64_t ix = i ^ j;
+          if (ix > i) {
+            int a = buf[i], c = buf[ix];
+            bool up = ((i & k) == 0);
+            if ((a > c) == up) {
+              buf[i] = c;
+              buf[ix] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int64_t i = threadIdx.x; i < len; i += blockDim.x) col[b + i] = buf[i];
+    __syncthreads();
+  }
+}
+
+__global__ void k_row_class(const int64_t *__restrict__ row_ptr, int64_t n, int64_t lo,
+                            int64_t hi, uint8_t *__restrict__ flag) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t len = row_ptr[i + 1] - row_ptr[i];
+    flag[i] = (len > lo && len <= hi) ? 1 : 0;
+  }
+}
+
+__global__ void k_finish_arrays(const int64_t *__restrict__ row_ptr64, int64_t n,
+                                const int32_t *__restrict__ node_list,
+                                const int *__restrict__ fdeg, const int *__restrict__ udeg,
+                                void *__restrict__ row_ptr_out, int row_ptr_64,
+                                int32_t *__restrict__ degrees, int64_t *__restrict__ node_ids,
+                                int32_t *__restrict__ udeg_out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (row_ptr_out) {
+      if (row_ptr_64)
+        ((int64_t *)row_ptr_out)[i] = row_ptr64[i];
+      else
+        ((int32_t *)row_ptr_out)[i] = (int32_t)row_ptr64[i];
+    }
+    if (i < n) {
+      if (degrees) degrees[i] = fdeg[i];
+      if (node_ids) node_ids[i] = node_list[i];
+      if (udeg_out) udeg_out[i] = udeg[i];
+    }
+  }
+}
+
+int sort_rows(const int64_t *row_ptr, int64_t n, int32_t *col, hipStream_t s) {
+  if (n <= 0) return TC_OK;
+  const unsigned g = grid_for(n * kWave, kBlock, 65536);
+  hipLaunchKernelGGL(k_sort_rows_wave, dim3(g), dim3(kBlock), 0, s, row_ptr, n, col);
+  TC_HIP(hipGetLastError());
+
+  Scratch flag, rows, cnt;
+  TC_TRY(flag.alloc(n, s));
+  TC_TRY(rows.alloc(sizeof(int32_t) * n, s));
+  TC_TRY(cnt.alloc(sizeof(int64_t), s));
+  int64_t h_cnt = 0;
+
+  // LDS tier
+  hipLaunchKernelGGL(k_row_class, dim3(grid_for(n)), dim3(kBlock), 0, s, row_ptr, n,
+                     (int64_t)kWave, (int64_t)kLdsSortMax, flag.as<uint8_t>());
+  TC_HIP(hipGetLastError());
+  TC_TRY(compact_indices(flag.as<uint8_t>(), n, rows.as<int32_t>(), cnt.as<int64_t>(), s));
+  TC_HIP(hipMemcpyAsync(&h_cnt, cnt.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  TC_HIP(hipStreamSynchronize(s));
+  if (h_cnt > 0) {
+    hipLaunchKernelGGL(k_sort_rows_lds, dim3((unsigned)std::min<int64_t>(h_cnt, 65536)),
+                       dim3(kBlock), 0, s, row_ptr, (const int32_t *)rows.as<int32_t>(), h_cnt,
+                       col);
+    TC_HIP(hipGetLastError());
+  }
+
+  // global tier
+  hipLaunchKernelGGL(k_row_class, dim3(grid_for(n)), dim3(kBlock), 0, s, row_ptr, n,
+                     (int64_t)kLdsSortMax, (int64_t)INT64_MAX, flag.as<uint8_t>());
+  TC_HIP(hipGetLastError());
+  TC_TRY(compact_indices(flag.as<uint8_t>(), n, rows.as<int32_t>(), cnt.as<int64_t>(), s));
+  TC_HIP(hipMemcpyAsync(&h_cnt, cnt.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  TC_HIP(hipStreamSynchronize(s));
+  if (h_cnt > 0) {
+    int64_t maxlen = 0;
+    // longest row bounds the scratch stride: read row lengths of the class on host
+    std::vector<int32_t> ids(h_cnt);
+    TC_HIP(hipMemcpy(ids.data(), rows.ptr, sizeof(int32_t) * h_cnt, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < h_cnt; ++i) {
+      int64_t be[2];
+      TC_HIP(hipMemcpy(be, row_ptr + ids[i], 2 * sizeof(int64_t), hipMemcpyDeviceToHost));
+      maxlen = std::max(maxlen, be[1] - be[0]);
+    }
+    int64_t P = 1;
+    while (P < maxlen) P <<= 1;
+    const int64_t blocks = std::min<int64_t>(h_cnt, 64);
+    Scratch buf;
+    TC_TRY(buf.alloc(sizeof(int) * P * blocks, s));
+    hipLaunchKernelGGL(k_sort_rows_global, dim3((unsigned)blocks), dim3(kBlock), 0, s, row_ptr,
+                       (const int32_t *)rows.as<int32_t>(), h_cnt, col, buf.as<int>(), P);
+    TC_HIP(hipGetLastError());
+  }
+  return TC_OK;
+}
+
+}  // namespace tc
+
+// ---------------------------------------------------------------------------
+// extern "C"
+// ---------------------------------------------------------------------------
+struct tc_csr_build {
+  hipStream_t stream = nullptr;
+  int64_t n = 0, E = 0;
+  uint64_t *keys = nullptr;      // oriented (src<<32 | dst), dense ids
+  int *udeg = nullptr, *fdeg = nullptr;
+  int64_t *row_ptr64 = nullptr;  // n+1
+  int32_t *node_list = nullptr;  // n original ids
+  ~tc_csr_build() {
+    for (void *p : {(void *)keys, (void *)udeg, (void *)fdeg, (void *)row_ptr64, (void *)node_list})
+      if (p) (void)hipFree(p);
+  }
+};
+
+extern "C" void tc_csr_build_free(tc_csr_build *b) { delete b; }
+
+extern "C" int tc_csr_build_begin(const int32_t *pairs, int64_t m, void *stream,
+                                  tc_csr_build **out, int64_t *n_nodes, int64_t *n_edges) {
+  using namespace tc;
+  TC_CHECK(out && n_nodes && n_edges, TC_ERR_INVALID, "null output pointer");
+  TC_CHECK(m >= 0 && (m == 0 || pairs), TC_ERR_INVALID, "invalid pair array (m=%lld)",
+           (long long)m);
+  TC_TRY(ensure_init());
+  hipStream_t s = (hipStream_t)stream;
+  tc_csr_build *b = new tc_csr_build();
+  b->stream = s;
+  *out = nullptr;
+  auto fail = [&](int rc) {
+    delete b;
+    return rc;
+  };
+
+  uint64_t tsize = 1024;
+  while (tsize < 2 * (uint64_t)m) tsize <<= 1;
+  Scratch table, counters;
+  int rc = table.alloc(sizeof(uint64_t) * tsize, s);
+  if (rc) return fail(rc);
+  if (hipMalloc(&b->keys, sizeof(uint64_t) * std::max<int64_t>(m, 1)) != hipSuccess) {
+    set_error(TC_ERR_OOM, "cannot allocate %lld edge keys", (long long)m);
+    return fail(TC_ERR_OOM);
+  }
+  rc = counters.alloc(64, s);
+  if (rc) return fail(rc);
+  if (hipMemsetAsync(table.ptr, 0xff, sizeof(uint64_t) * tsize, s) != hipSuccess ||
+      hipMemsetAsync(counters.ptr, 0, 64, s) != hipSuccess) {
+    set_error(TC_ERR_HIP, "memset failed");
+    return fail(TC_ERR_HIP);
+  }
+  unsigned long long *n_uniq = (unsigned long long *)counters.ptr;
+  int *max_id = (int *)((char *)counters.ptr + 8);
+  int *bad = (int *)((char *)counters.ptr + 12);
+  // max_id starts at -1
+  int minus1 = -1;
+  if (hipMemcpyAsync(max_id, &minus1, sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess) {
+    set_error(TC_ERR_HIP, "memcpy failed");
+    return fail(TC_ERR_HIP);
+  }
+  if (m > 0) {
+    hipLaunchKernelGGL(k_dedup_insert, dim3(grid_for(m, kBlock, 8192)), dim3(kBlock), 0, s,
+                       pairs, m, (unsigned long long *)table.ptr, tsize - 1, b->keys, n_uniq,
+                       max_id, bad);
+    if (hipGetLastError() != hipSuccess) {
+      set_error(TC_ERR_HIP, "k_dedup_insert launch failed");
+      return fail(TC_ERR_HIP);
+    }
+  }
+  struct {
+    unsigned long long nu;
+    int max_id;
+    int bad;
+  } h;
+  if (hipMemcpyAsync(&h, counters.ptr, 16, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error(TC_ERR_HIP, "dedup readback failed");
+    return fail(TC_ERR_HIP);
+  }
+  if (h.bad) {
+    set_error(TC_ERR_INVALID, "edge list holds negative node ids");
+    return fail(TC_ERR_INVALID);
+  }
+  const int64_t nu = (int64_t)h.nu;
+  const int64_t span = (int64_t)h.max_id + 1;  // 0 when no edges survive
+  b->E = nu;
+
+  Scratch present, nidx;
+  rc = present.alloc(std::max<int64_t>(span, 1), s);
+  if (rc) return fail(rc);
+  rc = nidx.alloc(sizeof(int64_t) * (span + 1), s);
+  if (rc) return fail(rc);
+  if (span > 0) {
+    if (hipMemsetAsync(present.ptr, 0, span, s) != hipSuccess) {
+      set_error(TC_ERR_HIP, "memset failed");
+      return fail(TC_ERR_HIP);
+    }
+    hipLaunchKernelGGL(k_mark_present, dim3(grid_for(nu, kBlock, 8192)), dim3(kBlock), 0, s,
+                       (const uint64_t *)b->keys, nu, present.as<uint8_t>());
+  }
+  rc = exclusive_scan<uint8_t>(present.as<uint8_t>(), nidx.as<int64_t>(), span, nullptr, s);
+  if (rc) return fail(rc);
+  int64_t n = 0;
+  if (hipMemcpyAsync(&n, nidx.as<int64_t>() + span, sizeof(int64_t), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error(TC_ERR_HIP, "relabel readback failed");
+    return fail(TC_ERR_HIP);
+  }
+  b->n = n;
+  if (hipMalloc(&b->udeg, sizeof(int) * std::max<int64_t>(n, 1)) != hipSuccess ||
+      hipMalloc(&b->fdeg, sizeof(int) * std::max<int64_t>(n, 1)) != hipSuccess ||
+      hipMalloc(&b->row_ptr64, sizeof(int64_t) * (n + 1)) != hipSuccess ||
+      hipMalloc(&b->node_list, sizeof(int32_t) * std::max<int64_t>(n, 1)) != hipSuccess) {
+    set_error(TC_ERR_OOM, "cannot allocate CSR build state for %lld nodes", (long long)n);
+    return fail(TC_ERR_OOM);
+  }
+  if (hipMemsetAsync(b->udeg, 0, sizeof(int) * std::max<int64_t>(n, 1), s) != hipSuccess ||
+      hipMemsetAsync(b->fdeg, 0, sizeof(int) * std::max<int64_t>(n, 1), s) != hipSuccess) {
+    set_error(TC_ERR_HIP, "memset failed");
+    return fail(TC_ERR_HIP);
+  }
+  if (span > 0) {
+    rc = compact_indices(present.as<uint8_t>(), span, b->node_list, nullptr, s);
+    if (rc) return fail(rc);
+  }
+  if (nu > 0) {
+    hipLaunchKernelGGL(k_relabel_degree, dim3(grid_for(nu, kBlock, 8192)), dim3(kBlock), 0, s,
+                       b->keys, nu, (const int64_t *)nidx.as<int64_t>(), b->udeg);
+    hipLaunchKernelGGL(k_orient, dim3(grid_for(nu, kBlock, 8192)), dim3(kBlock), 0, s, b->keys,
+                       nu, (const int *)b->udeg, b->fdeg);
+  }
+  rc = exclusive_scan<int32_t>(b->fdeg, b->row_ptr64, n, nullptr, s);
+  if (rc) return fail(rc);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error(TC_ERR_HIP, "CSR build phase 1 failed: %s", hipGetErrorString(hipGetLastError()));
+    return fail(TC_ERR_HIP);
+  }
+  *n_nodes = n;
+  *n_edges = nu;
+  *out = b;
+  return TC_OK;
+}
+
+extern "C" int tc_csr_build_finish(tc_csr_build *b, void *row_ptr, int row_ptr_64,
+                                   int32_t *col_idx, int32_t *degrees, int64_t *node_ids,
+                                   int32_t *edge_u, int32_t *edge_v, int32_t *undirected_deg,
+                                   void *stream) {
+  using namespace tc;
+  TC_CHECK(b, TC_ERR_INVALID, "null CSR build state");
+  TC_CHECK(b->E == 0 || col_idx, TC_ERR_INVALID, "col_idx is required");
+  TC_CHECK(row_ptr_64 || b->E <= INT32_MAX, TC_ERR_INVALID,
+           "%lld oriented edges do not fit int32 row offsets; use int64", (long long)b->E);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = b->n, E = b->E;
+  if (E > 0) {
+    Scratch cursor;
+    TC_TRY(cursor.alloc(sizeof(int) * std::max<int64_t>(n, 1), s));
+    TC_HIP(hipMemsetAsync(cursor.ptr, 0, sizeof(int) * std::max<int64_t>(n, 1), s));
+    hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(E, kBlock, 8192)), dim3(kBlock), 0, s,
+                       (const uint64_t *)b->keys, E, (const int64_t *)b->row_ptr64,
+                       cursor.as<int>(), col_idx, edge_u);
+    TC_HIP(hipGetLastError());
+    TC_TRY(sort_rows(b->row_ptr64, n, col_idx, s));
+    if (edge_v) TC_HIP(hipMemcpyAsync(edge_v, col_idx, sizeof(int32_t) * E,
+                                      hipMemcpyDeviceToDevice, s));
+  }
+  hipLaunchKernelGGL(k_finish_arrays, dim3(grid_for(n + 1)), dim3(kBlock), 0, s,
+                     (const int64_t *)b->row_ptr64, n, (const int32_t *)b->node_list,
+                     (const int *)b->fdeg, (const int *)b->udeg, row_ptr, row_ptr_64, degrees,
+                     node_ids, undirected_deg);
+  TC_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Degree-bucket sort: bucket(u) = ceil(log2(deg+(u) + 1)), stable in u.
+// Tiles of 64 vertices (one wavefront); counts laid out bucket-major so one
+// exclusive scan yields every (bucket, tile) output offset.
+// ---------------------------------------------------------------------------
+namespace tc {
+constexpr int kBuckets = 33;
+
+template <class I>
+__device__ __forceinline__ int deg_bucket(const I *rp, int64_t u) {
+  uint64_t d = (uint64_t)((int64_t)rp[u + 1] - (int64_t)rp[u]);
+  return d == 0 ? 0 : 64 - __clzll(d);  // ceil(log2(d+1)) = bit length of d
+}
+
+template <class I>
+__global__ void k_bucket_count(const I *__restrict__ rp, int64_t n, int64_t ntiles,
+                               int32_t *__restrict__ counts) {
+  const int64_t tile = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (tile >= ntiles) return;
+  const int lane = lane_id();
+  const int64_t u = tile * kWave + lane;
+  int b = (u < n) ? deg_bucket(rp, u) : -1;
+  for (int k = 0; k < kBuckets; ++k) {
+    uint64_t m = __ballot(b == k);
+    if (lane == 0) counts[(int64_t)k * ntiles + tile] = __popcll(m);
+  }
+}
+
+template <class I>
+__global__ void k_bucket_scatter(const I *__restrict__ rp, int64_t n, int64_t ntiles,
+                                 const int64_t *__restrict__ offs, int32_t *__restrict__ perm) {
+  const int64_t tile = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (tile >= ntiles) return;
+  const int lane = lane_id();
+  const int64_t u = tile * kWave + lane;
+  const bool valid = u < n;
+  int b = valid ? deg_bucket(rp, u) : 63;
+  uint64_t peers = ~0ull;
+#pragma unroll
+  for (int bit = 0; bit < 6; ++bit) {
+    uint64_t m = __ballot((b >> bit) & 1);
+    peers &= ((b >> bit) & 1) ? m : ~m;
+  }
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  if (valid) perm[offs[(int64_t)b * ntiles + tile] + __popcll(peers & lt)] = (int32_t)u;
+}
+
+__global__ void k_bucket_starts(const int64_t *__restrict__ offs, int64_t ntiles,
+                                int64_t *__restrict__ starts) {
+  int k = threadIdx.x;
+  if (k <= kBuckets) starts[k] = offs[(int64_t)k * ntiles];
+}
+}  // namespace tc
+
+extern "C" int tc_degree_buckets(const void *row_ptr, int row_ptr_64, int64_t n_nodes,
+                                 int64_t *bucket_start, int32_t *perm, void *stream) {
+  using namespace tc;
+  TC_CHECK(row_ptr && bucket_start && perm && n_nodes >= 0, TC_ERR_INVALID,
+           "invalid arguments to tc_degree_buckets");
+  TC_TRY(ensure_init());
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t ntiles = std::max<int64_t>((n_nodes + kWave - 1) / kWave, 1);
+  Scratch counts, offs;
+  TC_TRY(counts.alloc(sizeof(int32_t) * ntiles * kBuckets, s));
+  TC_TRY(offs.alloc(sizeof(int64_t) * (ntiles * kBuckets + 1), s));
+  const unsigned g = grid_for(ntiles * kWave);
+  TC_DISPATCH_ROWS(row_ptr_64, I, {
+    hipLaunchKernelGGL(k_bucket_count<I>, dim3(g), dim3(kBlock), 0, s, (const I *)row_ptr,
+                       n_nodes, ntiles, counts.as<int32_t>());
+    TC_HIP(hipGetLastError());
+    TC_TRY(exclusive_scan<int32_t>(counts.as<int32_t>(), offs.as<int64_t>(), ntiles * kBuckets,
+                                   nullptr, s));
+    hipLaunchKernelGGL(k_bucket_scatter<I>, dim3(g), dim3(kBlock), 0, s, (const I *)row_ptr,
+                       n_nodes, ntiles, (const int64_t *)offs.as<int64_t>(), perm);
+    TC_HIP(hipGetLastError());
+  });
+  hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(64), 0, s,
+                     (const int64_t *)offs.as<int64_t>(), ntiles, bucket_start);
+  TC_HIP(hipGetLastError());
+  return TC_OK;
+}

@@ -1,0 +1,394 @@
+// (b) Exact per-edge set intersection on gfx950.
+//
+// T = sum over oriented edges (u -> v) of |N+(u) ∩ N+(v)|, both rows sorted.
+//
+// Pipeline (no host synchronisation; every buffer is stream-ordered):
+//   1. classify: an edge whose two rows together hold <= light_max ids is
+//      merged by one lane right away.  Every other edge is assigned to its
+//      "center" = the endpoint with the longer row (ties: v); the other
+//      endpoint's row will be streamed against the center's row.
+//   2. counting sort of the heavy edges by center (atomic histogram, scan,
+//      scatter) -> for each center the list of its "others".
+//   3. work items = (center, chunk of <= `chunk` others).
+//   4. persistent heavy kernel: a wavefront pulls an item, builds an LDS
+//      open-addressing hash set of N+(center) (load <= 1/2), then streams the
+//      concatenation of the others' rows with all 64 lanes (flattened across
+//      rows, so short rows do not idle lanes) and probes each id.
+// Streaming the shorter row into a hash of the longer one costs
+// sum_e min(d+u, d+v) probes, against sum_e (d+u + d+v) for a merge.
+#include <algorithm>
+
+#include "tc_common.hpp"
+#include "tc_exact.hpp"
+
+namespace tc {
+
+constexpr int kTblSlots = 2048;          // per-wave LDS hash slots (8 KiB)
+constexpr int kTblMaxDeg = kTblSlots / 2;  // centers above this use the global path
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <class I>
+__global__ void k_expand_rows(const I *__restrict__ rp, int64_t n, int32_t *__restrict__ eu) {
+  const int lane = lane_id();
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  for (int64_t u = wid; u < n; u += nw)
+    for (int64_t k = (int64_t)rp[u] + lane; k < (int64_t)rp[u + 1]; k += kWave) eu[k] = (int32_t)u;
+}
+
+__device__ __forceinline__ uint32_t merge_count(const int32_t *__restrict__ col, int64_t a0,
+                                                int64_t a1, int64_t b0, int64_t b1) {
+  uint32_t c = 0;
+  if (a0 >= a1 || b0 >= b1) return 0;
+  int x = col[a0], y = col[b0];
+  while (true) {
+    if (x == y) {
+      ++c;
+      if (++a0 >= a1 || ++b0 >= b1) break;
+      x = col[a0];
+      y = col[b0];
+    } else if (x < y) {
+      if (++a0 >= a1) break;
+      x = col[a0];
+    } else {
+      if (++b0 >= b1) break;
+      y = col[b0];
+    }
+  }
+  return c;
+}
+
+// Edge filter shared by the exact and hybrid pipelines: mask == nullptr means
+// every edge is counted; otherwise edges with mask[e] != 0 are skipped.
+template <class I>
+__global__ void __launch_bounds__(kBlock)
+k_classify(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict__ eu,
+           const int32_t *__restrict__ ev, int64_t E, const uint8_t *__restrict__ skip,
+           int light_max, int32_t *__restrict__ center_cnt,
+           const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count) {
+  uint64_t acc = 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (skip && skip[e]) continue;
+    const int32_t u = eu[e], v = ev[e];
+    const int64_t ub = R.begin(u), ue = R.end(u), vb = R.begin(v), ve = R.end(v);
+    const int64_t du = ue - ub, dv = ve - vb;
+    if (du == 0 || dv == 0) continue;
+    if (du + dv <= light_max) {
+      uint32_t c = merge_count(col, ub, ue, vb, ve);
+      if (graph_of) {
+        if (c) atomicAdd(&count[graph_of[u]], (unsigned long long)c);
+      } else {
+        acc += c;
+      }
+    } else {
+      atomicAdd(&center_cnt[dv >= du ? v : u], 1);
+    }
+  }
+  if (!graph_of) {
+    acc = wave_sum(acc);
+    if (lane_id() == 0 && acc) atomicAdd(count, (unsigned long long)acc);
+  }
+}
+
+template <class I>
+__global__ void __launch_bounds__(kBlock)
+k_scatter_others(Rows<I> R, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+                 int64_t E, const uint8_t *__restrict__ skip, int light_max,
+                 const int64_t *__restrict__ cptr, int32_t *__restrict__ cursor,
+                 int32_t *__restrict__ others) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (skip && skip[e]) continue;
+    const int32_t u = eu[e], v = ev[e];
+    const int64_t du = R.deg(u), dv = R.deg(v);
+    if (du == 0 || dv == 0 || du + dv <= light_max) continue;
+    const bool cv = dv >= du;
+    const int32_t c = cv ? v : u, o = cv ? u : v;
+    others[cptr[c] + atomicAdd(&cursor[c], 1)] = o;
+  }
+}
+
+__global__ void k_item_counts(const int32_t *__restrict__ center_cnt, int64_t n, int chunk,
+                              int32_t *__restrict__ nitems) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * blockDim.x)
+    nitems[c] = (center_cnt[c] + chunk - 1) / chunk;
+}
+
+__global__ void k_item_fill(const int32_t *__restrict__ nitems, const int64_t *__restrict__ iptr,
+                            int64_t n, int2 *__restrict__ items) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const int k = nitems[c];
+    const int64_t b = iptr[c];
+    for (int j = 0; j < k; ++j) items[b + j] = make_int2((int)c, j);
+  }
+}
+
+__device__ __forceinline__ uint32_t tbl_hash(uint32_t x, int shift) {
+  return (x * 0x9E3779B1u) >> shift;
+}
+
+template <class I>
+__global__ void __launch_bounds__(kBlock)
+k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
+        const int32_t *__restrict__ others, const int2 *__restrict__ items,
+        const int64_t *__restrict__ n_items_p, int chunk,
+        unsigned long long *__restrict__ next_item, const int32_t *__restrict__ graph_of,
+        unsigned long long *__restrict__ count) {
+  __shared__ int tbl_all[kWavesPerBlock][kTblSlots];
+  __shared__ int seg_end_all[kWavesPerBlock][kWave];
+  __shared__ int64_t seg_base_all[kWavesPerBlock][kWave];
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  int *tbl = tbl_all[w];
+  int *seg_end = seg_end_all[w];
+  int64_t *seg_base = seg_base_all[w];
+  const int64_t n_items = *n_items_p;
+  uint64_t acc_total = 0;
+
+  while (true) {
+    unsigned long long it = 0;
+    if (lane == 0) it = atomicAdd(next_item, 1ull);
+    it = __shfl(it, 0);
+    if ((int64_t)it >= n_items) break;
+    const int2 item = items[it];
+    const int32_t c = item.x;
+    const int64_t ob = cptr[c] + (int64_t)item.y * chunk;
+    const int64_t oe = min(cptr[c + 1], ob + (int64_t)chunk);
+    const int64_t cb = R.begin(c);
+    const int dc = (int)(R.end(c) - cb);
+    uint64_t acc = 0;
+
+    if (dc <= kTblMaxDeg) {
+      int logS = 6;
+      while ((1 << logS) < 2 * dc) ++logS;
+      const int S = 1 << logS, shift = 32 - logS;
+      for (int i = lane; i < S; i += kWave) tbl[i] = -1;
+      wave_lds_sync();
+      for (int i = lane; i < dc; i += kWave) {
+        const int x = col[cb + i];
+        uint32_t h = tbl_hash((uint32_t)x, shift);
+        while (atomicCAS(&tbl[h], -1, x) != -1) h = (h + 1) & (S - 1);
+      }
+      wave_lds_sync();
+      for (int64_t base = ob; base < oe; base += kWave) {
+        const int64_t k = base + lane;
+        int len = 0;
+        int64_t st = 0;
+        if (k < oe) {
+          const int32_t o = others[k];
+          st = R.begin(o);
+          len = (int)(R.end(o) - st);
+        }
+        const int inc = wave_inclusive_scan(len);
+        const int total = __shfl(inc, kWave - 1);
+        seg_end[lane] = inc;
+        seg_base[lane] = st - (int64_t)(inc - len);
+        wave_lds_sync();
+        int seg = 0;
+        for (int t = lane; t < total; t += kWave) {
+          while (seg_end[seg] <= t) ++seg;
+          const int x = col[seg_base[seg] + t];
+          uint32_t h = tbl_hash((uint32_t)x, shift);
+          while (true) {
+            const int y = tbl[h];
+            if (y == x) {
+              ++acc;
+              break;
+            }
+            if (y == -1) break;
+            h = (h + 1) & (S - 1);
+          }
+        }
+        wave_lds_sync();
+      }
+    } else {
+      // Long center row: binary search each streamed id in the global row.
+      for (int64_t base = ob; base < oe; base += kWave) {
+        const int64_t k = base + lane;
+        int len = 0;
+        int64_t st = 0;
+        if (k < oe) {
+          const int32_t o = others[k];
+          st = R.begin(o);
+          len = (int)(R.end(o) - st);
+        }
+        const int inc = wave_inclusive_scan(len);
+        const int total = __shfl(inc, kWave - 1);
+        seg_end[lane] = inc;
+        seg_base[lane] = st - (int64_t)(inc - len);
+        wave_lds_sync();
+        int seg = 0;
+        for (int t = lane; t < total; t += kWave) {
+          while (seg_end[seg] <= t) ++seg;
+          const int x = col[seg_base[seg] + t];
+          int64_t lo = cb, hi = cb + dc;
+          while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (col[mid] < x) lo = mid + 1; else hi = mid;
+          }
+          if (lo < cb + dc && col[lo] == x) ++acc;
+        }
+        wave_lds_sync();
+      }
+    }
+    if (graph_of) {
+      acc = wave_sum(acc);
+      if (lane == 0 && acc) atomicAdd(&count[graph_of[c]], (unsigned long long)acc);
+    } else {
+      acc_total += acc;
+    }
+  }
+  if (!graph_of) {
+    acc_total = wave_sum(acc_total);
+    if (lane == 0 && acc_total) atomicAdd(count, (unsigned long long)acc_total);
+  }
+}
+
+// Per-edge supports (labels for the learned router; per-edge records of the
+// reference).  One lane per edge, binary search of the shorter row in the longer.
+template <class I>
+__global__ void k_edge_supports(Rows<I> R, const int32_t *__restrict__ col,
+                                const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+                                int64_t E, int32_t *__restrict__ out) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t a0 = R.begin(eu[e]), a1 = R.end(eu[e]), b0 = R.begin(ev[e]), b1 = R.end(ev[e]);
+    if (a1 - a0 > b1 - b0) {
+      int64_t t0 = a0, t1 = a1;
+      a0 = b0; a1 = b1; b0 = t0; b1 = t1;
+    }
+    int c = 0;
+    for (int64_t i = a0; i < a1; ++i) {
+      const int x = col[i];
+      int64_t lo = b0, hi = b1;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (col[mid] < x) lo = mid + 1; else hi = mid;
+      }
+      if (lo < b1 && col[lo] == x) ++c;
+    }
+    out[e] = c;
+  }
+}
+
+int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, int64_t n,
+                     const int32_t *eu, const int32_t *ev, int64_t E, const uint8_t *skip,
+                     const int32_t *graph_of, unsigned long long *count,
+                     const tc_exact_params *params, hipStream_t s) {
+  const int light_max = (params && params->light_max > 0) ? params->light_max : 16;
+  const int chunk = (params && params->chunk > 0) ? params->chunk : 256;
+  const int wpc = (params && params->waves_per_cu > 0) ? params->waves_per_cu : 16;
+  if (E <= 0 || n <= 0) return TC_OK;
+
+  Scratch eu_buf;
+  if (!eu) {
+    TC_TRY(eu_buf.alloc(sizeof(int32_t) * E, s));
+    TC_DISPATCH_ROWS(row_ptr_64, I, {
+      hipLaunchKernelGGL(k_expand_rows<I>, dim3(grid_for(n * kWave, kBlock, 65536)),
+                         dim3(kBlock), 0, s, (const I *)row_ptr, n, eu_buf.as<int32_t>());
+    });
+    TC_HIP(hipGetLastError());
+    eu = eu_buf.as<int32_t>();
+    ev = col;
+  }
+
+  Scratch cnt, cptr, others, nitems, iptr, items, ctr;
+  TC_TRY(cnt.alloc(sizeof(int32_t) * n, s));
+  TC_TRY(cptr.alloc(sizeof(int64_t) * (n + 1), s));
+  TC_TRY(others.alloc(sizeof(int32_t) * E, s));
+  TC_TRY(nitems.alloc(sizeof(int32_t) * n, s));
+  TC_TRY(iptr.alloc(sizeof(int64_t) * (n + 1), s));
+  const int64_t max_items = n + E / chunk + 1;
+  TC_TRY(items.alloc(sizeof(int2) * max_items, s));
+  TC_TRY(ctr.alloc(sizeof(unsigned long long), s));
+
+  const unsigned gn = grid_for(n, kBlock, 16384);
+  const unsigned ge = grid_for(E, kBlock, 16384);
+  hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(kBlock), 0, s, cnt.as<int32_t>(), n, 0);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr.as<unsigned long long>(), 1, 0ull);
+  TC_DISPATCH_ROWS(row_ptr_64, I, {
+    Rows<I> R{(const I *)row_ptr};
+    {
+      StageTimer _timer(TC_STAGE_CLASSIFY, s);
+      hipLaunchKernelGGL(k_classify<I>, dim3(ge), dim3(kBlock), 0, s, R, col, eu, ev, E, skip,
+                         light_max, cnt.as<int32_t>(), graph_of, count);
+    }
+    TC_HIP(hipGetLastError());
+    TC_TRY(exclusive_scan<int32_t>(cnt.as<int32_t>(), cptr.as<int64_t>(), n, nullptr, s));
+    // reuse nitems as the scatter cursor, then as the per-center item count
+    hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(kBlock), 0, s, nitems.as<int32_t>(), n, 0);
+    hipLaunchKernelGGL(k_scatter_others<I>, dim3(ge), dim3(kBlock), 0, s, R, eu, ev, E, skip,
+                       light_max, (const int64_t *)cptr.as<int64_t>(), nitems.as<int32_t>(),
+                       others.as<int32_t>());
+    TC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_item_counts, dim3(gn), dim3(kBlock), 0, s,
+                       (const int32_t *)cnt.as<int32_t>(), n, chunk, nitems.as<int32_t>());
+    TC_TRY(exclusive_scan<int32_t>(nitems.as<int32_t>(), iptr.as<int64_t>(), n, nullptr, s));
+    hipLaunchKernelGGL(k_item_fill, dim3(gn), dim3(kBlock), 0, s,
+                       (const int32_t *)nitems.as<int32_t>(), (const int64_t *)iptr.as<int64_t>(),
+                       n, items.as<int2>());
+    TC_HIP(hipGetLastError());
+    const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
+    {
+      StageTimer _timer(TC_STAGE_HEAVY, s);
+      hipLaunchKernelGGL(k_heavy<I>, dim3(gh), dim3(kBlock), 0, s, R, col,
+                         (const int64_t *)cptr.as<int64_t>(), (const int32_t *)others.as<int32_t>(),
+                         (const int2 *)items.as<int2>(), (const int64_t *)(iptr.as<int64_t>() + n),
+                         chunk, ctr.as<unsigned long long>(), graph_of, count);
+    }
+    TC_HIP(hipGetLastError());
+  });
+  return TC_OK;
+}
+
+}  // namespace tc
+
+extern "C" int tc_exact_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                              int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                              int64_t n_edges, uint64_t *count, const tc_exact_params *params,
+                              void *stream) {
+  using namespace tc;
+  TC_CHECK(row_ptr && count, TC_ERR_INVALID, "row_ptr and count are required");
+  TC_CHECK(n_nodes >= 0 && n_edges >= 0, TC_ERR_INVALID, "negative size");
+  TC_CHECK((edge_u == nullptr) == (edge_v == nullptr), TC_ERR_INVALID,
+           "edge_u and edge_v must both be given or both be NULL");
+  TC_CHECK(n_edges == 0 || col_idx, TC_ERR_INVALID, "col_idx is required");
+  TC_CHECK(n_nodes < INT32_MAX, TC_ERR_INVALID, "node ids must fit int32");
+  TC_TRY(ensure_init());
+  return exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges,
+                          nullptr, nullptr, (unsigned long long *)count, params,
+                          (hipStream_t)stream);
+}
+
+extern "C" int tc_edge_supports(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                                const int32_t *edge_u, const int32_t *edge_v, int64_t n_edges,
+                                int32_t *out, void *stream) {
+  using namespace tc;
+  TC_CHECK(row_ptr && out && n_edges >= 0, TC_ERR_INVALID, "invalid arguments");
+  TC_CHECK(n_edges == 0 || (col_idx && edge_u && edge_v), TC_ERR_INVALID,
+           "col_idx, edge_u and edge_v are required");
+  TC_TRY(ensure_init());
+  if (n_edges == 0) return TC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  TC_DISPATCH_ROWS(row_ptr_64, I, {
+    hipLaunchKernelGGL(k_edge_supports<I>, dim3(grid_for(n_edges, kBlock, 65536)), dim3(kBlock),
+                       0, s, Rows<I>{(const I *)row_ptr}, col_idx, edge_u, edge_v, n_edges, out);
+  });
+  TC_HIP(hipGetLastError());
+  return TC_OK;
+}

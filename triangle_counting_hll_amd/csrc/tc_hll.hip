@@ -1,0 +1,490 @@
+// (c) HLL sketch build, (d) FP64 union estimation, (e) hybrid exact/HLL router.
+//
+// Register semantics follow the reference's integer arithmetic exactly:
+//   h = splitmix64 finaliser(id); j = h >> (64 - p); w = low 64 - p bits;
+//   rho = (64 - p) - bit_length(w) + 1, or 65 - p when w == 0;
+//   reg[j] = max(reg[j], rho)                       (src/hll.py:81-91)
+// The estimator is the FP64 one of src/hll.py:37-54.  Its sum
+// Z = sum_j 2^-r_j is accumulated exactly as a 128-bit fixed-point integer
+// (unit 2^-64) plus a count of zero registers, so Z carries one rounding.
+#include <algorithm>
+
+#include "tc_common.hpp"
+#include "tc_exact.hpp"
+
+namespace tc {
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ uint32_t hll_rank(uint64_t h, int p) {
+  const uint64_t w = h & ((~0ull) >> p);
+  return w == 0 ? (uint32_t)(65 - p) : (uint32_t)(__clzll((long long)w) - p + 1);
+}
+
+__global__ void k_hash64(const uint64_t *__restrict__ x, uint64_t *__restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = mix64(x[i]);
+}
+
+template <class I>
+__global__ void k_sketch_flags(Rows<I> R, int64_t n, int64_t theta, uint8_t *__restrict__ has) {
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+       u += (int64_t)gridDim.x * blockDim.x)
+    has[u] = R.deg(u) >= theta ? 1 : 0;
+}
+
+// One workgroup per sketched vertex: registers packed 4 per 32-bit LDS word,
+// byte max through compare-and-swap, then one coalesced write of the row.
+template <class I>
+__global__ void __launch_bounds__(kBlock)
+k_hll_build(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict__ list,
+            const int64_t *__restrict__ n_list_p, int p, uint8_t *__restrict__ regs,
+            const int32_t *__restrict__ slot, int accumulate) {
+  extern __shared__ uint32_t t32[];
+  const int m = 1 << p;
+  const int words = m >> 2;
+  const int64_t n_list = *n_list_p;
+  const int shift = 64 - p;
+  for (int64_t li = blockIdx.x; li < n_list; li += gridDim.x) {
+    const int32_t u = list[li];
+    const int64_t r = slot ? (int64_t)slot[u] : (int64_t)u;
+    if (r < 0) continue;
+    uint32_t *row = (uint32_t *)(regs + r * (int64_t)m);
+    for (int i = threadIdx.x; i < words; i += blockDim.x) t32[i] = accumulate ? row[i] : 0u;
+    __syncthreads();
+    const int64_t b = R.begin(u), e = R.end(u);
+    for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+      const uint64_t h = mix64((uint64_t)(int64_t)col[k]);
+      const uint32_t j = (uint32_t)(h >> shift);
+      const uint32_t rho = hll_rank(h, p);
+      uint32_t *wp = &t32[j >> 2];
+      const int sh = (j & 3) * 8;
+      uint32_t old = *wp;
+      while (((old >> sh) & 0xffu) < rho) {
+        const uint32_t nw = (old & ~(0xffu << sh)) | (rho << sh);
+        const uint32_t prev = atomicCAS(wp, old, nw);
+        if (prev == old) break;
+        old = prev;
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < words; i += blockDim.x) row[i] = t32[i];
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ double hll_alpha(int m) {
+  if (m == 16) return 0.673;
+  if (m == 32) return 0.697;
+  if (m == 64) return 0.709;
+  return 0.7213 / (1.0 + 1.079 / (double)m);
+}
+
+// Estimate from the merged register statistics (src/hll.py:37-54).
+// Z = zeros + (hi * 2^64 + lo) * 2^-64.
+__device__ __forceinline__ double hll_estimate(int m, uint64_t zeros, uint64_t hi, uint64_t lo) {
+  const double md = (double)m;
+  const double Z = (double)(zeros + hi) + (double)lo * 0x1p-64;
+  double E = hll_alpha(m) * md * md / Z;
+  if (E <= 2.5 * md && zeros > 0) E = md * log(md / (double)zeros);
+  const double two64 = 18446744073709551616.0;
+  if (E > two64 / 30.0) E = -two64 * log(1.0 - E / two64);
+  return E;
+}
+
+__device__ __forceinline__ void acc_reg(uint32_t r, uint64_t &zeros, uint64_t &hi, uint64_t &lo) {
+  if (r == 0) {
+    ++zeros;
+  } else {
+    const uint64_t t = 1ull << (64 - r);
+    lo += t;
+    hi += (lo < t);
+  }
+}
+
+// Wave-cooperative register merge: every lane sees the totals on return.
+__device__ __forceinline__ void wave_union_stats(const uint8_t *__restrict__ ra,
+                                                 const uint8_t *__restrict__ rb, int m,
+                                                 uint64_t &zeros, uint64_t &hi, uint64_t &lo) {
+  const int lane = lane_id();
+  zeros = 0;
+  hi = 0;
+  lo = 0;
+  if (m >= 16 * kWave) {
+    for (int j = lane * 16; j < m; j += 16 * kWave) {
+      const uint4 a = *(const uint4 *)(ra + j);
+      const uint4 b = *(const uint4 *)(rb + j);
+      const uint32_t aw[4] = {a.x, a.y, a.z, a.w}, bw[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int s = 0; s < 32; s += 8)
+          acc_reg(max((aw[q] >> s) & 0xffu, (bw[q] >> s) & 0xffu), zeros, hi, lo);
+    }
+  } else if (m >= 4 * kWave) {
+    for (int j = lane * 4; j < m; j += 4 * kWave) {
+      const uint32_t a = *(const uint32_t *)(ra + j), b = *(const uint32_t *)(rb + j);
+#pragma unroll
+      for (int s = 0; s < 32; s += 8) acc_reg(max((a >> s) & 0xffu, (b >> s) & 0xffu), zeros, hi, lo);
+    }
+  } else {
+    for (int j = lane; j < m; j += kWave) acc_reg(max((uint32_t)ra[j], (uint32_t)rb[j]), zeros, hi, lo);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    zeros += __shfl_xor(zeros, off);
+    const uint64_t olo = __shfl_xor(lo, off), ohi = __shfl_xor(hi, off);
+    const uint64_t nlo = lo + olo;
+    hi = hi + ohi + (nlo < lo);
+    lo = nlo;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_union_estimate(const uint8_t *__restrict__ regs, const int32_t *__restrict__ slot, int p,
+                 const int32_t *__restrict__ us, const int32_t *__restrict__ vs, int64_t k,
+                 double *__restrict__ out) {
+  const int m = 1 << p;
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  for (int64_t i = wid; i < k; i += nw) {
+    const int64_t a = slot ? slot[us[i]] : us[i];
+    const int64_t b = slot ? slot[vs[i]] : vs[i];
+    uint64_t zeros, hi, lo;
+    wave_union_stats(regs + a * m, regs + b * m, m, zeros, hi, lo);
+    if (lane_id() == 0) out[i] = hll_estimate(m, zeros, hi, lo);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hybrid
+// ---------------------------------------------------------------------------
+
+// Degree used by the router and the support formula: the caller's `degrees`
+// array when given (the reference passes forward degrees), else the row length.
+template <class I> struct Degrees {
+  Rows<I> R;
+  const int32_t *d;
+  __device__ __forceinline__ int64_t operator()(int64_t u) const {
+    return d ? (int64_t)d[u] : R.deg(u);
+  }
+};
+
+// Routing flags: use[e] = wants HLL && both sketched.  counters[0] +=
+// HLL-routed edges, counters[1] += fallbacks (wanted HLL, missing sketch).
+template <class I>
+__global__ void k_route(Degrees<I> D, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+                        int64_t E, const uint8_t *__restrict__ has, int64_t gamma,
+                        const uint8_t *__restrict__ route, uint8_t *__restrict__ use,
+                        unsigned long long *__restrict__ counters) {
+  uint64_t n_use = 0, n_fb = 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t u = eu[e], v = ev[e];
+    bool wants;
+    if (route) {
+      wants = route[e] != 0;
+    } else {
+      wants = min(D(u), D(v)) >= gamma;
+    }
+    const bool both = has[u] == 1 && has[v] == 1;
+    const bool hll = wants && both;
+    use[e] = hll ? 1 : 0;
+    n_use += hll;
+    n_fb += (wants && !both);
+  }
+  n_use = wave_sum(n_use);
+  n_fb = wave_sum(n_fb);
+  if (lane_id() == 0) {
+    if (n_use) atomicAdd(&counters[0], (unsigned long long)n_use);
+    if (n_fb) atomicAdd(&counters[1], (unsigned long long)n_fb);
+  }
+}
+
+// One wavefront per HLL-routed edge: support estimate, clamp, clamp count.
+template <class I>
+__global__ void __launch_bounds__(kBlock)
+k_hll_edges(Degrees<I> D, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+            const int32_t *__restrict__ list, const int64_t *__restrict__ n_list_p,
+            const uint8_t *__restrict__ regs, const int32_t *__restrict__ slot, int p,
+            double *__restrict__ s_out, double *__restrict__ per_edge,
+            unsigned long long *__restrict__ n_clamped) {
+  const int m = 1 << p;
+  const int64_t n_list = *n_list_p;
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  uint64_t clamped = 0;
+  for (int64_t i = wid; i < n_list; i += nw) {
+    const int32_t e = list[i];
+    const int32_t u = eu[e], v = ev[e];
+    const int64_t a = slot ? slot[u] : u;
+    const int64_t b = slot ? slot[v] : v;
+    uint64_t zeros, hi, lo;
+    wave_union_stats(regs + a * m, regs + b * m, m, zeros, hi, lo);
+    if (lane_id() == 0) {
+      const double E = hll_estimate(m, zeros, hi, lo);
+      const int64_t du = D(u), dv = D(v);
+      double s = (double)du + (double)dv - E;
+      const double upper = (double)min(du, dv);
+      if (s < 0.0) {
+        s = 0.0;
+        ++clamped;
+      } else if (s > upper) {
+        s = upper;
+        ++clamped;
+      }
+      s_out[i] = s;
+      if (per_edge) per_edge[e] = s;
+    }
+  }
+  clamped = wave_sum(clamped);
+  if (lane_id() == 0 && clamped) atomicAdd(n_clamped, (unsigned long long)clamped);
+}
+
+template <class I>
+__global__ void k_exact_per_edge(Rows<I> R, const int32_t *__restrict__ col,
+                                 const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+                                 int64_t E, const uint8_t *__restrict__ use,
+                                 double *__restrict__ per_edge) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (use && use[e]) continue;
+    int64_t a0 = R.begin(eu[e]), a1 = R.end(eu[e]), b0 = R.begin(ev[e]), b1 = R.end(ev[e]);
+    if (a1 - a0 > b1 - b0) {
+      int64_t t0 = a0, t1 = a1;
+      a0 = b0; a1 = b1; b0 = t0; b1 = t1;
+    }
+    int c = 0;
+    for (int64_t i = a0; i < a1; ++i) {
+      const int x = col[i];
+      int64_t lo = b0, hi = b1;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (col[mid] < x) lo = mid + 1; else hi = mid;
+      }
+      if (lo < b1 && col[lo] == x) ++c;
+    }
+    per_edge[e] = (double)c;
+  }
+}
+
+// Deterministic FP64 sum: fixed-size tiles summed in a fixed tree order, then
+// one workgroup folds the tile sums with Neumaier compensation.
+constexpr int kSumTile = 2048;
+
+__device__ __forceinline__ double block_sum_fixed(double v) {
+  __shared__ double buf[kBlock];
+  buf[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) buf[threadIdx.x] += buf[threadIdx.x + s];
+    __syncthreads();
+  }
+  double r = buf[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_tile_sums_f64(const double *__restrict__ x, const int64_t *__restrict__ n_p,
+                double *__restrict__ partial, int64_t max_tiles) {
+  const int64_t n = *n_p;
+  const int64_t tiles = (n + kSumTile - 1) / kSumTile;
+  for (int64_t t = blockIdx.x; t < tiles && t < max_tiles; t += gridDim.x) {
+    double v = 0.0;
+    for (int64_t i = t * kSumTile + threadIdx.x; i < min(n, (t + 1) * kSumTile); i += kBlock)
+      v += x[i];
+    v = block_sum_fixed(v);
+    if (threadIdx.x == 0) partial[t] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_hybrid_finish(const double *__restrict__ partial, const int64_t *__restrict__ n_hll_p,
+                const unsigned long long *__restrict__ exact_sum,
+                const unsigned long long *__restrict__ counters, int64_t E, int part,
+                double *__restrict__ counts, int64_t *__restrict__ stats,
+                int64_t *__restrict__ fallback) {
+  const int64_t n = *n_hll_p;
+  const int64_t tiles = (n + kSumTile - 1) / kSumTile;
+  double s = 0.0, c = 0.0;  // Neumaier
+  for (int64_t t = threadIdx.x; t < tiles; t += kBlock) {
+    const double x = partial[t];
+    const double y = s + x;
+    c += (fabs(s) >= fabs(x)) ? (s - y) + x : (x - y) + s;
+    s = y;
+  }
+  __shared__ double bs[kBlock], bc[kBlock];
+  bs[threadIdx.x] = s;
+  bc[threadIdx.x] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double S = 0.0, C = 0.0;
+    for (int i = 0; i < kBlock; ++i) {
+      const double x = bs[i];
+      const double y = S + x;
+      C += (fabs(S) >= fabs(x)) ? (S - y) + x : (x - y) + S;
+      S = y;
+      C += bc[i];
+    }
+    const double hll_sum = S + C;
+    const unsigned long long n_hll = counters[0];
+    double add = 0.0;
+    if (part != TC_PART_HLL) add += (double)exact_sum[0];
+    if (part != TC_PART_EXACT) add += hll_sum;
+    counts[0] += add;
+    if (part != TC_PART_EXACT) {
+      stats[0] += (int64_t)n_hll;
+      stats[2] += (int64_t)counters[2];
+    }
+    if (part != TC_PART_HLL) stats[1] += E - (int64_t)n_hll;
+    if (fallback) fallback[0] += (int64_t)counters[1];
+  }
+}
+
+}  // namespace tc
+
+// ---------------------------------------------------------------------------
+// extern "C"
+// ---------------------------------------------------------------------------
+extern "C" int tc_hash64(const uint64_t *x, uint64_t *out, int64_t n, void *stream) {
+  using namespace tc;
+  TC_CHECK(n >= 0 && (n == 0 || (x && out)), TC_ERR_INVALID, "invalid arguments to tc_hash64");
+  TC_TRY(ensure_init());
+  if (n == 0) return TC_OK;
+  hipLaunchKernelGGL(k_hash64, dim3(grid_for(n, kBlock, 65536)), dim3(kBlock), 0,
+                     (hipStream_t)stream, x, out, n);
+  TC_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+extern "C" int tc_hll_build(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                            int64_t n_nodes, int64_t theta, int p, uint8_t *regs,
+                            const int32_t *slot, uint8_t *has_sketch, int accumulate,
+                            void *stream) {
+  using namespace tc;
+  TC_CHECK(p >= 4 && p <= 16, TC_ERR_INVALID, "p must be in [4, 16], got %d", p);
+  TC_CHECK(row_ptr && has_sketch && n_nodes >= 0, TC_ERR_INVALID,
+           "row_ptr and has_sketch are required");
+  TC_CHECK(n_nodes == 0 || regs, TC_ERR_INVALID, "regs is required");
+  TC_CHECK(((uintptr_t)regs & 3) == 0, TC_ERR_INVALID, "regs must be 4-byte aligned");
+  TC_TRY(ensure_init());
+  if (n_nodes == 0) return TC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Scratch list, cnt;
+  TC_TRY(list.alloc(sizeof(int32_t) * n_nodes, s));
+  TC_TRY(cnt.alloc(sizeof(int64_t), s));
+  TC_DISPATCH_ROWS(row_ptr_64, I, {
+    Rows<I> R{(const I *)row_ptr};
+    hipLaunchKernelGGL(k_sketch_flags<I>, dim3(grid_for(n_nodes, kBlock, 16384)), dim3(kBlock),
+                       0, s, R, n_nodes, theta, has_sketch);
+    TC_HIP(hipGetLastError());
+    TC_TRY(compact_indices(has_sketch, n_nodes, list.as<int32_t>(), cnt.as<int64_t>(), s));
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(n_nodes, num_cus() * 8));
+    {
+      StageTimer _timer(TC_STAGE_HLL_BUILD, s);
+      hipLaunchKernelGGL(k_hll_build<I>, dim3(g), dim3(kBlock), (size_t)1 << p, s, R, col_idx,
+                         (const int32_t *)list.as<int32_t>(), (const int64_t *)cnt.as<int64_t>(), p,
+                         regs, slot, accumulate);
+    }
+    TC_HIP(hipGetLastError());
+  });
+  return TC_OK;
+}
+
+extern "C" int tc_union_estimate(const uint8_t *regs, const int32_t *slot, int p,
+                                 const int32_t *us, const int32_t *vs, int64_t k, double *out,
+                                 void *stream) {
+  using namespace tc;
+  TC_CHECK(p >= 4 && p <= 16, TC_ERR_INVALID, "p must be in [4, 16], got %d", p);
+  TC_CHECK(k >= 0 && (k == 0 || (regs && us && vs && out)), TC_ERR_INVALID,
+           "invalid arguments to tc_union_estimate");
+  TC_CHECK(((uintptr_t)regs & 15) == 0, TC_ERR_INVALID, "regs must be 16-byte aligned");
+  TC_TRY(ensure_init());
+  if (k == 0) return TC_OK;
+  hipLaunchKernelGGL(k_union_estimate, dim3(grid_for(k * kWave, kBlock, 65536)), dim3(kBlock), 0,
+                     (hipStream_t)stream, regs, slot, p, us, vs, k, out);
+  TC_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                               int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                               int64_t n_edges, const int32_t *degrees, const uint8_t *regs,
+                               const int32_t *slot, const uint8_t *has_sketch, int64_t gamma, int p,
+                               const uint8_t *route, int part, double *counts, int64_t *stats,
+                               int64_t *fallback, double *per_edge, void *stream) {
+  using namespace tc;
+  TC_CHECK(part == TC_PART_BOTH || part == TC_PART_EXACT || part == TC_PART_HLL, TC_ERR_INVALID,
+           "unknown part %d", part);
+  TC_CHECK(part == TC_PART_EXACT || (p >= 4 && p <= 16), TC_ERR_INVALID,
+           "p must be in [4, 16], got %d", p);
+  TC_CHECK(counts && stats && has_sketch && n_edges >= 0 && n_nodes >= 0, TC_ERR_INVALID,
+           "invalid arguments to tc_hybrid_count");
+  TC_CHECK(n_edges == 0 || (edge_u && edge_v), TC_ERR_INVALID, "edge_u and edge_v are required");
+  TC_CHECK(part == TC_PART_HLL || n_edges == 0 || (row_ptr && col_idx), TC_ERR_INVALID,
+           "row_ptr and col_idx are required for the exact part");
+  TC_CHECK(row_ptr || degrees, TC_ERR_INVALID, "row_ptr or degrees is required");
+  TC_CHECK(part == TC_PART_EXACT || n_edges == 0 || regs, TC_ERR_INVALID,
+           "sketch registers are required for the HLL part");
+  TC_CHECK(((uintptr_t)regs & 15) == 0, TC_ERR_INVALID, "regs must be 16-byte aligned");
+  TC_CHECK(n_edges < INT32_MAX, TC_ERR_INVALID, "edge ids must fit int32");
+  TC_TRY(ensure_init());
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t E = n_edges;
+  Scratch use, list, nlist, svals, partial, acc;
+  TC_TRY(use.alloc(std::max<int64_t>(E, 1), s));
+  TC_TRY(list.alloc(sizeof(int32_t) * std::max<int64_t>(E, 1), s));
+  TC_TRY(nlist.alloc(sizeof(int64_t), s));
+  TC_TRY(svals.alloc(sizeof(double) * std::max<int64_t>(E, 1), s));
+  const int64_t max_tiles = (E + kSumTile - 1) / kSumTile + 1;
+  TC_TRY(partial.alloc(sizeof(double) * max_tiles, s));
+  TC_TRY(acc.alloc(sizeof(unsigned long long) * 4, s));
+  unsigned long long *exact_sum = acc.as<unsigned long long>();
+  unsigned long long *counters = exact_sum + 1;  // [n_hll, n_fallback, n_clamped]
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, exact_sum, 4, 0ull);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t *)nlist.ptr, 2, 0);
+  TC_HIP(hipGetLastError());
+  TC_DISPATCH_ROWS(row_ptr_64, I, {
+    Rows<I> R{(const I *)row_ptr};
+    Degrees<I> D{R, degrees};
+    if (E > 0) {
+      hipLaunchKernelGGL(k_route<I>, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock), 0, s, D,
+                         edge_u, edge_v, E, has_sketch, gamma, route, use.as<uint8_t>(), counters);
+      TC_HIP(hipGetLastError());
+      if (part != TC_PART_HLL) {
+        TC_TRY(exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, E,
+                                use.as<uint8_t>(), nullptr, exact_sum, nullptr, s));
+        if (per_edge)
+          hipLaunchKernelGGL(k_exact_per_edge<I>, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock),
+                             0, s, R, col_idx, edge_u, edge_v, E,
+                             (const uint8_t *)use.as<uint8_t>(), per_edge);
+      }
+      if (part != TC_PART_EXACT) {
+        TC_TRY(compact_indices(use.as<uint8_t>(), E, list.as<int32_t>(), nlist.as<int64_t>(), s));
+        const unsigned gw = (unsigned)std::max(1, num_cus() * 8);
+        {
+          StageTimer _timer(TC_STAGE_HLL_EDGES, s);
+          hipLaunchKernelGGL(k_hll_edges<I>, dim3(gw), dim3(kBlock), 0, s, D, edge_u, edge_v,
+                             (const int32_t *)list.as<int32_t>(), (const int64_t *)nlist.as<int64_t>(),
+                             regs, slot, p, svals.as<double>(), per_edge, counters + 2);
+        }
+        hipLaunchKernelGGL(k_tile_sums_f64, dim3((unsigned)std::min<int64_t>(max_tiles, 65536)),
+                           dim3(kBlock), 0, s, (const double *)svals.as<double>(),
+                           (const int64_t *)nlist.as<int64_t>(), partial.as<double>(), max_tiles);
+        TC_HIP(hipGetLastError());
+      }
+    }
+  });
+  hipLaunchKernelGGL(k_hybrid_finish, dim3(1), dim3(kBlock), 0, s,
+                     (const double *)partial.as<double>(), (const int64_t *)nlist.as<int64_t>(),
+                     (const unsigned long long *)exact_sum, (const unsigned long long *)counters, E,
+                     part, counts, stats, fallback);
+  TC_HIP(hipGetLastError());
+  return TC_OK;
+}

@@ -1,0 +1,276 @@
+"""Exact and hybrid (exact / HLL) triangle counting (drop-in for the reference's ``src/gpu_hybrid.py``).
+
+Same public surface as src/gpu_hybrid.py:
+
+* ``_THREADS``, ``_require_supported_block_size()``,
+  ``_warn_if_float64_atomic_slow(mod)`` (src/gpu_hybrid.py:13-39);
+* launch objects ``exact_count_kernel``, ``hybrid_count_kernel``,
+  ``hybrid_exact_part_kernel``, ``hybrid_hll_part_kernel`` taking the
+  reference argument lists with Numba ``[grid, block]`` syntax
+  (src/gpu_hybrid.py:96-325).  Grid and block are accepted and ignored: the
+  HIP pipeline chooses its own launch shapes.  Like the reference kernels they
+  ADD into ``counts`` (and ``stats``);
+* ``gpu_exact_triangle_count(row_ptr_d, col_idx_d, edge_u_d, edge_v_d)``
+  -> ``(count, seconds)`` (src/gpu_hybrid.py:332-354);
+* ``gpu_hybrid_triangle_count(..., gamma, p, split_kernels=True)``
+  -> ``(estimate, seconds, stats)`` (src/gpu_hybrid.py:357-431).
+
+Differences that keep results closer to the CPU oracle than the reference GPU
+path: the union estimator runs in FP64 (src/hll.py:37-54) instead of FP32, and
+the FP64 total is reduced in a fixed order instead of by float atomics, so
+repeated runs return identical estimates.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+
+_THREADS = 256
+_FLOAT64_ATOMIC_WARNED = False
+
+
+def _require_supported_block_size() -> None:
+    """Kept for API parity (src/gpu_hybrid.py:19-24)."""
+    if _THREADS != 256:
+        raise ValueError(
+            "gpu_hybrid kernels currently require _THREADS == 256 because "
+            "their shared-memory reduction buffers are statically sized to 256."
+        )
+
+
+def _warn_if_float64_atomic_slow(cuda_module=None) -> None:
+    """No-op on gfx950 (native FP64 atomics; the FP64 sum here uses none).
+
+    Kept for API parity with src/gpu_hybrid.py:27-39.
+    """
+    global _FLOAT64_ATOMIC_WARNED
+    _FLOAT64_ATOMIC_WARNED = True
+
+
+def _exact_params(params: Optional[dict]):
+    if not params:
+        return None
+    return _lib.ExactParams(int(params.get("light_max", 0)), int(params.get("chunk", 0)),
+                            int(params.get("waves_per_cu", 0)), 0)
+
+
+def exact_count(row_ptr, col_idx, edge_u=None, edge_v=None, count=None, params=None,
+                stream=None):
+    """Enqueue the exact count; returns the device uint64 accumulator (``count`` if given)."""
+    t = _lib.require_device()
+    rp, is64 = _lib.row_ptr_arg(row_ptr)
+    col = _lib.as_device(col_idx, t.int32, "col_idx")
+    n = rp.numel() - 1
+    if edge_u is None:
+        eu = ev = None
+        E = col.numel()
+    else:
+        eu = _lib.as_device(edge_u, t.int32, "edge_u")
+        ev = _lib.as_device(edge_v, t.int32, "edge_v")
+        if eu.numel() != ev.numel():
+            raise ValueError("edge_u and edge_v must have the same length")
+        E = eu.numel()
+    if count is None:
+        count = t.zeros(1, dtype=t.int64, device=rp.device)
+    p = _exact_params(params)
+    _lib.check(_lib.load().tc_exact_count(_lib.ptr(rp), is64, _lib.ptr(col), n, _lib.ptr(eu),
+                                          _lib.ptr(ev), E, _lib.ptr(count),
+                                          ctypes.byref(p) if p is not None else None,
+                                          _lib.stream_ptr(stream)))
+    return count
+
+
+def edge_supports(row_ptr, col_idx, edge_u, edge_v, stream=None):
+    """Per-edge ``|N+(u) ∩ N+(v)|`` as a device int32 tensor."""
+    t = _lib.require_device()
+    rp, is64 = _lib.row_ptr_arg(row_ptr)
+    col = _lib.as_device(col_idx, t.int32, "col_idx")
+    eu = _lib.as_device(edge_u, t.int32, "edge_u")
+    ev = _lib.as_device(edge_v, t.int32, "edge_v")
+    out = t.empty(eu.numel(), dtype=t.int32, device=rp.device)
+    _lib.check(_lib.load().tc_edge_supports(_lib.ptr(rp), is64, _lib.ptr(col), _lib.ptr(eu),
+                                            _lib.ptr(ev), eu.numel(), _lib.ptr(out),
+                                            _lib.stream_ptr(stream)))
+    return out
+
+
+def hybrid_count(row_ptr, col_idx, edge_u, edge_v, degrees, sketch_regs, has_sketch, gamma,
+                 p, part=_lib.TC_PART_BOTH, counts=None, stats=None, fallback=None,
+                 route=None, slot=None, per_edge=None, stream=None):
+    """Enqueue one hybrid pass (``tc_hybrid_count``); returns ``(counts, stats, fallback)``."""
+    t = _lib.require_device()
+    eu = _lib.as_device(edge_u, t.int32, "edge_u")
+    ev = _lib.as_device(edge_v, t.int32, "edge_v")
+    if eu.numel() != ev.numel():
+        raise ValueError("edge_u and edge_v must have the same length")
+    dev = eu.device
+    if row_ptr is not None:
+        rp, is64 = _lib.row_ptr_arg(row_ptr)
+        n = rp.numel() - 1
+    else:
+        rp, is64 = None, 0
+        n = int(degrees.numel())
+    col = _lib.as_device(col_idx, t.int32, "col_idx") if col_idx is not None else None
+    deg = _lib.as_device(degrees, t.int32, "degrees") if degrees is not None else None
+    regs = _lib.as_device(sketch_regs, t.uint8) if sketch_regs is not None else None
+    has = _lib.as_device(has_sketch, t.uint8)
+    rt = _lib.as_device(route, t.uint8) if route is not None else None
+    sl = _lib.as_device(slot, t.int32) if slot is not None else None
+    if counts is None:
+        counts = t.zeros(1, dtype=t.float64, device=dev)
+    if stats is None:
+        stats = t.zeros(3, dtype=t.int64, device=dev)
+    if fallback is None:
+        fallback = t.zeros(1, dtype=t.int64, device=dev)
+    _lib.check(_lib.load().tc_hybrid_count(
+        _lib.ptr(rp), is64, _lib.ptr(col), n, _lib.ptr(eu), _lib.ptr(ev), eu.numel(),
+        _lib.ptr(deg), _lib.ptr(regs), _lib.ptr(sl), _lib.ptr(has), int(gamma), int(p),
+        _lib.ptr(rt), int(part), _lib.ptr(counts), _lib.ptr(stats), _lib.ptr(fallback),
+        _lib.ptr(per_edge), _lib.stream_ptr(stream)))
+    return counts, stats, fallback
+
+
+def _n(x, n_edges) -> int:
+    return int(n_edges)
+
+
+def _edge_slice(a, n_edges):
+    t = _lib.torch()
+    if isinstance(a, t.Tensor):
+        return a[: int(n_edges)]
+    return np.asarray(a)[: int(n_edges)]
+
+
+class _Launcher:
+    """Numba-style ``kernel[grid, block](*args)`` front for a HIP pipeline."""
+
+    def __init__(self, fn, name):
+        self._fn = fn
+        self.__name__ = name
+
+    def __getitem__(self, config):
+        return self._fn
+
+    def __call__(self, *args, **kwargs):
+        raise TypeError(f"launch with {self.__name__}[grid, block](...)")
+
+
+def _exact_kernel(row_ptr, col_idx, edge_u, edge_v, n_edges, counts):
+    exact_count(row_ptr, col_idx, _edge_slice(edge_u, n_edges), _edge_slice(edge_v, n_edges),
+                count=counts)
+
+
+def _hybrid_fused(row_ptr, col_idx, edge_u, edge_v, n_edges, degrees, sketch_regs, has_sketch,
+                  gamma, p, counts, stats):
+    hybrid_count(row_ptr, col_idx, _edge_slice(edge_u, n_edges), _edge_slice(edge_v, n_edges),
+                 degrees, sketch_regs, has_sketch, gamma, p, _lib.TC_PART_BOTH, counts, stats)
+
+
+def _hybrid_exact_part(row_ptr, col_idx, edge_u, edge_v, n_edges, degrees, has_sketch, gamma,
+                       counts, stats):
+    hybrid_count(row_ptr, col_idx, _edge_slice(edge_u, n_edges), _edge_slice(edge_v, n_edges),
+                 degrees, None, has_sketch, gamma, 8, _lib.TC_PART_EXACT, counts, stats)
+
+
+def _hybrid_hll_part(edge_u, edge_v, n_edges, degrees, sketch_regs, has_sketch, gamma, p, counts,
+                     stats):
+    hybrid_count(None, None, _edge_slice(edge_u, n_edges), _edge_slice(edge_v, n_edges),
+                 degrees, sketch_regs, has_sketch, gamma, p, _lib.TC_PART_HLL, counts, stats)
+
+
+exact_count_kernel = _Launcher(_exact_kernel, "exact_count_kernel")
+hybrid_count_kernel = _Launcher(_hybrid_fused, "hybrid_count_kernel")
+hybrid_exact_part_kernel = _Launcher(_hybrid_exact_part, "hybrid_exact_part_kernel")
+hybrid_hll_part_kernel = _Launcher(_hybrid_hll_part, "hybrid_hll_part_kernel")
+
+
+def _sync():
+    _lib.torch().cuda.synchronize()
+
+
+def gpu_exact_triangle_count(row_ptr_d, col_idx_d, edge_u_d, edge_v_d, params=None):
+    """Exact count and wall seconds around the device work (src/gpu_hybrid.py:332-354)."""
+    _require_supported_block_size()
+    t = _lib.require_device()
+    rp, _ = _lib.row_ptr_arg(row_ptr_d)
+    col = _lib.as_device(col_idx_d, t.int32, "col_idx")
+    eu = _lib.as_device(edge_u_d, t.int32, "edge_u")
+    ev = _lib.as_device(edge_v_d, t.int32, "edge_v")
+    counts = t.zeros(1, dtype=t.int64, device=rp.device)
+    _sync()
+    t0 = time.perf_counter()
+    exact_count(rp, col, eu, ev, count=counts, params=params)
+    _sync()
+    elapsed = time.perf_counter() - t0
+    return int(counts.item()), elapsed
+
+
+def gpu_hybrid_triangle_count(row_ptr_d, col_idx_d, edge_u_d, edge_v_d, degrees_d,
+                              sketch_regs_d, has_sketch_d, gamma: int, p: int,
+                              split_kernels: bool = True, truth: Optional[int] = None,
+                              route=None, slot=None):
+    """Hybrid estimate, wall seconds and stats (src/gpu_hybrid.py:357-431).
+
+    ``stats`` holds the reference keys ``n_approx, n_exact, n_clamped,
+    approx_coverage, clamp_ratio, kernel_mode`` plus ``n_fallback`` and
+    ``fallback_ratio`` (src/hybrid.py:152-164) and, when ``truth`` is given,
+    ``rel_err`` (src/utils.py:36-43).
+    """
+    _require_supported_block_size()
+    _warn_if_float64_atomic_slow()
+    t = _lib.require_device()
+    rp, _ = _lib.row_ptr_arg(row_ptr_d)
+    col = _lib.as_device(col_idx_d, t.int32, "col_idx")
+    eu = _lib.as_device(edge_u_d, t.int32, "edge_u")
+    ev = _lib.as_device(edge_v_d, t.int32, "edge_v")
+    deg = _lib.as_device(degrees_d, t.int32, "degrees")
+    regs = _lib.as_device(sketch_regs_d, t.uint8)
+    has = _lib.as_device(has_sketch_d, t.uint8)
+    n_edges = eu.numel()
+    dev = rp.device
+    counts = t.zeros(1, dtype=t.float64, device=dev)
+    stats_d = t.zeros(3, dtype=t.int64, device=dev)
+    fb = t.zeros(1, dtype=t.int64, device=dev)
+    _sync()
+    t0 = time.perf_counter()
+    if split_kernels:
+        hybrid_count(rp, col, eu, ev, deg, None, has, gamma, p, _lib.TC_PART_EXACT, counts,
+                     stats_d, fb, route=route, slot=slot)
+        hybrid_count(None, None, eu, ev, deg, regs, has, gamma, p, _lib.TC_PART_HLL, counts,
+                     stats_d, None, route=route, slot=slot)
+    else:
+        hybrid_count(rp, col, eu, ev, deg, regs, has, gamma, p, _lib.TC_PART_BOTH, counts,
+                     stats_d, fb, route=route, slot=slot)
+    _sync()
+    elapsed = time.perf_counter() - t0
+    estimate = float(counts.item())
+    s = stats_d.cpu().numpy()
+    n_approx, n_exact, n_clamped = int(s[0]), int(s[1]), int(s[2])
+    n_fallback = int(fb.item())
+    stats = {
+        "n_approx": n_approx,
+        "n_exact": n_exact,
+        "n_clamped": n_clamped,
+        "approx_coverage": n_approx / n_edges if n_edges > 0 else 0.0,
+        "clamp_ratio": n_clamped / n_approx if n_approx > 0 else 0.0,
+        "kernel_mode": "split_block_reduction" if split_kernels else "fused_block_reduction",
+        "n_fallback": n_fallback,
+        "fallback_ratio": (n_fallback / (n_approx + n_fallback)
+                           if (n_approx + n_fallback) > 0 else 0.0),
+    }
+    if truth is not None:
+        stats["rel_err"] = relative_error(estimate, truth)
+    return estimate, elapsed, stats
+
+
+def relative_error(estimate: float, truth: int) -> float:
+    """``|estimate - truth| / truth``; inf if truth == 0 != estimate (src/utils.py:36-43)."""
+    if truth == 0:
+        return float("inf") if estimate != 0 else 0.0
+    return abs(estimate - truth) / truth
