@@ -86,9 +86,13 @@ def main():
                     choices=["chunglu", "rgg21", "rmat14", "rmat20", "rmat23"])
     ap.add_argument("--scale-down", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--light-max", type=int, default=16)
-    ap.add_argument("--chunk", type=int, default=256)
+    ap.add_argument("--light-max", type=int, default=32)
+    ap.add_argument("--chunk", type=int, default=1024)
     ap.add_argument("--waves-per-cu", type=int, default=16)
+    ap.add_argument("--unroll", type=int, default=4)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (reference point for "
+                         "HBM byte counters under a profiler)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -108,6 +112,13 @@ def main():
     torch.cuda.set_device(0)
     info = _lib.device_info()
     _log(f"device {info}")
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        _log("calibration copy: 1 GiB read + 1 GiB written")
 
     t0 = time.perf_counter()
     pairs = config_pairs(args.workload, args.scale_down)
@@ -125,7 +136,7 @@ def main():
     _log(f"device CSR: n={csr.n_nodes} E={E} in {t_csr * 1e3:.1f} ms")
 
     params = {"light_max": args.light_max, "chunk": args.chunk,
-              "waves_per_cu": args.waves_per_cu}
+              "waves_per_cu": args.waves_per_cu, "unroll": args.unroll}
     count = torch.zeros(1, dtype=torch.int64, device="cuda")
     for _ in range(args.warmup):
         count.zero_()
@@ -183,7 +194,7 @@ def main():
                    else f"{args.workload}/2^{args.scale_down}",
                    "n_nodes": csr.n_nodes, "oriented_edges": E, "triangles": gpu_count,
                    "row_ptr": "int64", "light_max": args.light_max, "chunk": args.chunk,
-                   "waves_per_cu": args.waves_per_cu},
+                   "waves_per_cu": args.waves_per_cu, "unroll": args.unroll},
         "parity_vs_cpu_oracle": parity,
         "roofline": {
             "bound": "hbm",

@@ -100,7 +100,7 @@ typedef struct tc_exact_params {
   int light_max;     /* edges with deg+u + deg+v <= light_max use a lane merge */
   int chunk;         /* others per (center, chunk) work item                   */
   int waves_per_cu;  /* persistent waves per CU for the heavy kernel           */
-  int reserved;
+  int unroll;        /* streamed ids in flight per lane: 1, 2, 4 or 8          */
 } tc_exact_params;
 
 /* count[0] += sum over the given oriented edges (edge_u[e] -> edge_v[e]) of

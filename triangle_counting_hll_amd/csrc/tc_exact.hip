@@ -23,8 +23,9 @@
 
 namespace tc {
 
-constexpr int kTblSlots = 2048;          // per-wave LDS hash slots (8 KiB)
-constexpr int kTblMaxDeg = kTblSlots / 2;  // centers above this use the global path
+constexpr int kTblInts = 2048;    // per-wave LDS hash table: 512 buckets x 4 slots (8 KiB)
+constexpr int kTblMaxDeg = 1024;  // longer center rows use the global binary-search path
+constexpr int kDefaultUnroll = 4; // streamed ids in flight per lane
 constexpr int kWavesPerBlock = kBlock / kWave;
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -138,23 +139,135 @@ __global__ void k_item_fill(const int32_t *__restrict__ nitems, const int64_t *_
   }
 }
 
-__device__ __forceinline__ uint32_t tbl_hash(uint32_t x, int shift) {
+// LDS hash set of one center row: buckets of 4 int slots read with one
+// 16-byte LDS load.  A bucket's slots fill in order 0..3 and a key moves to the
+// next bucket only when its home bucket is full, so a lookup stops at the
+// first bucket whose last slot is empty.  Load factor <= 1/4 up to 512 ids and
+// <= 1/2 up to kTblMaxDeg ids.
+__device__ __forceinline__ uint32_t bucket_of(uint32_t x, int shift) {
   return (x * 0x9E3779B1u) >> shift;
 }
 
-template <class I>
+__device__ __forceinline__ void table_build(int *__restrict__ tbl, const int32_t *__restrict__ row,
+                                            int dc, int &shift, uint32_t &mask) {
+  const int lane = lane_id();
+  int logNB = 2;
+  while ((1 << logNB) < dc && (4 << logNB) < kTblInts) ++logNB;
+  const int NB = 1 << logNB;
+  shift = 32 - logNB;
+  mask = (uint32_t)NB - 1;
+  for (int i = lane; i < 4 * NB; i += kWave) tbl[i] = -1;
+  wave_lds_sync();
+  for (int i = lane; i < dc; i += kWave) {
+    const int x = row[i];
+    uint32_t b = bucket_of((uint32_t)x, shift);
+    while (true) {
+      int *sl = tbl + 4 * b;
+      if (atomicCAS(sl, -1, x) == -1 || atomicCAS(sl + 1, -1, x) == -1 ||
+          atomicCAS(sl + 2, -1, x) == -1 || atomicCAS(sl + 3, -1, x) == -1)
+        break;
+      b = (b + 1) & mask;
+    }
+  }
+  wave_lds_sync();
+}
+
+__device__ __forceinline__ uint32_t table_probe(const int4 *__restrict__ tbl, int x, int shift,
+                                                uint32_t mask) {
+  uint32_t b = bucket_of((uint32_t)x, shift);
+  while (true) {
+    const int4 v = tbl[b];  // one ds_read_b128
+    const bool hit = (v.x == x) | (v.y == x) | (v.z == x) | (v.w == x);
+    if (hit) return 1;
+    if (v.w == -1) return 0;
+    b = (b + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ uint32_t global_probe(const int32_t *__restrict__ col, int64_t cb,
+                                                 int dc, int x) {
+  int64_t lo = cb, hi = cb + dc;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (col[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return (lo < cb + dc && col[lo] == x) ? 1u : 0u;
+}
+
+// Streams the rows of others[ob:oe] as one flattened sequence, U ids
+// per lane in flight, and counts the ids found in the center's set.
+template <class I, bool kLds, int U>
+__device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__restrict__ col,
+                                                  const int32_t *__restrict__ others, int64_t ob,
+                                                  int64_t oe, int *__restrict__ seg_end,
+                                                  int64_t *__restrict__ seg_base,
+                                                  const int4 *__restrict__ tbl, int shift,
+                                                  uint32_t mask, int64_t cb, int dc) {
+  const int lane = lane_id();
+  uint64_t acc = 0;
+  for (int64_t base = ob; base < oe; base += kWave) {
+    const int64_t k = base + lane;
+    int len = 0;
+    int64_t st = 0;
+    if (k < oe) {
+      const int32_t o = others[k];
+      st = R.begin(o);
+      len = (int)(R.end(o) - st);
+    }
+    const int inc = wave_inclusive_scan(len);
+    const int total = __shfl(inc, kWave - 1);
+    seg_end[lane] = inc;
+    seg_base[lane] = st - (int64_t)(inc - len);
+    wave_lds_sync();
+    // current row of this lane: its end in the flattened sequence and the
+    // offset that maps a flattened position to col[]; LDS is read only when
+    // the lane crosses into the next row
+    int seg = 0;
+    int seg_hi = seg_end[0];
+    int64_t seg_off = seg_base[0];
+    for (int t0 = lane; t0 < total; t0 += kWave * U) {
+      int xs[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = t0 + u * kWave;
+        xs[u] = -1;
+        if (t < total) {
+          while (seg_hi <= t) {
+            ++seg;
+            seg_hi = seg_end[seg];
+            seg_off = seg_base[seg];
+          }
+          xs[u] = col[seg_off + t];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (xs[u] >= 0) {
+          if constexpr (kLds)
+            acc += table_probe(tbl, xs[u], shift, mask);
+          else
+            acc += global_probe(col, cb, dc, xs[u]);
+        }
+      }
+    }
+    wave_lds_sync();
+  }
+  return acc;
+}
+
+template <class I, int U>
 __global__ void __launch_bounds__(kBlock)
 k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
         const int32_t *__restrict__ others, const int2 *__restrict__ items,
         const int64_t *__restrict__ n_items_p, int chunk,
         unsigned long long *__restrict__ next_item, const int32_t *__restrict__ graph_of,
         unsigned long long *__restrict__ count) {
-  __shared__ int tbl_all[kWavesPerBlock][kTblSlots];
+  __shared__ int4 tbl_all[kWavesPerBlock][kTblInts / 4];
   __shared__ int seg_end_all[kWavesPerBlock][kWave];
   __shared__ int64_t seg_base_all[kWavesPerBlock][kWave];
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
-  int *tbl = tbl_all[w];
+  int4 *tbl = tbl_all[w];
   int *seg_end = seg_end_all[w];
   int64_t *seg_base = seg_base_all[w];
   const int64_t n_items = *n_items_p;
@@ -171,80 +284,15 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
     const int64_t oe = min(cptr[c + 1], ob + (int64_t)chunk);
     const int64_t cb = R.begin(c);
     const int dc = (int)(R.end(c) - cb);
-    uint64_t acc = 0;
-
+    uint64_t acc;
     if (dc <= kTblMaxDeg) {
-      int logS = 6;
-      while ((1 << logS) < 2 * dc) ++logS;
-      const int S = 1 << logS, shift = 32 - logS;
-      for (int i = lane; i < S; i += kWave) tbl[i] = -1;
-      wave_lds_sync();
-      for (int i = lane; i < dc; i += kWave) {
-        const int x = col[cb + i];
-        uint32_t h = tbl_hash((uint32_t)x, shift);
-        while (atomicCAS(&tbl[h], -1, x) != -1) h = (h + 1) & (S - 1);
-      }
-      wave_lds_sync();
-      for (int64_t base = ob; base < oe; base += kWave) {
-        const int64_t k = base + lane;
-        int len = 0;
-        int64_t st = 0;
-        if (k < oe) {
-          const int32_t o = others[k];
-          st = R.begin(o);
-          len = (int)(R.end(o) - st);
-        }
-        const int inc = wave_inclusive_scan(len);
-        const int total = __shfl(inc, kWave - 1);
-        seg_end[lane] = inc;
-        seg_base[lane] = st - (int64_t)(inc - len);
-        wave_lds_sync();
-        int seg = 0;
-        for (int t = lane; t < total; t += kWave) {
-          while (seg_end[seg] <= t) ++seg;
-          const int x = col[seg_base[seg] + t];
-          uint32_t h = tbl_hash((uint32_t)x, shift);
-          while (true) {
-            const int y = tbl[h];
-            if (y == x) {
-              ++acc;
-              break;
-            }
-            if (y == -1) break;
-            h = (h + 1) & (S - 1);
-          }
-        }
-        wave_lds_sync();
-      }
+      int shift;
+      uint32_t mask;
+      table_build(reinterpret_cast<int *>(tbl), col + cb, dc, shift, mask);
+      acc = stream_others<I, true, U>(R, col, others, ob, oe, seg_end, seg_base, tbl, shift, mask,
+                                   cb, dc);
     } else {
-      // Long center row: binary search each streamed id in the global row.
-      for (int64_t base = ob; base < oe; base += kWave) {
-        const int64_t k = base + lane;
-        int len = 0;
-        int64_t st = 0;
-        if (k < oe) {
-          const int32_t o = others[k];
-          st = R.begin(o);
-          len = (int)(R.end(o) - st);
-        }
-        const int inc = wave_inclusive_scan(len);
-        const int total = __shfl(inc, kWave - 1);
-        seg_end[lane] = inc;
-        seg_base[lane] = st - (int64_t)(inc - len);
-        wave_lds_sync();
-        int seg = 0;
-        for (int t = lane; t < total; t += kWave) {
-          while (seg_end[seg] <= t) ++seg;
-          const int x = col[seg_base[seg] + t];
-          int64_t lo = cb, hi = cb + dc;
-          while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (col[mid] < x) lo = mid + 1; else hi = mid;
-          }
-          if (lo < cb + dc && col[lo] == x) ++acc;
-        }
-        wave_lds_sync();
-      }
+      acc = stream_others<I, false, U>(R, col, others, ob, oe, seg_end, seg_base, tbl, 0, 0, cb, dc);
     }
     if (graph_of) {
       acc = wave_sum(acc);
@@ -286,13 +334,24 @@ __global__ void k_edge_supports(Rows<I> R, const int32_t *__restrict__ col,
   }
 }
 
+template <class I>
+static auto heavy_kernel(int unroll) -> decltype(&k_heavy<I, 4>) {
+  switch (unroll) {
+    case 1: return &k_heavy<I, 1>;
+    case 2: return &k_heavy<I, 2>;
+    case 8: return &k_heavy<I, 8>;
+    default: return &k_heavy<I, 4>;
+  }
+}
+
 int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, int64_t n,
                      const int32_t *eu, const int32_t *ev, int64_t E, const uint8_t *skip,
                      const int32_t *graph_of, unsigned long long *count,
                      const tc_exact_params *params, hipStream_t s) {
-  const int light_max = (params && params->light_max > 0) ? params->light_max : 16;
-  const int chunk = (params && params->chunk > 0) ? params->chunk : 256;
+  const int light_max = (params && params->light_max > 0) ? params->light_max : 32;
+  const int chunk = (params && params->chunk > 0) ? params->chunk : 1024;
   const int wpc = (params && params->waves_per_cu > 0) ? params->waves_per_cu : 16;
+  const int unroll = (params && params->unroll > 0) ? params->unroll : kDefaultUnroll;
   if (E <= 0 || n <= 0) return TC_OK;
 
   Scratch eu_buf;
@@ -346,7 +405,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
-      hipLaunchKernelGGL(k_heavy<I>, dim3(gh), dim3(kBlock), 0, s, R, col,
+      hipLaunchKernelGGL(heavy_kernel<I>(unroll), dim3(gh), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)cptr.as<int64_t>(), (const int32_t *)others.as<int32_t>(),
                          (const int2 *)items.as<int2>(), (const int64_t *)(iptr.as<int64_t>() + n),
                          chunk, ctr.as<unsigned long long>(), graph_of, count);

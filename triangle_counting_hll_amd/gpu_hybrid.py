@@ -57,7 +57,7 @@ def _exact_params(params: Optional[dict]):
     if not params:
         return None
     return _lib.ExactParams(int(params.get("light_max", 0)), int(params.get("chunk", 0)),
-                            int(params.get("waves_per_cu", 0)), 0)
+                            int(params.get("waves_per_cu", 0)), int(params.get("unroll", 0)))
 
 
 def exact_count(row_ptr, col_idx, edge_u=None, edge_v=None, count=None, params=None,
