@@ -1,4 +1,5 @@
 // Library-wide state, error reporting and device-wide scan / compaction.
+#include <algorithm>
 #include <cstdarg>
 #include <mutex>
 #include <vector>
@@ -171,6 +172,72 @@ __global__ void k_scatter_flagged(const uint8_t *__restrict__ flag, const int64_
 }
 
 __global__ void k_copy_i64(const int64_t *src, int64_t *dst) { *dst = *src; }
+
+// ---------------------------------------------------------------------------
+// Stable bucket sort by uint8 key.  Tiles of 64 indices (one wavefront); the
+// per-(bucket, tile) counts are laid out bucket-major so one exclusive scan
+// gives every output offset; ranks inside a tile come from ballot masks.
+// ---------------------------------------------------------------------------
+__global__ void k_bucket_count(const uint8_t *__restrict__ key, int64_t n, int64_t ntiles,
+                               int nb, int32_t *__restrict__ counts) {
+  const int64_t tile = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (tile >= ntiles) return;
+  const int lane = lane_id();
+  const int64_t i = tile * kWave + lane;
+  const int b = (i < n) ? (int)key[i] : 255;
+  for (int k = 0; k < nb; ++k) {
+    const uint64_t m = __ballot(b == k);
+    if (lane == 0) counts[(int64_t)k * ntiles + tile] = __popcll(m);
+  }
+}
+
+__global__ void k_bucket_scatter(const uint8_t *__restrict__ key, int64_t n, int64_t ntiles,
+                                 int nb, const int64_t *__restrict__ offs,
+                                 int32_t *__restrict__ perm) {
+  const int64_t tile = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (tile >= ntiles) return;
+  const int lane = lane_id();
+  const int64_t i = tile * kWave + lane;
+  const int b = (i < n) ? (int)key[i] : 255;
+  uint64_t peers = ~0ull;
+#pragma unroll
+  for (int bit = 0; bit < 8; ++bit) {
+    const uint64_t m = __ballot((b >> bit) & 1);
+    peers &= ((b >> bit) & 1) ? m : ~m;
+  }
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  if (b < nb) perm[offs[(int64_t)b * ntiles + tile] + __popcll(peers & lt)] = (int32_t)i;
+}
+
+__global__ void k_bucket_starts(const int64_t *__restrict__ offs, int64_t ntiles, int nb,
+                                int64_t *__restrict__ starts) {
+  const int k = threadIdx.x;
+  if (k <= nb) starts[k] = offs[(int64_t)k * ntiles];
+}
+
+int bucket_sort_u8(const uint8_t *key, int64_t n, int nb, int32_t *perm, int64_t *starts,
+                   hipStream_t s) {
+  if (nb < 1 || nb > 64) {
+    set_error(TC_ERR_INVALID, "bucket count %d out of range", nb);
+    return TC_ERR_INVALID;
+  }
+  const int64_t ntiles = std::max<int64_t>((n + kWave - 1) / kWave, 1);
+  Scratch counts, offs;
+  TC_TRY(counts.alloc(sizeof(int32_t) * ntiles * nb, s));
+  TC_TRY(offs.alloc(sizeof(int64_t) * (ntiles * nb + 1), s));
+  const unsigned g = grid_for(ntiles * kWave);
+  hipLaunchKernelGGL(k_bucket_count, dim3(g), dim3(kBlock), 0, s, key, n, ntiles, nb,
+                     counts.as<int32_t>());
+  TC_HIP(hipGetLastError());
+  TC_TRY(exclusive_scan<int32_t>(counts.as<int32_t>(), offs.as<int64_t>(), ntiles * nb, nullptr,
+                                 s));
+  hipLaunchKernelGGL(k_bucket_scatter, dim3(g), dim3(kBlock), 0, s, key, n, ntiles, nb,
+                     (const int64_t *)offs.as<int64_t>(), perm);
+  hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(128), 0, s,
+                     (const int64_t *)offs.as<int64_t>(), ntiles, nb, starts);
+  TC_HIP(hipGetLastError());
+  return TC_OK;
+}
 
 int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d_count,
                     hipStream_t s) {

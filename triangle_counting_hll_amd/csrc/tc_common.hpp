@@ -164,6 +164,13 @@ template <class I> struct Rows {
 template <class T>
 int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t *d_total, hipStream_t s);
 
+// Stable bucket sort of 0..n-1 by a small key (key[i] < nbuckets <= 64, keys
+// equal to nbuckets or more are dropped).  perm receives the indices grouped
+// by ascending key, ascending index inside a key; starts[k] (k = 0..nbuckets,
+// device int64) is the offset of bucket k in perm, starts[nbuckets] the total.
+int bucket_sort_u8(const uint8_t *key, int64_t n, int nbuckets, int32_t *perm, int64_t *starts,
+                   hipStream_t s);
+
 // Stream compaction by flag: idx_out receives the indices i with flag[i] != 0 in
 // increasing order; *d_count (device) receives their number.
 int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d_count,

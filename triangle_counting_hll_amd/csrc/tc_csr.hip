@@ -475,56 +475,19 @@ extern "C" int tc_csr_build_finish(tc_csr_build *b, void *row_ptr, int row_ptr_6
 }
 
 // ---------------------------------------------------------------------------
-// Degree-bucket sort: bucket(u) = ceil(log2(deg+(u) + 1)), stable in u.
-// Tiles of 64 vertices (one wavefront); counts laid out bucket-major so one
-// exclusive scan yields every (bucket, tile) output offset.
+// Degree-bucket sort: bucket(u) = ceil(log2(deg+(u) + 1)), stable in u,
+// on top of the generic stable bucket sort of tc_common.
 // ---------------------------------------------------------------------------
 namespace tc {
-constexpr int kBuckets = 33;
+constexpr int kDegBuckets = 33;
 
 template <class I>
-__device__ __forceinline__ int deg_bucket(const I *rp, int64_t u) {
-  uint64_t d = (uint64_t)((int64_t)rp[u + 1] - (int64_t)rp[u]);
-  return d == 0 ? 0 : 64 - __clzll(d);  // ceil(log2(d+1)) = bit length of d
-}
-
-template <class I>
-__global__ void k_bucket_count(const I *__restrict__ rp, int64_t n, int64_t ntiles,
-                               int32_t *__restrict__ counts) {
-  const int64_t tile = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  if (tile >= ntiles) return;
-  const int lane = lane_id();
-  const int64_t u = tile * kWave + lane;
-  int b = (u < n) ? deg_bucket(rp, u) : -1;
-  for (int k = 0; k < kBuckets; ++k) {
-    uint64_t m = __ballot(b == k);
-    if (lane == 0) counts[(int64_t)k * ntiles + tile] = __popcll(m);
+__global__ void k_degree_keys(const I *__restrict__ rp, int64_t n, uint8_t *__restrict__ key) {
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t d = (uint64_t)((int64_t)rp[u + 1] - (int64_t)rp[u]);
+    key[u] = (uint8_t)(d == 0 ? 0 : 64 - __clzll((long long)d));  // bit length of d
   }
-}
-
-template <class I>
-__global__ void k_bucket_scatter(const I *__restrict__ rp, int64_t n, int64_t ntiles,
-                                 const int64_t *__restrict__ offs, int32_t *__restrict__ perm) {
-  const int64_t tile = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  if (tile >= ntiles) return;
-  const int lane = lane_id();
-  const int64_t u = tile * kWave + lane;
-  const bool valid = u < n;
-  int b = valid ? deg_bucket(rp, u) : 63;
-  uint64_t peers = ~0ull;
-#pragma unroll
-  for (int bit = 0; bit < 6; ++bit) {
-    uint64_t m = __ballot((b >> bit) & 1);
-    peers &= ((b >> bit) & 1) ? m : ~m;
-  }
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  if (valid) perm[offs[(int64_t)b * ntiles + tile] + __popcll(peers & lt)] = (int32_t)u;
-}
-
-__global__ void k_bucket_starts(const int64_t *__restrict__ offs, int64_t ntiles,
-                                int64_t *__restrict__ starts) {
-  int k = threadIdx.x;
-  if (k <= kBuckets) starts[k] = offs[(int64_t)k * ntiles];
 }
 }  // namespace tc
 
@@ -535,23 +498,12 @@ extern "C" int tc_degree_buckets(const void *row_ptr, int row_ptr_64, int64_t n_
            "invalid arguments to tc_degree_buckets");
   TC_TRY(ensure_init());
   hipStream_t s = (hipStream_t)stream;
-  const int64_t ntiles = std::max<int64_t>((n_nodes + kWave - 1) / kWave, 1);
-  Scratch counts, offs;
-  TC_TRY(counts.alloc(sizeof(int32_t) * ntiles * kBuckets, s));
-  TC_TRY(offs.alloc(sizeof(int64_t) * (ntiles * kBuckets + 1), s));
-  const unsigned g = grid_for(ntiles * kWave);
+  Scratch keys;
+  TC_TRY(keys.alloc(std::max<int64_t>(n_nodes, 1), s));
   TC_DISPATCH_ROWS(row_ptr_64, I, {
-    hipLaunchKernelGGL(k_bucket_count<I>, dim3(g), dim3(kBlock), 0, s, (const I *)row_ptr,
-                       n_nodes, ntiles, counts.as<int32_t>());
-    TC_HIP(hipGetLastError());
-    TC_TRY(exclusive_scan<int32_t>(counts.as<int32_t>(), offs.as<int64_t>(), ntiles * kBuckets,
-                                   nullptr, s));
-    hipLaunchKernelGGL(k_bucket_scatter<I>, dim3(g), dim3(kBlock), 0, s, (const I *)row_ptr,
-                       n_nodes, ntiles, (const int64_t *)offs.as<int64_t>(), perm);
-    TC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_degree_keys<I>, dim3(grid_for(n_nodes, kBlock, 16384)), dim3(kBlock), 0,
+                       s, (const I *)row_ptr, n_nodes, keys.as<uint8_t>());
   });
-  hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(64), 0, s,
-                     (const int64_t *)offs.as<int64_t>(), ntiles, bucket_start);
   TC_HIP(hipGetLastError());
-  return TC_OK;
+  return bucket_sort_u8(keys.as<uint8_t>(), n_nodes, kDegBuckets, perm, bucket_start, s);
 }

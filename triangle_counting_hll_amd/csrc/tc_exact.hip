@@ -122,20 +122,42 @@ k_scatter_others(Rows<I> R, const int32_t *__restrict__ eu, const int32_t *__res
   }
 }
 
-__global__ void k_item_counts(const int32_t *__restrict__ center_cnt, int64_t n, int chunk,
-                              int32_t *__restrict__ nitems) {
-  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-       c += (int64_t)gridDim.x * blockDim.x)
-    nitems[c] = (center_cnt[c] + chunk - 1) / chunk;
-}
+// Work items are ordered longest center row first: key = 32 - bit_length(d+c)
+// for centers with heavy edges, no key (255) otherwise.
+constexpr int kCenterBuckets = 32;
+constexpr int kBigCenterBits = 8;  // centers with d+ >= 128 get one item per queue ticket
+constexpr int kSmallBatch = 16;    // items per ticket for the shorter centers
 
-__global__ void k_item_fill(const int32_t *__restrict__ nitems, const int64_t *__restrict__ iptr,
-                            int64_t n, int2 *__restrict__ items) {
+template <class I>
+__global__ void k_center_keys(Rows<I> R, const int32_t *__restrict__ center_cnt, int64_t n,
+                              uint8_t *__restrict__ key) {
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += (int64_t)gridDim.x * blockDim.x) {
-    const int k = nitems[c];
-    const int64_t b = iptr[c];
-    for (int j = 0; j < k; ++j) items[b + j] = make_int2((int)c, j);
+    const uint64_t d = (uint64_t)R.deg(c);
+    key[c] = (center_cnt[c] > 0 && d > 0) ? (uint8_t)(kCenterBuckets - (64 - __clzll((long long)d)))
+                                          : (uint8_t)255;
+  }
+}
+
+__global__ void k_item_counts(const int32_t *__restrict__ center_cnt,
+                              const int32_t *__restrict__ perm, const int64_t *__restrict__ n_cent_p,
+                              int64_t n, int chunk, int32_t *__restrict__ nitems) {
+  const int64_t nc = *n_cent_p;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    nitems[i] = i < nc ? (center_cnt[perm[i]] + chunk - 1) / chunk : 0;
+}
+
+__global__ void k_item_fill(const int32_t *__restrict__ nitems, const int32_t *__restrict__ perm,
+                            const int64_t *__restrict__ iptr, const int64_t *__restrict__ n_cent_p,
+                            int2 *__restrict__ items) {
+  const int64_t nc = *n_cent_p;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = nitems[i];
+    const int c = perm[i];
+    const int64_t b = iptr[i];
+    for (int j = 0; j < k; ++j) items[b + j] = make_int2(c, j);
   }
 }
 
@@ -259,7 +281,8 @@ template <class I, int U>
 __global__ void __launch_bounds__(kBlock)
 k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
         const int32_t *__restrict__ others, const int2 *__restrict__ items,
-        const int64_t *__restrict__ n_items_p, int chunk,
+        const int64_t *__restrict__ n_items_p, const int64_t *__restrict__ iptr,
+        const int64_t *__restrict__ n_big_centers_p, int chunk,
         unsigned long long *__restrict__ next_item, const int32_t *__restrict__ graph_of,
         unsigned long long *__restrict__ count) {
   __shared__ int4 tbl_all[kWavesPerBlock][kTblInts / 4];
@@ -271,14 +294,27 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
   int *seg_end = seg_end_all[w];
   int64_t *seg_base = seg_base_all[w];
   const int64_t n_items = *n_items_p;
+  // items of the long centers come first; each queue ticket maps to one of
+  // them, past that point a ticket covers kSmallBatch items
+  const int64_t n_big = iptr[*n_big_centers_p];
   uint64_t acc_total = 0;
+  int64_t it = 0, it_end = 0;
 
   while (true) {
-    unsigned long long it = 0;
-    if (lane == 0) it = atomicAdd(next_item, 1ull);
-    it = __shfl(it, 0);
-    if ((int64_t)it >= n_items) break;
-    const int2 item = items[it];
+    if (it >= it_end) {
+      unsigned long long ticket = 0;
+      if (lane == 0) ticket = atomicAdd(next_item, 1ull);
+      const int64_t tk = (int64_t)__shfl(ticket, 0);
+      if (tk < n_big) {
+        it = tk;
+        it_end = tk + 1;
+      } else {
+        it = n_big + (tk - n_big) * kSmallBatch;
+        it_end = min(it + (int64_t)kSmallBatch, n_items);
+      }
+      if (it >= n_items) break;
+    }
+    const int2 item = items[it++];
     const int32_t c = item.x;
     const int64_t ob = cptr[c] + (int64_t)item.y * chunk;
     const int64_t oe = min(cptr[c + 1], ob + (int64_t)chunk);
@@ -366,7 +402,10 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
     ev = col;
   }
 
-  Scratch cnt, cptr, others, nitems, iptr, items, ctr;
+  Scratch cnt, cptr, others, nitems, iptr, items, ctr, ckey, perm, starts;
+  TC_TRY(ckey.alloc(n, s));
+  TC_TRY(perm.alloc(sizeof(int32_t) * n, s));
+  TC_TRY(starts.alloc(sizeof(int64_t) * (kCenterBuckets + 1), s));
   TC_TRY(cnt.alloc(sizeof(int32_t) * n, s));
   TC_TRY(cptr.alloc(sizeof(int64_t) * (n + 1), s));
   TC_TRY(others.alloc(sizeof(int32_t) * E, s));
@@ -395,12 +434,21 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                        light_max, (const int64_t *)cptr.as<int64_t>(), nitems.as<int32_t>(),
                        others.as<int32_t>());
     TC_HIP(hipGetLastError());
+    // order centers longest row first (stable bucket sort on bit length)
+    hipLaunchKernelGGL(k_center_keys<I>, dim3(gn), dim3(kBlock), 0, s, R,
+                       (const int32_t *)cnt.as<int32_t>(), n, ckey.as<uint8_t>());
+    TC_HIP(hipGetLastError());
+    TC_TRY(bucket_sort_u8(ckey.as<uint8_t>(), n, kCenterBuckets, perm.as<int32_t>(),
+                          starts.as<int64_t>(), s));
+    const int64_t *n_cent = starts.as<int64_t>() + kCenterBuckets;
+    const int64_t *n_big_cent = starts.as<int64_t>() + (kCenterBuckets - kBigCenterBits + 1);
     hipLaunchKernelGGL(k_item_counts, dim3(gn), dim3(kBlock), 0, s,
-                       (const int32_t *)cnt.as<int32_t>(), n, chunk, nitems.as<int32_t>());
+                       (const int32_t *)cnt.as<int32_t>(), (const int32_t *)perm.as<int32_t>(),
+                       n_cent, n, chunk, nitems.as<int32_t>());
     TC_TRY(exclusive_scan<int32_t>(nitems.as<int32_t>(), iptr.as<int64_t>(), n, nullptr, s));
     hipLaunchKernelGGL(k_item_fill, dim3(gn), dim3(kBlock), 0, s,
-                       (const int32_t *)nitems.as<int32_t>(), (const int64_t *)iptr.as<int64_t>(),
-                       n, items.as<int2>());
+                       (const int32_t *)nitems.as<int32_t>(), (const int32_t *)perm.as<int32_t>(),
+                       (const int64_t *)iptr.as<int64_t>(), n_cent, items.as<int2>());
     TC_HIP(hipGetLastError());
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
     {
@@ -408,7 +456,8 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
       hipLaunchKernelGGL(heavy_kernel<I>(unroll), dim3(gh), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)cptr.as<int64_t>(), (const int32_t *)others.as<int32_t>(),
                          (const int2 *)items.as<int2>(), (const int64_t *)(iptr.as<int64_t>() + n),
-                         chunk, ctr.as<unsigned long long>(), graph_of, count);
+                         (const int64_t *)iptr.as<int64_t>(), n_big_cent, chunk,
+                         ctr.as<unsigned long long>(), graph_of, count);
     }
     TC_HIP(hipGetLastError());
   });
