@@ -101,7 +101,14 @@ typedef struct tc_exact_params {
   int chunk;         /* others per (center, chunk) work item                   */
   int waves_per_cu;  /* persistent waves per CU for the heavy kernel           */
   int unroll;        /* streamed ids in flight per lane: 1, 2, 4 or 8          */
+  int plan;          /* TC_PLAN_SORTED (default) or TC_PLAN_ATOMIC             */
+  int reserved[3];
 } tc_exact_params;
+
+/* How heavy edges are grouped by center before the intersection kernel:
+ * a radix sort of packed (center, other) words, or an atomic counting sort. */
+#define TC_PLAN_SORTED 0
+#define TC_PLAN_ATOMIC 1
 
 /* count[0] += sum over the given oriented edges (edge_u[e] -> edge_v[e]) of
  * |N+(u) ∩ N+(v)|, with N+(x) = col_idx[row_ptr[x] : row_ptr[x+1]] sorted.

@@ -58,13 +58,16 @@ def test_rmat14_config(device):
     assert got == truth and secs > 0
 
 
-@pytest.mark.parametrize("light_max,chunk,wpc", [(0, 1, 4), (2, 7, 8), (16, 256, 16),
-                                                 (64, 32, 32), (10**6, 256, 16)])
-def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc):
+@pytest.mark.parametrize("plan", ["sorted", "atomic"])
+@pytest.mark.parametrize("light_max,chunk,wpc,unroll", [(0, 1, 4, 1), (2, 7, 8, 2),
+                                                        (16, 256, 16, 4), (64, 32, 32, 8),
+                                                        (10**6, 256, 16, 4)])
+def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc, unroll, plan):
     pairs = rmat(12, 16, seed=7)
     truth = _truth(pairs)
     csr = gpu_graph.build_csr_gpu(pairs)
-    params = {"light_max": light_max, "chunk": chunk, "waves_per_cu": wpc}
+    params = {"light_max": light_max, "chunk": chunk, "waves_per_cu": wpc, "unroll": unroll,
+              "plan": plan}
     assert _count(csr, params=params) == truth
 
 

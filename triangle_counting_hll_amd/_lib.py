@@ -58,7 +58,11 @@ class ExactParams(ctypes.Structure):
     """``tc_exact_params`` of include/tchll.h (0 = library default)."""
 
     _fields_ = [("light_max", ctypes.c_int), ("chunk", ctypes.c_int),
-                ("waves_per_cu", ctypes.c_int), ("unroll", ctypes.c_int)]
+                ("waves_per_cu", ctypes.c_int), ("unroll", ctypes.c_int),
+                ("plan", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+
+
+PLANS = {"sorted": 0, "atomic": 1}
 
 
 _lock = threading.Lock()

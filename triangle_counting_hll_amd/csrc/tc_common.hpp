@@ -171,6 +171,12 @@ int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t *d_total, hipSt
 int bucket_sort_u8(const uint8_t *key, int64_t n, int nbuckets, int32_t *perm, int64_t *starts,
                    hipStream_t s);
 
+// Stable LSD radix sort of 64-bit words by their bits [bit_lo, bit_hi), 8 bits
+// per pass.  `a` holds the input; `b` is a buffer of the same size.  On return
+// *result points at whichever of the two holds the sorted words.
+int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
+                   uint64_t **result, hipStream_t s);
+
 // Stream compaction by flag: idx_out receives the indices i with flag[i] != 0 in
 // increasing order; *d_count (device) receives their number.
 int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d_count,

@@ -216,11 +216,29 @@ __device__ __forceinline__ uint32_t global_probe(const int32_t *__restrict__ col
   return (lo < cb + dc && col[lo] == x) ? 1u : 0u;
 }
 
+// The "others" of a work item, in either planning layout: a plain id array
+// indexed through per-center offsets (item.x = center), or the sorted packed
+// (center << 32 | other) words indexed through run offsets (item.x = run).
+struct OthersIds {
+  const int32_t *p;
+  __device__ __forceinline__ int32_t operator()(int64_t k) const { return p[k]; }
+  __device__ __forceinline__ int32_t center(int item_x, int64_t) const { return item_x; }
+};
+struct OthersPacked {
+  const uint64_t *p;
+  __device__ __forceinline__ int32_t operator()(int64_t k) const {
+    return (int32_t)(uint32_t)p[k];
+  }
+  __device__ __forceinline__ int32_t center(int, int64_t ob) const {
+    return (int32_t)(p[ob] >> 32);
+  }
+};
+
 // Streams the rows of others[ob:oe] as one flattened sequence, U ids
 // per lane in flight, and counts the ids found in the center's set.
-template <class I, bool kLds, int U>
+template <class I, bool kLds, int U, class O>
 __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__restrict__ col,
-                                                  const int32_t *__restrict__ others, int64_t ob,
+                                                  O others, int64_t ob,
                                                   int64_t oe, int *__restrict__ seg_end,
                                                   int64_t *__restrict__ seg_base,
                                                   const int4 *__restrict__ tbl, int shift,
@@ -232,7 +250,7 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
     int len = 0;
     int64_t st = 0;
     if (k < oe) {
-      const int32_t o = others[k];
+      const int32_t o = others(k);
       st = R.begin(o);
       len = (int)(R.end(o) - st);
     }
@@ -277,10 +295,10 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
   return acc;
 }
 
-template <class I, int U>
+template <class I, int U, class O>
 __global__ void __launch_bounds__(kBlock)
 k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
-        const int32_t *__restrict__ others, const int2 *__restrict__ items,
+        O others, const int2 *__restrict__ items,
         const int64_t *__restrict__ n_items_p, const int64_t *__restrict__ iptr,
         const int64_t *__restrict__ n_big_centers_p, int chunk,
         unsigned long long *__restrict__ next_item, const int32_t *__restrict__ graph_of,
@@ -315,9 +333,9 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
       if (it >= n_items) break;
     }
     const int2 item = items[it++];
-    const int32_t c = item.x;
-    const int64_t ob = cptr[c] + (int64_t)item.y * chunk;
-    const int64_t oe = min(cptr[c + 1], ob + (int64_t)chunk);
+    const int64_t ob = cptr[item.x] + (int64_t)item.y * chunk;
+    const int64_t oe = min(cptr[item.x + 1], ob + (int64_t)chunk);
+    const int32_t c = others.center(item.x, ob);
     const int64_t cb = R.begin(c);
     const int dc = (int)(R.end(c) - cb);
     uint64_t acc;
@@ -326,9 +344,10 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
       uint32_t mask;
       table_build(reinterpret_cast<int *>(tbl), col + cb, dc, shift, mask);
       acc = stream_others<I, true, U>(R, col, others, ob, oe, seg_end, seg_base, tbl, shift, mask,
-                                   cb, dc);
+                                      cb, dc);
     } else {
-      acc = stream_others<I, false, U>(R, col, others, ob, oe, seg_end, seg_base, tbl, 0, 0, cb, dc);
+      acc = stream_others<I, false, U>(R, col, others, ob, oe, seg_end, seg_base, tbl, 0, 0, cb,
+                                       dc);
     }
     if (graph_of) {
       acc = wave_sum(acc);
@@ -370,14 +389,199 @@ __global__ void k_edge_supports(Rows<I> R, const int32_t *__restrict__ col,
   }
 }
 
-template <class I>
-static auto heavy_kernel(int unroll) -> decltype(&k_heavy<I, 4>) {
+template <class I, class O>
+static auto heavy_kernel(int unroll) -> decltype(&k_heavy<I, 4, O>) {
   switch (unroll) {
-    case 1: return &k_heavy<I, 1>;
-    case 2: return &k_heavy<I, 2>;
-    case 8: return &k_heavy<I, 8>;
-    default: return &k_heavy<I, 4>;
+    case 1: return &k_heavy<I, 1, O>;
+    case 2: return &k_heavy<I, 2, O>;
+    case 8: return &k_heavy<I, 8, O>;
+    default: return &k_heavy<I, 4, O>;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Sorted plan: one packed word per edge -- (center << 32 | other) for a heavy
+// edge, (n << 32) for every other edge -- then a radix sort on the center
+// bits groups the others of each center (in edge order) and moves the rest
+// to the end.  No per-edge global atomics.
+// ---------------------------------------------------------------------------
+template <class I>
+__global__ void __launch_bounds__(kBlock)
+k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict__ eu,
+                const int32_t *__restrict__ ev, int64_t E, const uint8_t *__restrict__ skip,
+                int light_max, uint64_t sentinel, uint64_t *__restrict__ packed,
+                const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count,
+                unsigned long long *__restrict__ n_heavy) {
+  uint64_t acc = 0, nh = 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t word = sentinel;
+    if (!(skip && skip[e])) {
+      const int32_t u = eu[e], v = ev[e];
+      const int64_t ub = R.begin(u), ue = R.end(u), vb = R.begin(v), ve = R.end(v);
+      const int64_t du = ue - ub, dv = ve - vb;
+      if (du > 0 && dv > 0) {
+        if (du + dv <= light_max) {
+          const uint32_t c = merge_count(col, ub, ue, vb, ve);
+          if (graph_of) {
+            if (c) atomicAdd(&count[graph_of[u]], (unsigned long long)c);
+          } else {
+            acc += c;
+          }
+        } else {
+          const bool cv = dv >= du;
+          const uint32_t c = (uint32_t)(cv ? v : u), o = (uint32_t)(cv ? u : v);
+          word = ((uint64_t)c << 32) | o;
+          ++nh;
+        }
+      }
+    }
+    packed[e] = word;
+  }
+  nh = wave_sum(nh);
+  if (lane_id() == 0 && nh) atomicAdd(n_heavy, (unsigned long long)nh);
+  if (!graph_of) {
+    acc = wave_sum(acc);
+    if (lane_id() == 0 && acc) atomicAdd(count, (unsigned long long)acc);
+  }
+}
+
+__global__ void k_run_flags(const uint64_t *__restrict__ s, const unsigned long long *n_heavy_p,
+                            int64_t E, uint8_t *__restrict__ flag) {
+  const int64_t H = (int64_t)*n_heavy_p;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
+       i += (int64_t)gridDim.x * blockDim.x)
+    flag[i] = (i < H && (i == 0 || (s[i] >> 32) != (s[i - 1] >> 32))) ? 1 : 0;
+}
+
+// run offsets (int64, R + 1 entries) and the bucket key of each run's center
+template <class I>
+__global__ void k_run_offsets(Rows<I> R, const uint64_t *__restrict__ s,
+                              const int32_t *__restrict__ run_idx, const int64_t *__restrict__ R_p,
+                              const unsigned long long *__restrict__ n_heavy_p, int64_t max_runs,
+                              int64_t *__restrict__ run_ptr, uint8_t *__restrict__ key) {
+  const int64_t nr = *R_p;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= max_runs;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    if (r < nr) {
+      const int64_t b = run_idx[r];
+      run_ptr[r] = b;
+      const uint64_t d = (uint64_t)R.deg((int64_t)(s[b] >> 32));
+      key[r] = (uint8_t)(kCenterBuckets - (64 - __clzll((long long)d)));
+    } else {
+      if (r == nr) run_ptr[r] = (int64_t)*n_heavy_p;
+      if (r < max_runs) key[r] = 255;
+    }
+  }
+}
+
+__global__ void k_run_item_counts(const int64_t *__restrict__ run_ptr,
+                                  const int32_t *__restrict__ perm,
+                                  const int64_t *__restrict__ n_runs_p, int64_t max_runs, int chunk,
+                                  int32_t *__restrict__ nitems) {
+  const int64_t nr = *n_runs_p;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < max_runs;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int k = 0;
+    if (i < nr) {
+      const int32_t r = perm[i];
+      k = (int)((run_ptr[r + 1] - run_ptr[r] + chunk - 1) / chunk);
+    }
+    nitems[i] = k;
+  }
+}
+
+static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t *col, int64_t n,
+                             const int32_t *eu, const int32_t *ev, int64_t E,
+                             const uint8_t *skip, const int32_t *graph_of,
+                             unsigned long long *count, int light_max, int chunk, int wpc,
+                             int unroll, hipStream_t s) {
+  Scratch eu_buf;
+  if (!eu) {
+    TC_TRY(eu_buf.alloc(sizeof(int32_t) * E, s));
+    TC_DISPATCH_ROWS(row_ptr_64, I, {
+      hipLaunchKernelGGL(k_expand_rows<I>, dim3(grid_for(n * kWave, kBlock, 65536)),
+                         dim3(kBlock), 0, s, (const I *)row_ptr, n, eu_buf.as<int32_t>());
+    });
+    TC_HIP(hipGetLastError());
+    eu = eu_buf.as<int32_t>();
+    ev = col;
+  }
+  int key_bits = 1;
+  while ((int64_t(1) << key_bits) <= n) ++key_bits;  // center ids and the sentinel n
+  const int64_t max_runs = n;                         // at most one run per center
+  Scratch packed, tmp, flags, run_idx, n_runs, run_ptr, rkey, perm, starts, nitems, iptr,
+      items, ctr;
+  TC_TRY(packed.alloc(sizeof(uint64_t) * E, s));
+  TC_TRY(tmp.alloc(sizeof(uint64_t) * E, s));
+  TC_TRY(flags.alloc(E, s));
+  TC_TRY(run_idx.alloc(sizeof(int32_t) * max_runs, s));
+  TC_TRY(n_runs.alloc(sizeof(int64_t), s));
+  TC_TRY(run_ptr.alloc(sizeof(int64_t) * (max_runs + 1), s));
+  TC_TRY(rkey.alloc(max_runs, s));
+  TC_TRY(perm.alloc(sizeof(int32_t) * max_runs, s));
+  TC_TRY(starts.alloc(sizeof(int64_t) * (kCenterBuckets + 1), s));
+  TC_TRY(nitems.alloc(sizeof(int32_t) * max_runs, s));
+  TC_TRY(iptr.alloc(sizeof(int64_t) * (max_runs + 1), s));
+  const int64_t max_items = max_runs + E / chunk + 1;
+  TC_TRY(items.alloc(sizeof(int2) * max_items, s));
+  TC_TRY(ctr.alloc(2 * sizeof(unsigned long long), s));
+  unsigned long long *next_item = ctr.as<unsigned long long>();
+  unsigned long long *n_heavy = next_item + 1;
+
+  const unsigned gr = grid_for(max_runs + 1, kBlock, 16384);
+  const unsigned ge = grid_for(E, kBlock, 16384);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, next_item, 2, 0ull);
+  TC_DISPATCH_ROWS(row_ptr_64, I, {
+    Rows<I> R{(const I *)row_ptr};
+    {
+      StageTimer _timer(TC_STAGE_CLASSIFY, s);
+      hipLaunchKernelGGL(k_classify_pack<I>, dim3(ge), dim3(kBlock), 0, s, R, col, eu, ev, E,
+                         skip, light_max, (uint64_t)n << 32, packed.as<uint64_t>(), graph_of,
+                         count, n_heavy);
+    }
+    TC_HIP(hipGetLastError());
+    uint64_t *sorted = nullptr;
+    TC_TRY(radix_sort_u64(packed.as<uint64_t>(), tmp.as<uint64_t>(), E, 32, 32 + key_bits,
+                          &sorted, s));
+    hipLaunchKernelGGL(k_run_flags, dim3(ge), dim3(kBlock), 0, s, (const uint64_t *)sorted,
+                       (const unsigned long long *)n_heavy, E, flags.as<uint8_t>());
+    TC_HIP(hipGetLastError());
+    TC_TRY(compact_indices(flags.as<uint8_t>(), E, run_idx.as<int32_t>(), n_runs.as<int64_t>(),
+                           s));
+    hipLaunchKernelGGL(k_run_offsets<I>, dim3(gr), dim3(kBlock), 0, s, R,
+                       (const uint64_t *)sorted, (const int32_t *)run_idx.as<int32_t>(),
+                       (const int64_t *)n_runs.as<int64_t>(), (const unsigned long long *)n_heavy,
+                       max_runs, run_ptr.as<int64_t>(), rkey.as<uint8_t>());
+    TC_HIP(hipGetLastError());
+    // runs ordered longest center row first
+    TC_TRY(bucket_sort_u8(rkey.as<uint8_t>(), max_runs, kCenterBuckets, perm.as<int32_t>(),
+                          starts.as<int64_t>(), s));
+    const int64_t *n_sorted_runs = starts.as<int64_t>() + kCenterBuckets;
+    const int64_t *n_big_runs = starts.as<int64_t>() + (kCenterBuckets - kBigCenterBits + 1);
+    hipLaunchKernelGGL(k_run_item_counts, dim3(gr), dim3(kBlock), 0, s,
+                       (const int64_t *)run_ptr.as<int64_t>(), (const int32_t *)perm.as<int32_t>(),
+                       n_sorted_runs, max_runs, chunk, nitems.as<int32_t>());
+    TC_TRY(exclusive_scan<int32_t>(nitems.as<int32_t>(), iptr.as<int64_t>(), max_runs, nullptr,
+                                   s));
+    hipLaunchKernelGGL(k_item_fill, dim3(gr), dim3(kBlock), 0, s,
+                       (const int32_t *)nitems.as<int32_t>(), (const int32_t *)perm.as<int32_t>(),
+                       (const int64_t *)iptr.as<int64_t>(), n_sorted_runs, items.as<int2>());
+    TC_HIP(hipGetLastError());
+    const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
+    {
+      StageTimer _timer(TC_STAGE_HEAVY, s);
+      const auto kfn = heavy_kernel<I, OthersPacked>(unroll);
+      hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R,
+                         col, (const int64_t *)run_ptr.as<int64_t>(),
+                         OthersPacked{(const uint64_t *)sorted}, (const int2 *)items.as<int2>(),
+                         (const int64_t *)(iptr.as<int64_t>() + max_runs),
+                         (const int64_t *)iptr.as<int64_t>(), n_big_runs, chunk, next_item,
+                         graph_of, count);
+    }
+    TC_HIP(hipGetLastError());
+  });
+  return TC_OK;
 }
 
 int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, int64_t n,
@@ -385,10 +589,14 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                      const int32_t *graph_of, unsigned long long *count,
                      const tc_exact_params *params, hipStream_t s) {
   const int light_max = (params && params->light_max > 0) ? params->light_max : 32;
-  const int chunk = (params && params->chunk > 0) ? params->chunk : 1024;
+  const int chunk = (params && params->chunk > 0) ? params->chunk : 256;
   const int wpc = (params && params->waves_per_cu > 0) ? params->waves_per_cu : 16;
   const int unroll = (params && params->unroll > 0) ? params->unroll : kDefaultUnroll;
+  const int plan = params ? params->plan : TC_PLAN_SORTED;
   if (E <= 0 || n <= 0) return TC_OK;
+  if (plan == TC_PLAN_SORTED)
+    return exact_sorted_plan(row_ptr, row_ptr_64, col, n, eu, ev, E, skip, graph_of, count,
+                             light_max, chunk, wpc, unroll, s);
 
   Scratch eu_buf;
   if (!eu) {
@@ -453,8 +661,10 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
-      hipLaunchKernelGGL(heavy_kernel<I>(unroll), dim3(gh), dim3(kBlock), 0, s, R, col,
-                         (const int64_t *)cptr.as<int64_t>(), (const int32_t *)others.as<int32_t>(),
+      const auto kfn = heavy_kernel<I, OthersIds>(unroll);
+      hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R, col,
+                         (const int64_t *)cptr.as<int64_t>(),
+                         OthersIds{(const int32_t *)others.as<int32_t>()},
                          (const int2 *)items.as<int2>(), (const int64_t *)(iptr.as<int64_t>() + n),
                          (const int64_t *)iptr.as<int64_t>(), n_big_cent, chunk,
                          ctr.as<unsigned long long>(), graph_of, count);
@@ -477,6 +687,7 @@ extern "C" int tc_exact_count(const void *row_ptr, int row_ptr_64, const int32_t
            "edge_u and edge_v must both be given or both be NULL");
   TC_CHECK(n_edges == 0 || col_idx, TC_ERR_INVALID, "col_idx is required");
   TC_CHECK(n_nodes < INT32_MAX, TC_ERR_INVALID, "node ids must fit int32");
+  TC_CHECK(n_edges < INT32_MAX, TC_ERR_INVALID, "edge positions must fit int32");
   TC_TRY(ensure_init());
   return exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges,
                           nullptr, nullptr, (unsigned long long *)count, params,

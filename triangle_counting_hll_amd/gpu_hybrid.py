@@ -56,8 +56,10 @@ def _warn_if_float64_atomic_slow(cuda_module=None) -> None:
 def _exact_params(params: Optional[dict]):
     if not params:
         return None
+    plan = params.get("plan", "sorted")
     return _lib.ExactParams(int(params.get("light_max", 0)), int(params.get("chunk", 0)),
-                            int(params.get("waves_per_cu", 0)), int(params.get("unroll", 0)))
+                            int(params.get("waves_per_cu", 0)), int(params.get("unroll", 0)),
+                            _lib.PLANS[plan] if isinstance(plan, str) else int(plan))
 
 
 def exact_count(row_ptr, col_idx, edge_u=None, edge_v=None, count=None, params=None,
