@@ -77,6 +77,108 @@ def cpu_baseline(rp, col, truth_hint=None, budget_s=20.0):
             "sample": f"every {stride}th oriented edge ({n_s} of {E}), u-major order"}
 
 
+def _timed(fn, reps):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return out, (time.perf_counter() - t0) * 1e3 / reps
+
+
+def run_hybrid_extra(csr, truth, reps=3):
+    """Fixed-threshold hybrid on the same graph (src/gpu_hybrid.py:357-431 rule).
+
+    HLL p = 12, sketches for forward degree >= theta = gamma (the reference
+    experiment passes theta = gamma, experiment/exp_gpu.py:556-557), compact
+    register slots.  gamma = 1024 is the configured threshold; gamma = 128 is
+    also reported because no forward degree of this graph reaches 1024.
+    """
+    import torch
+
+    from triangle_counting_hll_amd import _lib, gpu_hll, gpu_hybrid
+
+    out = {}
+    p = 12
+    deg = csr.degrees
+    for gamma in (1024, 128):
+        sk = deg >= gamma
+        n_sk = int(sk.sum().item())
+        slot = torch.full_like(deg, -1)
+        slot[sk] = torch.arange(n_sk, dtype=torch.int32, device=deg.device)
+        regs = torch.zeros(max(n_sk, 1) << p, dtype=torch.uint8, device=deg.device)
+        has = torch.zeros_like(deg, dtype=torch.uint8)
+
+        def build():
+            gpu_hll.build_sketches(csr.row_ptr, csr.col_idx, csr.n_nodes, gamma, p, regs, has,
+                                   slot=slot, accumulate=False)
+
+        _, build_ms = _timed(build, reps)
+
+        def count():
+            c, s, fb = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
+                                               csr.degrees, regs, has, gamma, p, slot=slot)
+            return c, s
+
+        (c, s), ms = _timed(count, reps)
+        est = float(c.item())
+        st = s.cpu().numpy()
+        out[f"gamma{gamma}"] = {
+            "p": p, "sketched_vertices": n_sk, "sketch_build_ms": build_ms, "count_ms": ms,
+            "estimate": est, "rel_err": gpu_hybrid.relative_error(est, truth),
+            "n_approx": int(st[0]), "n_exact": int(st[1]), "n_clamped": int(st[2]),
+            "approx_coverage": int(st[0]) / max(csr.n_oriented, 1),
+        }
+    return out
+
+
+def run_batch_extra(args, reps=10):
+    """4,096 graphs (half ER, half R-MAT, seed 3) in one HIP-graph launch vs per-graph calls."""
+    import torch
+
+    from oracle import cbind
+    from triangle_counting_hll_amd import autotune, batch
+
+    spec = batch.many_graphs_spec()
+    t0 = time.perf_counter()
+    b = batch.build_batch(spec)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    E = b.csr.n_oriented
+    key = f"batch|G={b.n_graphs}|n={b.csr.n_nodes}|E={E}"
+    params, table = autotune.tune(lambda p: batch.batch_exact_count(b, p), key, reps=2)
+    g = batch.BatchCountGraph(b, params)
+    _, ms = _timed(g.launch, reps)
+    counts = g.counts.cpu().numpy()
+    plain, plain_s = batch.plain_per_graph_counts(b, params)
+    # CPU oracle per graph on the same packed arrays
+    rp = b.csr.row_ptr.cpu().numpy().astype(np.int64)
+    col = b.csr.col_idx.cpu().numpy()
+    eu = b.csr.edge_u.cpu().numpy()
+    t0 = time.perf_counter()
+    cpu_counts = np.array([cbind.exact_count(rp, col, eu[b.edge_off[i]:b.edge_off[i + 1]],
+                                             col[b.edge_off[i]:b.edge_off[i + 1]])
+                           for i in range(b.n_graphs)], dtype=np.int64)
+    cpu_s = time.perf_counter() - t0
+    g.close()
+    return {
+        "graphs": b.n_graphs, "n_nodes": b.csr.n_nodes, "oriented_edges": E,
+        "raw_pairs": int(spec.pair_off[-1]), "build_s": build_s,
+        "ms_per_launch": ms, "ms_per_graph": ms / b.n_graphs,
+        "oriented_edges_per_s": E / (ms * 1e-3),
+        "plain_ms_total": plain_s * 1e3, "plain_ms_per_graph": plain_s * 1e3 / b.n_graphs,
+        "speedup_vs_plain": plain_s * 1e3 / ms,
+        "cpu_oracle_s": cpu_s, "cpu_threads": len(os.sched_getaffinity(0)),
+        "parity_batched_vs_cpu": bool(np.array_equal(counts, cpu_counts)),
+        "parity_plain_vs_cpu": bool(np.array_equal(plain, cpu_counts)),
+        "tuned_params": params, "autotune_evals": len(table),
+        "workspace_bytes": g.workspace_bytes,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,9 +189,11 @@ def main():
     ap.add_argument("--scale-down", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--light-max", type=int, default=32)
-    ap.add_argument("--chunk", type=int, default=1024)
+    ap.add_argument("--chunk", type=int, default=256)
     ap.add_argument("--waves-per-cu", type=int, default=16)
-    ap.add_argument("--unroll", type=int, default=4)
+    ap.add_argument("--unroll", type=int, default=8)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the hybrid and many-graph measurements added to the line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (reference point for "
                          "HBM byte counters under a profiler)")
@@ -177,6 +281,18 @@ def main():
         else:
             parity = cbind.exact_count(rp, col) == gpu_count
 
+    extras = {}
+    if not args.no_extras and not args.scale_down:
+        for name, fn in (("hybrid_fixed_threshold", lambda: run_hybrid_extra(csr, gpu_count)),
+                         ("batch4096", lambda: run_batch_extra(args))):
+            try:
+                t0 = time.perf_counter()
+                extras[name] = fn()
+                _log(f"{name}: {extras[name]} ({time.perf_counter() - t0:.1f}s)")
+            except Exception as exc:  # reported, never silently dropped
+                extras[name] = {"error": f"{type(exc).__name__}: {exc}"}
+                _log(f"{name} failed: {exc}")
+
     line = {
         "metric": "oriented edges/sec (exact triangle count)",
         "value": E / (ms_per_step * 1e-3),
@@ -214,6 +330,7 @@ def main():
         "csr_build_ms": t_csr * 1e3,
         "cpu_baseline": cpu,
         "device": info["name"],
+        "extras": extras,
     }
     print(json.dumps(line), flush=True)
 

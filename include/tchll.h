@@ -183,6 +183,48 @@ int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
                     const uint8_t *route, int part, double *counts, int64_t *stats,
                     int64_t *fallback, double *per_edge, void *stream);
 
+/* ---- batched counting of many graphs ------------------------------------- */
+/* No reference counterpart: the reference counts one graph per process
+ * (experiment/exp_gpu.py:632-935).  Graphs are packed into one CSR over
+ * disjoint id ranges (built by tc_csr_build_* from the concatenated pairs). */
+
+/* graph_of[v] = the g with id_off[g] <= node_ids[v] < id_off[g+1]
+ * (id_off: int64 [n_graphs + 1], ascending). */
+int tc_vertex_graph_ids(const int64_t *node_ids, int64_t n_nodes, const int64_t *id_off,
+                        int n_graphs, int32_t *graph_of, void *stream);
+
+/* counts[g] += triangles of graph g (uint64 [n_graphs]); same pipeline as
+ * tc_exact_count, each triangle credited to graph_of[center]. */
+int tc_batch_exact_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                         int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                         int64_t n_edges, const int32_t *graph_of, int n_graphs,
+                         uint64_t *counts, const tc_exact_params *params, void *stream);
+
+/* Captures the batched pipeline (clear counts, then tc_batch_exact_count)
+ * into one HIP graph over a workspace allocated here; the arrays passed are
+ * bound into the graph.  Runs the pipeline once eagerly to size the
+ * workspace (host output *workspace_bytes, may be NULL).  Synchronises. */
+typedef struct tc_batch_graph tc_batch_graph;
+int tc_batch_graph_create(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                          int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                          int64_t n_edges, const int32_t *graph_of, int n_graphs,
+                          uint64_t *counts, const tc_exact_params *params, void *stream,
+                          tc_batch_graph **out, int64_t *workspace_bytes);
+/* One launch of the captured graph on `stream`: counts[] = per-graph counts. */
+int tc_batch_graph_launch(tc_batch_graph *g, void *stream);
+void tc_batch_graph_free(tc_batch_graph *g);
+
+/* Raw pairs of many R-MAT / Erdos-Renyi graphs from the counter-based stream
+ * draw(key, i) = splitmix64(key + (i + 1) * 0x9E3779B97F4A7C15).  Graph g:
+ * kind[g] 0 = ER with param[g] = n, 1 = R-MAT with param[g] = scale and
+ * quadrant thresholds t_a <= t_ab <= t_abc on 32-bit uniforms; its pairs fill
+ * pairs[pair_off[g] .. pair_off[g+1]) (int32 [2] each) with ids offset by
+ * id_off[g].  No reference counterpart (the reference reads SNAP files). */
+int tc_generate_pairs(const uint8_t *kind, const int32_t *param, const int64_t *id_off,
+                      const int64_t *pair_off, const uint64_t *key, int n_graphs, uint32_t t_a,
+                      uint32_t t_ab, uint32_t t_abc, int32_t *pairs, int64_t n_pairs,
+                      void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,13 @@ _SIGNATURES = {
     "tc_union_estimate": (I32, [P, P, I32, P, P, I64, P, P]),
     "tc_hybrid_count": (I32, [P, I32, P, I64, P, P, I64, P, P, P, P, I64, I32, P, I32, P, P, P, P,
                               P]),
+    "tc_vertex_graph_ids": (I32, [P, I64, P, I32, P, P]),
+    "tc_batch_exact_count": (I32, [P, I32, P, I64, P, P, I64, P, I32, P, P, P]),
+    "tc_batch_graph_create": (I32, [P, I32, P, I64, P, P, I64, P, I32, P, P, P, P, P]),
+    "tc_batch_graph_launch": (I32, [P, P]),
+    "tc_batch_graph_free": (None, [P]),
+    "tc_generate_pairs": (I32, [P, P, P, P, P, I32, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_uint32, P, I64, P]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -1,0 +1,171 @@
+// Batched exact counting of many graphs packed into one CSR.
+//
+// The graphs occupy disjoint id ranges, so one device CSR build over the
+// concatenated pairs keeps every graph's vertices contiguous, and one exact
+// pipeline counts them all; triangles are credited to the graph of their
+// center vertex.  For repeated counts the whole pipeline (fill, classify,
+// radix plan, work items, intersection kernel) is captured once into a HIP
+// graph over a preallocated workspace and replayed with one launch.
+#include <algorithm>
+
+#include "tc_common.hpp"
+#include "tc_exact.hpp"
+
+namespace tc {
+
+__global__ void k_graph_of(const int64_t *__restrict__ node_ids, int64_t n,
+                           const int64_t *__restrict__ id_off, int G,
+                           int32_t *__restrict__ graph_of) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t id = node_ids[v];
+    int lo = 0, hi = G;  // id_off[lo] <= id < id_off[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (id_off[mid] <= id) lo = mid; else hi = mid;
+    }
+    graph_of[v] = lo;
+  }
+}
+
+}  // namespace tc
+
+struct tc_batch_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void *workspace = nullptr;
+  size_t workspace_bytes = 0;
+  ~tc_batch_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (workspace) (void)hipFree(workspace);
+  }
+};
+
+extern "C" int tc_vertex_graph_ids(const int64_t *node_ids, int64_t n_nodes, const int64_t *id_off,
+                                   int n_graphs, int32_t *graph_of, void *stream) {
+  using namespace tc;
+  TC_CHECK(n_nodes >= 0 && n_graphs > 0 && id_off && (n_nodes == 0 || (node_ids && graph_of)),
+           TC_ERR_INVALID, "invalid arguments to tc_vertex_graph_ids");
+  TC_TRY(ensure_init());
+  if (n_nodes == 0) return TC_OK;
+  hipLaunchKernelGGL(k_graph_of, dim3(grid_for(n_nodes, kBlock, 16384)), dim3(kBlock), 0,
+                     (hipStream_t)stream, node_ids, n_nodes, id_off, n_graphs, graph_of);
+  TC_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+static int check_batch_args(const void *row_ptr, const int32_t *col_idx, int64_t n_nodes,
+                            const int32_t *edge_u, const int32_t *edge_v, int64_t n_edges,
+                            const int32_t *graph_of, int n_graphs, uint64_t *counts) {
+  using namespace tc;
+  TC_CHECK(row_ptr && graph_of && counts && n_graphs > 0 && n_nodes >= 0 && n_edges >= 0,
+           TC_ERR_INVALID, "invalid arguments to the batched count");
+  TC_CHECK(n_edges == 0 || (col_idx && edge_u && edge_v), TC_ERR_INVALID,
+           "col_idx, edge_u and edge_v are required");
+  TC_CHECK(n_nodes < INT32_MAX && n_edges < INT32_MAX, TC_ERR_INVALID,
+           "node ids and edge positions must fit int32");
+  return TC_OK;
+}
+
+extern "C" int tc_batch_exact_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                                    int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                                    int64_t n_edges, const int32_t *graph_of, int n_graphs,
+                                    uint64_t *counts, const tc_exact_params *params,
+                                    void *stream) {
+  using namespace tc;
+  TC_TRY(check_batch_args(row_ptr, col_idx, n_nodes, edge_u, edge_v, n_edges, graph_of, n_graphs,
+                          counts));
+  TC_TRY(ensure_init());
+  return exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges, nullptr,
+                          graph_of, (unsigned long long *)counts, params, (hipStream_t)stream);
+}
+
+extern "C" int tc_batch_graph_create(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                                     int64_t n_nodes, const int32_t *edge_u,
+                                     const int32_t *edge_v, int64_t n_edges,
+                                     const int32_t *graph_of, int n_graphs, uint64_t *counts,
+                                     const tc_exact_params *params, void *stream,
+                                     tc_batch_graph **out, int64_t *workspace_bytes) {
+  using namespace tc;
+  TC_CHECK(out, TC_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  TC_TRY(check_batch_args(row_ptr, col_idx, n_nodes, edge_u, edge_v, n_edges, graph_of, n_graphs,
+                          counts));
+  TC_TRY(ensure_init());
+  hipStream_t s = (hipStream_t)stream;
+  auto *h = new tc_batch_graph();
+  auto fail = [&](int rc) {
+    current_arena() = nullptr;
+    delete h;
+    return rc;
+  };
+
+  // 1. eager run that measures the workspace (its counts are overwritten by
+  //    the first replay, which starts by clearing them)
+  Arena measure;
+  current_arena() = &measure;
+  int rc = exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges,
+                            nullptr, graph_of, (unsigned long long *)counts, params, s);
+  current_arena() = nullptr;
+  if (rc) return fail(rc);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error(TC_ERR_HIP, "measuring run failed: %s", hipGetErrorString(hipGetLastError()));
+    return fail(TC_ERR_HIP);
+  }
+  h->workspace_bytes = std::max<size_t>(measure.total, 256);
+  if (hipMalloc(&h->workspace, h->workspace_bytes) != hipSuccess) {
+    set_error(TC_ERR_OOM, "cannot allocate a %zu-byte graph workspace", h->workspace_bytes);
+    return fail(TC_ERR_OOM);
+  }
+
+  // 2. capture the same allocation sequence carved from the workspace
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+    set_error(TC_ERR_HIP, "cannot create the capture stream");
+    return fail(TC_ERR_HIP);
+  }
+  Arena carve;
+  carve.carve = true;
+  carve.base = (char *)h->workspace;
+  carve.cap = h->workspace_bytes;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    set_error(TC_ERR_HIP, "hipStreamBeginCapture failed");
+    return fail(TC_ERR_HIP);
+  }
+  current_arena() = &carve;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n_graphs, kBlock, 1024)), dim3(kBlock), 0, cs,
+                     (unsigned long long *)counts, (int64_t)n_graphs, 0ull);
+  rc = exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges, nullptr,
+                        graph_of, (unsigned long long *)counts, params, cs);
+  current_arena() = nullptr;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(cs, &g);
+  (void)hipStreamDestroy(cs);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(rc);
+  }
+  if (ec != hipSuccess || !g) {
+    set_error(TC_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+    return fail(TC_ERR_HIP);
+  }
+  h->graph = g;
+  if (hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    set_error(TC_ERR_HIP, "hipGraphInstantiate failed");
+    return fail(TC_ERR_HIP);
+  }
+  if (workspace_bytes) *workspace_bytes = (int64_t)h->workspace_bytes;
+  *out = h;
+  return TC_OK;
+}
+
+extern "C" int tc_batch_graph_launch(tc_batch_graph *h, void *stream) {
+  using namespace tc;
+  TC_CHECK(h && h->exec, TC_ERR_INVALID, "invalid batch graph handle");
+  TC_HIP(hipGraphLaunch(h->exec, (hipStream_t)stream));
+  return TC_OK;
+}
+
+extern "C" void tc_batch_graph_free(tc_batch_graph *h) { delete h; }

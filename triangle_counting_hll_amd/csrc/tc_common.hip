@@ -26,7 +26,12 @@ struct TimedLaunch {
 std::vector<TimedLaunch> g_timed;
 }  // namespace
 
-bool timing_on() { return g_timing; }
+Arena *&current_arena() {
+  thread_local Arena *a = nullptr;
+  return a;
+}
+
+bool timing_on() { return g_timing && current_arena() == nullptr; }
 
 void timing_record(int stage, hipEvent_t a, hipEvent_t b) {
   std::lock_guard<std::mutex> lk(g_timing_mu);

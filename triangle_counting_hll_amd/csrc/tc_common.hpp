@@ -47,21 +47,51 @@ int last_code();
     if (_rc != TC_OK) return _rc;                                               \
   } while (0)
 
+// Workspace arena for graph capture.  While an arena is installed on the
+// calling thread, Scratch::alloc either measures (allocates normally and adds
+// the rounded size to `total`) or carves consecutive pieces of `base`
+// (no allocation, no free), so a pipeline whose allocation sequence depends
+// only on its sizes can be replayed from one preallocated workspace.
+struct Arena {
+  bool carve = false;
+  char *base = nullptr;
+  size_t cap = 0;
+  size_t total = 0;
+};
+Arena *&current_arena();
+inline size_t arena_round(size_t b) { return (b + 255) & ~size_t(255); }
+
 // Stream-ordered scratch buffer from the device's default memory pool.  The
 // pool keeps freed blocks (release threshold raised at init), so repeated
 // calls do not go back to the driver.
 struct Scratch {
   void *ptr = nullptr;
   hipStream_t stream = nullptr;
+  bool owned = false;
   Scratch() = default;
   Scratch(const Scratch &) = delete;
   Scratch &operator=(const Scratch &) = delete;
   ~Scratch() {
-    if (ptr) (void)hipFreeAsync(ptr, stream);
+    if (ptr && owned) (void)hipFreeAsync(ptr, stream);
   }
   int alloc(size_t bytes, hipStream_t s) {
     stream = s;
     if (bytes == 0) bytes = 256;
+    Arena *ar = current_arena();
+    if (ar && ar->carve) {
+      const size_t need = arena_round(bytes);
+      if (ar->total + need > ar->cap) {
+        set_error(TC_ERR_OOM, "graph workspace exhausted (%zu of %zu bytes)", ar->total + need,
+                  ar->cap);
+        return TC_ERR_OOM;
+      }
+      ptr = ar->base + ar->total;
+      ar->total += need;
+      owned = false;
+      return TC_OK;
+    }
+    if (ar) ar->total += arena_round(bytes);
+    owned = true;
     hipError_t e = hipMallocAsync(&ptr, bytes, s);
     if (e != hipSuccess) {
       ptr = nullptr;
@@ -75,6 +105,7 @@ struct Scratch {
 };
 
 // Stage timing (see tc_timing_enable): RAII pair of events around a launch.
+// Off while a pipeline is being captured into a graph.
 bool timing_on();
 void timing_record(int stage, hipEvent_t a, hipEvent_t b);
 struct StageTimer {

@@ -25,7 +25,7 @@ namespace tc {
 
 constexpr int kTblInts = 2048;    // per-wave LDS hash table: 512 buckets x 4 slots (8 KiB)
 constexpr int kTblMaxDeg = 1024;  // longer center rows use the global binary-search path
-constexpr int kDefaultUnroll = 4; // streamed ids in flight per lane
+constexpr int kDefaultUnroll = 8; // streamed ids in flight per lane
 constexpr int kWavesPerBlock = kBlock / kWave;
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -413,30 +413,50 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
                 const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count,
                 unsigned long long *__restrict__ n_heavy) {
   uint64_t acc = 0, nh = 0;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t word = sentinel;
-    if (!(skip && skip[e])) {
-      const int32_t u = eu[e], v = ev[e];
-      const int64_t ub = R.begin(u), ue = R.end(u), vb = R.begin(v), ve = R.end(v);
-      const int64_t du = ue - ub, dv = ve - vb;
-      if (du > 0 && dv > 0) {
-        if (du + dv <= light_max) {
-          const uint32_t c = merge_count(col, ub, ue, vb, ve);
-          if (graph_of) {
-            if (c) atomicAdd(&count[graph_of[u]], (unsigned long long)c);
+  const int lane = lane_id();
+  // the loop bound is uniform per wavefront, so every lane stays active for
+  // the cross-lane aggregation below
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < E;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = base + threadIdx.x;
+    uint32_t tri = 0;
+    int32_t u = 0;
+    if (e < E) {
+      uint64_t word = sentinel;
+      if (!(skip && skip[e])) {
+        const int32_t v = ev[e];
+        u = eu[e];
+        const int64_t ub = R.begin(u), ue = R.end(u), vb = R.begin(v), ve = R.end(v);
+        const int64_t du = ue - ub, dv = ve - vb;
+        if (du > 0 && dv > 0) {
+          if (du + dv <= light_max) {
+            tri = merge_count(col, ub, ue, vb, ve);
           } else {
-            acc += c;
+            const bool cv = dv >= du;
+            const uint32_t c = (uint32_t)(cv ? v : u), o = (uint32_t)(cv ? u : v);
+            word = ((uint64_t)c << 32) | o;
+            ++nh;
           }
-        } else {
-          const bool cv = dv >= du;
-          const uint32_t c = (uint32_t)(cv ? v : u), o = (uint32_t)(cv ? u : v);
-          word = ((uint64_t)c << 32) | o;
-          ++nh;
         }
       }
+      packed[e] = word;
     }
-    packed[e] = word;
+    if (graph_of) {
+      // one atomic per wavefront when all its triangles belong to one graph
+      const int g = tri ? graph_of[u] : -1;
+      const uint64_t has = __ballot(g >= 0);
+      if (has) {
+        const int g0 = __shfl(g, __ffsll((long long)has) - 1);
+        if (__ballot(g >= 0 && g != g0) == 0) {
+          const uint64_t sum = wave_sum((uint64_t)tri);
+          if (lane == 0) atomicAdd(&count[g0], (unsigned long long)sum);
+        } else if (g >= 0) {
+          atomicAdd(&count[g], (unsigned long long)tri);
+        }
+      }
+    } else {
+      acc += tri;
+    }
   }
   nh = wave_sum(nh);
   if (lane_id() == 0 && nh) atomicAdd(n_heavy, (unsigned long long)nh);

@@ -8,8 +8,11 @@ both orientations and self-loops may be present.  Cleaning (canonical
 ``(min, max)`` pairs, self-loop removal, dedup, dense relabel) is the job of
 the CSR builder, as in ``experiment/exp_gpu.py:291-322``.
 
-All randomness comes from ``numpy.random.default_rng(seed)`` (PCG64), so a
-workload is fully named by its parameters and seed.
+R-MAT and Erdos-Renyi pairs come from a counter-based generator (below) so
+that the numpy functions here and the HIP kernel behind
+``generate_batch_gpu`` produce identical pairs; the random geometric and
+Chung-Lu graphs use ``numpy.random.default_rng(seed)`` (PCG64).  A workload is
+fully named by its parameters and seed.
 """
 
 from __future__ import annotations
@@ -19,64 +22,108 @@ from dataclasses import dataclass
 
 import numpy as np
 
+# ---------------------------------------------------------------------------
+# Counter-based stream: draw(key, i) = mix(key + (i + 1) * GOLDEN), the
+# splitmix64 sequence started at state `key`; key(seed, stream) =
+# mix(seed * GOLDEN ^ (stream + 1) * STREAM_MUL).  All arithmetic is mod 2^64.
+# ---------------------------------------------------------------------------
 
-def _raw_u32_pairs(rng: np.random.Generator, count: int):
-    """Yield 32-bit uniform integers, two per PCG64 raw draw."""
-    raw = rng.bit_generator.random_raw(count)
-    return (raw & np.uint64(0xFFFFFFFF)).astype(np.uint32), (raw >> np.uint64(32)).astype(np.uint32)
+GOLDEN = 0x9E3779B97F4A7C15
+STREAM_MUL = 0xD1B54A32D192ED03
+_M64 = (1 << 64) - 1
+
+
+def _mix_np(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def _mix_int(x: int) -> int:
+    x &= _M64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & _M64
+    return x ^ (x >> 31)
+
+
+def stream_key(seed: int, stream: int) -> int:
+    """Key of the counter stream `stream` under `seed`."""
+    return _mix_int(((seed * GOLDEN) & _M64) ^ (((stream + 1) * STREAM_MUL) & _M64))
+
+
+def draws(key: int, start: int, count: int) -> np.ndarray:
+    """``count`` consecutive 64-bit draws of the stream from index ``start``."""
+    i = np.arange(start + 1, start + count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        return _mix_np(np.uint64(key) + i * np.uint64(GOLDEN))
+
+
+def rmat_thresholds(a: float, b: float, c: float):
+    """Quadrant thresholds on a 32-bit uniform: floor(a 2^32), floor((a+b) 2^32), ..."""
+    f = lambda x: min(int(x * 2**32), 2**32 - 1)
+    return f(a), f(a + b), f(a + b + c)
+
+
+def _rmat_from_key(key: int, scale: int, m: int, a, b, c) -> np.ndarray:
+    t_a, t_ab, t_abc = (np.uint64(t) for t in rmat_thresholds(a, b, c))
+    D = (scale + 1) // 2
+    src = np.zeros(m, dtype=np.int64)
+    dst = np.zeros(m, dtype=np.int64)
+    idx = np.arange(m, dtype=np.uint64) * np.uint64(D)
+    for j in range(D):
+        with np.errstate(over="ignore"):
+            r64 = _mix_np(np.uint64(key) + (idx + np.uint64(j + 1)) * np.uint64(GOLDEN))
+        for half in range(2):
+            level = 2 * j + half
+            if level >= scale:
+                break
+            r = (r64 >> np.uint64(32 * half)) & np.uint64(0xFFFFFFFF)
+            bit = np.int64(1) << np.int64(scale - 1 - level)
+            s_bit = r >= t_ab
+            d_bit = ((r >= t_a) & (r < t_ab)) | (r >= t_abc)
+            src |= s_bit.astype(np.int64) * bit
+            dst |= d_bit.astype(np.int64) * bit
+    return np.stack([src, dst], axis=1).astype(np.int32)
 
 
 def rmat(scale: int, edge_factor: int, a: float = 0.57, b: float = 0.19,
-         c: float = 0.19, d: float = 0.05, seed: int = 0) -> np.ndarray:
+         c: float = 0.19, d: float = 0.05, seed: int = 0, stream: int = 0) -> np.ndarray:
     """R-MAT pairs: ``edge_factor * 2**scale`` samples over ``2**scale`` ids.
 
-    Each level picks a quadrant with probabilities (a, b, c, d) by comparing a
-    32-bit uniform integer against the thresholds ``floor(a * 2**32)``,
-    ``floor((a+b) * 2**32)`` and ``floor((a+b+c) * 2**32)``; two levels are
-    drawn per 64-bit PCG64 output.  No label permutation is applied.
+    Pair i uses draws ``i*D .. i*D + D - 1`` of its stream (D = ceil(scale/2));
+    level l compares the low (l even) or high (l odd) 32 bits of draw
+    ``i*D + l//2`` against the thresholds of ``rmat_thresholds``:
+    [0,a) -> (0,0), [a,a+b) -> (0,1), [a+b,a+b+c) -> (1,0), else (1,1), most
+    significant level first.  No label permutation is applied.
     """
     if not (1 <= scale <= 30):
         raise ValueError(f"scale must be in [1, 30], got {scale}")
     if abs(a + b + c + d - 1.0) > 1e-9:
         raise ValueError("R-MAT probabilities must sum to 1")
     m = int(edge_factor) << int(scale)
-    rng = np.random.default_rng(seed)
-    t_a = np.uint32(min(int(a * 2**32), 2**32 - 1))
-    t_ab = np.uint32(min(int((a + b) * 2**32), 2**32 - 1))
-    t_abc = np.uint32(min(int((a + b + c) * 2**32), 2**32 - 1))
-    src = np.zeros(m, dtype=np.int64)
-    dst = np.zeros(m, dtype=np.int64)
-    level = 0
-    while level < scale:
-        lo, hi = _raw_u32_pairs(rng, m)
-        for r in (lo, hi):
-            if level >= scale:
-                break
-            bit = np.int64(1) << np.int64(scale - 1 - level)
-            # quadrant: [0,a) -> (0,0); [a,a+b) -> (0,1); [a+b,a+b+c) -> (1,0); else (1,1)
-            s_bit = r >= t_ab
-            d_bit = ((r >= t_a) & (r < t_ab)) | (r >= t_abc)
-            src |= s_bit.astype(np.int64) * bit
-            dst |= d_bit.astype(np.int64) * bit
-            level += 1
-        del lo, hi
-    return np.stack([src, dst], axis=1).astype(np.int32)
+    return _rmat_from_key(stream_key(seed, stream), scale, m, a, b, c)
 
 
-def erdos_renyi(n: int, p: float, seed: int = 0) -> np.ndarray:
-    """G(n, M) stand-in for G(n, p): ``M = round(p * n * (n - 1) / 2)`` uniform pairs.
+def er_pairs_count(n: int, p: float) -> int:
+    """M = round(p * n * (n - 1) / 2)."""
+    return int(round(p * n * (n - 1) / 2.0)) if n >= 2 else 0
 
-    Endpoints are ``(r * n) >> 32`` for 32-bit uniform ``r`` (exact integer
-    arithmetic).  Self-loops and repeats are left for the CSR builder to drop.
+
+def erdos_renyi(n: int, p: float, seed: int = 0, stream: int = 0) -> np.ndarray:
+    """G(n, M) stand-in for G(n, p): M = round(p n (n-1) / 2) uniform pairs.
+
+    Pair i takes draw i of its stream: u = (low32 * n) >> 32,
+    v = (high32 * n) >> 32 (exact integer arithmetic).  Self-loops and
+    repeats are left for the CSR builder to drop.
     """
-    if n < 2:
+    m = er_pairs_count(n, p)
+    if m == 0:
         return np.zeros((0, 2), dtype=np.int32)
-    m = int(round(p * n * (n - 1) / 2.0))
-    rng = np.random.default_rng(seed)
-    lo, hi = _raw_u32_pairs(rng, m)
-    u = ((lo.astype(np.uint64) * np.uint64(n)) >> np.uint64(32)).astype(np.int32)
-    v = ((hi.astype(np.uint64) * np.uint64(n)) >> np.uint64(32)).astype(np.int32)
-    return np.stack([u, v], axis=1)
+    r = draws(stream_key(seed, stream), 0, m)
+    u = ((r & np.uint64(0xFFFFFFFF)) * np.uint64(n)) >> np.uint64(32)
+    v = ((r >> np.uint64(32)) * np.uint64(n)) >> np.uint64(32)
+    return np.stack([u, v], axis=1).astype(np.int32)
 
 
 def rgg_radius(n: int, mean_degree: float) -> float:
