@@ -179,6 +179,78 @@ def run_batch_extra(args, reps=10):
     }
 
 
+def run_router_extra(reps=3, p=12, theta=128, budget=0.005):
+    """Learned router: trained on R-MAT scale 20 (seed 5), evaluated on R-MAT
+    scale 23 (seed 4), HLL p = 12, against the fixed-threshold hybrid."""
+    import torch
+
+    from triangle_counting_hll_amd import batch, gpu_graph, gpu_hybrid, router
+
+    out = {"p": p, "theta": theta, "error_budget_train": budget}
+    # ---- training data from the scale-20 graph
+    t0 = time.perf_counter()
+    csr = gpu_graph.build_csr_gpu(batch.rmat_gpu(20, 16, seed=5))
+    sk = router.build_sketches(csr, theta, p)
+    cand = router.candidate_edges(csr, sk)
+    feats = router.features(csr, sk, cand)
+    exact, s_hll = router.edge_errors(csr, sk, cand)
+    labels = router.make_labels(exact, s_hll)
+    truth20 = int(gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v).item())
+    params, norm, hist = router.train(feats, labels)
+    prob = router.mlp_route_bf16(params, norm, feats, 0.5)
+    thr = router.choose_threshold(prob, s_hll - exact, truth20, budget)
+    model = router.RouterModel(params, norm, thr, p, 8, theta)
+    out["train"] = {"edges": csr.n_oriented, "candidates": int(cand.numel()),
+                    "positive_frac": float(labels.mean().item()), "loss": hist,
+                    "threshold": thr, "seconds": time.perf_counter() - t0}
+    del csr, sk, cand, feats, exact, s_hll, labels, prob
+    torch.cuda.empty_cache()
+
+    # ---- evaluation graph
+    csr = gpu_graph.build_csr_gpu(batch.rmat_gpu(23, 16, seed=4))
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+
+    def exact_fn():
+        count.zero_()
+        gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v, count)
+
+    _, exact_ms = _timed(exact_fn, reps)
+    truth = int(count.item())
+    out["eval"] = {"graph": "rmat23 seed 4", "n_nodes": csr.n_nodes, "edges": csr.n_oriented,
+                   "triangles": truth, "exact_ms": exact_ms}
+
+    def fixed(gamma):
+        def fn():
+            skf = router.build_sketches(csr, gamma, p)
+            c, s, fb = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
+                                               csr.degrees, skf.regs, skf.has, gamma, p,
+                                               slot=skf.slot)
+            return c, s
+        (c, s), ms = _timed(fn, reps)
+        est = float(c.item())
+        return {"ms": ms, "rel_err": gpu_hybrid.relative_error(est, truth),
+                "coverage": int(s[0].item()) / csr.n_oriented}
+
+    out["fixed_threshold"] = {f"gamma{g}": fixed(g) for g in (64, 128, 256)}
+
+    def learned(gate):
+        m = router.RouterModel(params, norm, thr, p, 8, theta, min_cost_degree=gate)
+
+        def fn():
+            return router.routed_count(m, csr)
+        (est, st), ms = _timed(fn, reps)
+        return {"ms": ms, "rel_err": gpu_hybrid.relative_error(est, truth),
+                "coverage": st["n_approx"] / csr.n_oriented, "candidates": st["n_candidates"],
+                "min_cost_degree": gate}
+
+    out["learned"] = {"accuracy_policy": learned(0), "cost_gated": learned(1 << (p - 1))}
+    ok_fixed = [v["ms"] for v in out["fixed_threshold"].values() if v["rel_err"] <= 0.01]
+    ok_learned = [v["ms"] for v in out["learned"].values() if v["rel_err"] <= 0.01]
+    if ok_fixed and ok_learned:
+        out["speedup_learned_vs_fixed_at_1pct"] = min(ok_fixed) / min(ok_learned)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,7 +356,8 @@ def main():
     extras = {}
     if not args.no_extras and not args.scale_down:
         for name, fn in (("hybrid_fixed_threshold", lambda: run_hybrid_extra(csr, gpu_count)),
-                         ("batch4096", lambda: run_batch_extra(args))):
+                         ("batch4096", lambda: run_batch_extra(args)),
+                         ("learned_router", lambda: run_router_extra())):
             try:
                 t0 = time.perf_counter()
                 extras[name] = fn()

@@ -183,6 +183,47 @@ int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
                     const uint8_t *route, int part, double *counts, int64_t *stats,
                     int64_t *fallback, double *per_edge, void *stream);
 
+/* ---- learned exact-vs-sketch router -------------------------------------- */
+/* No reference counterpart: the reference routes by a fixed degree threshold
+ * (src/hybrid.py:107-112, src/gpu_hybrid.py:80-89); these entry points let a
+ * small MLP make that decision per edge. */
+
+/* Features of candidate edges cand[0..n_cand) (edge ids into edge_u/edge_v),
+ * float32 [n_cand][6]: log2(1 + deg u), log2(1 + deg v), their difference,
+ * log2(1 + card[u]), log2(1 + card[v]) (card: float32 per-vertex sketch
+ * estimates) and log2(1 + sampled union), the HLL estimate over the first
+ * 2^sample_bits registers of both rows merged, times 2^(p - sample_bits).
+ * Rows addressed as in tc_hll_build. */
+int tc_router_features(const int32_t *degrees, const int32_t *edge_u, const int32_t *edge_v,
+                       const int32_t *cand, int64_t n_cand, const uint8_t *regs,
+                       const int32_t *slot, int p, int sample_bits, const float *card,
+                       float *feats, void *stream);
+
+/* Number of float32 parameters of the 6 -> 64 -> 1 network (513):
+ * W1[64][6], b1[64], w2[64], b2.  `norm` holds mean[6] then inv_std[6]. */
+int tc_router_n_params(void);
+
+/* FP32 forward: prob[i] = sigmoid(w2 . relu(W1 x_i + b1) + b2). */
+int tc_mlp_forward(const float *params, const float *norm, const float *feats, int64_t n,
+                   float *prob, void *stream);
+
+/* Gradient (float32 [513]) and value (*loss, device float) of the mean
+ * weighted binary cross-entropy -(w t log p + (1 - t) log(1 - p)) over n
+ * samples with targets t in [0, 1] and w = pos_weight.  Deterministic. */
+int tc_mlp_grad(const float *params, const float *norm, const float *feats, const float *target,
+                int64_t n, float pos_weight, float *grad, float *loss, void *stream);
+
+/* One Adam step (bias-corrected; step counts from 1) on params in place. */
+int tc_mlp_adam(float *params, const float *grad, float *m, float *v, int n_params, int step,
+                float lr, float beta1, float beta2, float eps, void *stream);
+
+/* BF16 inference (weights, inputs and hidden activations rounded to bf16,
+ * FP32 accumulation).  route[cand[i]] (or route[i] when cand is NULL) =
+ * prob_i >= threshold; prob (may be NULL) receives the probabilities. */
+int tc_mlp_route_bf16(const float *params, const float *norm, const float *feats, int64_t n,
+                      float threshold, const int32_t *cand, uint8_t *route, float *prob,
+                      void *stream);
+
 /* ---- batched counting of many graphs ------------------------------------- */
 /* No reference counterpart: the reference counts one graph per process
  * (experiment/exp_gpu.py:632-935).  Graphs are packed into one CSR over

@@ -49,6 +49,13 @@ _SIGNATURES = {
     "tc_union_estimate": (I32, [P, P, I32, P, P, I64, P, P]),
     "tc_hybrid_count": (I32, [P, I32, P, I64, P, P, I64, P, P, P, P, I64, I32, P, I32, P, P, P, P,
                               P]),
+    "tc_router_features": (I32, [P, P, P, P, I64, P, P, I32, I32, P, P, P]),
+    "tc_router_n_params": (I32, []),
+    "tc_mlp_forward": (I32, [P, P, P, I64, P, P]),
+    "tc_mlp_grad": (I32, [P, P, P, P, I64, ctypes.c_float, P, P, P]),
+    "tc_mlp_adam": (I32, [P, P, P, P, I32, I32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                          ctypes.c_float, P]),
+    "tc_mlp_route_bf16": (I32, [P, P, P, I64, ctypes.c_float, P, P, P, P]),
     "tc_vertex_graph_ids": (I32, [P, I64, P, I32, P, P]),
     "tc_batch_exact_count": (I32, [P, I32, P, I64, P, P, I64, P, I32, P, P, P]),
     "tc_batch_graph_create": (I32, [P, I32, P, I64, P, P, I64, P, I32, P, P, P, P, P]),

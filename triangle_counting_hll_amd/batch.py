@@ -124,6 +124,15 @@ def generate_pairs_gpu(spec: BatchSpec, stream=None):
     return pairs
 
 
+def rmat_gpu(scale: int, edge_factor: int, seed: int, abcd=RMAT_ABCD):
+    """Device twin of ``graphgen.rmat(scale, edge_factor, seed=seed)`` (stream 0)."""
+    spec = BatchSpec(np.array([1], np.uint8), np.array([scale], np.int32),
+                     np.array([1 << scale], np.int64),
+                     np.array([int(edge_factor) << int(scale)], np.int64),
+                     np.array([stream_key(seed, 0)], np.uint64), seed, tuple(abcd))
+    return generate_pairs_gpu(spec)
+
+
 @dataclass
 class BatchedCSR:
     """One forward CSR holding every graph of a batch, plus the graph of each vertex."""

@@ -162,6 +162,48 @@ k_union_estimate(const uint8_t *__restrict__ regs, const int32_t *__restrict__ s
 }
 
 // ---------------------------------------------------------------------------
+// Router features: one wavefront per candidate edge.
+//   f0 = log2(1 + deg u), f1 = log2(1 + deg v), f2 = f0 - f1,
+//   f3 = log2(1 + card u), f4 = log2(1 + card v)  (per-vertex estimates),
+//   f5 = log2(1 + sampled union): the estimator on the first 2^s registers of
+//        both rows merged -- the elements whose register index is below 2^s,
+//        i.e. whose top p - s index bits are zero, a 2^-(p-s) sample of the
+//        union -- scaled by 2^(p-s).  It reads 2 * 2^s bytes per edge instead
+//        of 2 * 2^p.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock)
+k_router_features(const int32_t *__restrict__ degrees, const int32_t *__restrict__ eu,
+                  const int32_t *__restrict__ ev, const int32_t *__restrict__ cand, int64_t n_cand,
+                  const uint8_t *__restrict__ regs, const int32_t *__restrict__ slot, int p,
+                  int sample_bits, const float *__restrict__ card, float *__restrict__ feats) {
+  const int m = 1 << p;
+  const int ms = 1 << sample_bits;
+  const double scale = (double)(1 << (p - sample_bits));
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  for (int64_t i = wid; i < n_cand; i += nw) {
+    const int32_t e = cand[i];
+    const int32_t u = eu[e], v = ev[e];
+    const int64_t a = slot ? slot[u] : u;
+    const int64_t b = slot ? slot[v] : v;
+    uint64_t zeros, hi, lo;
+    wave_union_stats(regs + a * m, regs + b * m, ms, zeros, hi, lo);
+    if (lane_id() == 0) {
+      const double us = hll_estimate(ms, zeros, hi, lo) * scale;
+      const float f0 = log2f(1.0f + (float)degrees[u]);
+      const float f1 = log2f(1.0f + (float)degrees[v]);
+      float *f = feats + i * 6;
+      f[0] = f0;
+      f[1] = f1;
+      f[2] = f0 - f1;
+      f[3] = log2f(1.0f + card[u]);
+      f[4] = log2f(1.0f + card[v]);
+      f[5] = (float)log2(1.0 + us);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Hybrid
 // ---------------------------------------------------------------------------
 
@@ -409,6 +451,26 @@ extern "C" int tc_union_estimate(const uint8_t *regs, const int32_t *slot, int p
   if (k == 0) return TC_OK;
   hipLaunchKernelGGL(k_union_estimate, dim3(grid_for(k * kWave, kBlock, 65536)), dim3(kBlock), 0,
                      (hipStream_t)stream, regs, slot, p, us, vs, k, out);
+  TC_HIP(hipGetLastError());
+  return TC_OK;
+}
+
+extern "C" int tc_router_features(const int32_t *degrees, const int32_t *edge_u,
+                                  const int32_t *edge_v, const int32_t *cand, int64_t n_cand,
+                                  const uint8_t *regs, const int32_t *slot, int p, int sample_bits,
+                                  const float *card, float *feats, void *stream) {
+  using namespace tc;
+  TC_CHECK(p >= 4 && p <= 16 && sample_bits >= 4 && sample_bits <= p, TC_ERR_INVALID,
+           "need 4 <= sample_bits <= p <= 16 (got p=%d, sample_bits=%d)", p, sample_bits);
+  TC_CHECK(n_cand >= 0 && (n_cand == 0 || (degrees && edge_u && edge_v && cand && regs && card &&
+                                           feats)),
+           TC_ERR_INVALID, "invalid arguments to tc_router_features");
+  TC_CHECK(((uintptr_t)regs & 15) == 0, TC_ERR_INVALID, "regs must be 16-byte aligned");
+  TC_TRY(ensure_init());
+  if (n_cand == 0) return TC_OK;
+  hipLaunchKernelGGL(k_router_features, dim3(grid_for(n_cand * kWave, kBlock, 65536)),
+                     dim3(kBlock), 0, (hipStream_t)stream, degrees, edge_u, edge_v, cand, n_cand,
+                     regs, slot, p, sample_bits, card, feats);
   TC_HIP(hipGetLastError());
   return TC_OK;
 }
