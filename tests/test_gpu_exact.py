@@ -46,6 +46,18 @@ def test_clique_beyond_lds_table(device):
     assert _count(csr) == math.comb(n, 3)
 
 
+@pytest.mark.parametrize("plan", ["sorted", "atomic"])
+def test_big_centers_mixed_with_small(device, plan):
+    # a K_2100 block (forward rows of 1024..2099 ids take the workgroup kernel)
+    # plus an R-MAT graph on disjoint ids (wavefront kernel): counts add up
+    n = 2100
+    clique = np.stack(np.triu_indices(n, 1), axis=1)
+    other = rmat(12, 16, seed=8).astype(np.int64) + n
+    pairs = np.concatenate([clique, other]).astype(np.int32)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    assert _count(csr, params={"plan": plan}) == math.comb(n, 3) + _truth(other)
+
+
 def test_rmat14_config(device):
     pairs = config_pairs("rmat14")
     rp, col, fdeg, nid, eu, ev, _ = O.csr_from_pairs(pairs)

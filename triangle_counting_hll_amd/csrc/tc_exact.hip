@@ -24,7 +24,10 @@
 namespace tc {
 
 constexpr int kTblInts = 2048;    // per-wave LDS hash table: 512 buckets x 4 slots (8 KiB)
-constexpr int kTblMaxDeg = 1024;  // longer center rows use the global binary-search path
+constexpr int kTblMaxDeg = 1023;  // longer center rows go to the workgroup kernel
+constexpr int kBigMinBits = 11;   // d+ >= 1024 (bit length >= 11): workgroup kernel
+constexpr int kBigTblInts = 32768;  // block-shared table: 8192 buckets x 4 slots (128 KiB)
+constexpr int kBigMaxDeg = kBigTblInts / 2 - 1;  // longer rows: global binary search
 constexpr int kDefaultUnroll = 8; // streamed ids in flight per lane
 constexpr int kWavesPerBlock = kBlock / kWave;
 
@@ -300,9 +303,9 @@ __global__ void __launch_bounds__(kBlock)
 k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
         O others, const int2 *__restrict__ items,
         const int64_t *__restrict__ n_items_p, const int64_t *__restrict__ iptr,
-        const int64_t *__restrict__ n_big_centers_p, int chunk,
-        unsigned long long *__restrict__ next_item, const int32_t *__restrict__ graph_of,
-        unsigned long long *__restrict__ count) {
+        const int64_t *__restrict__ n_first_runs_p, const int64_t *__restrict__ n_big_centers_p,
+        int chunk, unsigned long long *__restrict__ next_item,
+        const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count) {
   __shared__ int4 tbl_all[kWavesPerBlock][kTblInts / 4];
   __shared__ int seg_end_all[kWavesPerBlock][kWave];
   __shared__ int64_t seg_base_all[kWavesPerBlock][kWave];
@@ -312,8 +315,9 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
   int *seg_end = seg_end_all[w];
   int64_t *seg_base = seg_base_all[w];
   const int64_t n_items = *n_items_p;
-  // items of the long centers come first; each queue ticket maps to one of
-  // them, past that point a ticket covers kSmallBatch items
+  // items [0, first) belong to k_heavy_big; then the items of the long
+  // centers, one per queue ticket; past those a ticket covers kSmallBatch items
+  const int64_t first = iptr[*n_first_runs_p];
   const int64_t n_big = iptr[*n_big_centers_p];
   uint64_t acc_total = 0;
   int64_t it = 0, it_end = 0;
@@ -323,11 +327,11 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
       unsigned long long ticket = 0;
       if (lane == 0) ticket = atomicAdd(next_item, 1ull);
       const int64_t tk = (int64_t)__shfl(ticket, 0);
-      if (tk < n_big) {
-        it = tk;
-        it_end = tk + 1;
+      if (tk < n_big - first) {
+        it = first + tk;
+        it_end = it + 1;
       } else {
-        it = n_big + (tk - n_big) * kSmallBatch;
+        it = n_big + (tk - (n_big - first)) * kSmallBatch;
         it_end = min(it + (int64_t)kSmallBatch, n_items);
       }
       if (it >= n_items) break;
@@ -386,6 +390,91 @@ __global__ void k_edge_supports(Rows<I> R, const int32_t *__restrict__ col,
       if (lo < b1 && col[lo] == x) ++c;
     }
     out[e] = c;
+  }
+}
+
+// Centers with d+ >= kBigMinDeg: one workgroup per work item, the center row
+// in one block-shared LDS table (up to kBigMaxDeg ids at load <= 1/2), the
+// four wavefronts streaming disjoint quarters of the item's others.  Items of
+// these centers come first in the heavy-first order; this kernel takes items
+// [0, n_first) and k_heavy the rest.
+template <class I, int U, class O>
+__global__ void __launch_bounds__(kBlock)
+k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
+            O others, const int2 *__restrict__ items, const int64_t *__restrict__ iptr,
+            const int64_t *__restrict__ n_first_runs_p, int chunk,
+            unsigned long long *__restrict__ next_item, const int32_t *__restrict__ graph_of,
+            unsigned long long *__restrict__ count) {
+  __shared__ int4 tbl[kBigTblInts / 4];
+  __shared__ int seg_end_all[kWavesPerBlock][kWave];
+  __shared__ int64_t seg_base_all[kWavesPerBlock][kWave];
+  __shared__ int64_t s_item;
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int64_t n_first = iptr[*n_first_runs_p];
+  int *tbl_i = reinterpret_cast<int *>(tbl);
+  uint64_t acc_total = 0;
+  while (true) {
+    if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(next_item, 1ull);
+    __syncthreads();
+    const int64_t it = s_item;
+    __syncthreads();
+    if (it >= n_first) break;
+    const int2 item = items[it];
+    const int64_t ob = cptr[item.x] + (int64_t)item.y * chunk;
+    const int64_t oe = min(cptr[item.x + 1], ob + (int64_t)chunk);
+    const int32_t c = others.center(item.x, ob);
+    const int64_t cb = R.begin(c);
+    const int dc = (int)(R.end(c) - cb);
+    const int64_t part = (oe - ob + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int64_t wb = min(oe, ob + w * part), we = min(oe, wb + part);
+    uint64_t acc;
+    if (dc <= kBigMaxDeg) {
+      int logNB = 2;
+      while ((1 << logNB) < dc && (4 << logNB) < kBigTblInts) ++logNB;
+      const int NB = 1 << logNB, shift = 32 - logNB;
+      const uint32_t mask = (uint32_t)NB - 1;
+      for (int i = threadIdx.x; i < 4 * NB; i += blockDim.x) tbl_i[i] = -1;
+      __syncthreads();
+      for (int i = threadIdx.x; i < dc; i += blockDim.x) {
+        const int x = col[cb + i];
+        uint32_t b = bucket_of((uint32_t)x, shift);
+        while (true) {
+          int *sl = tbl_i + 4 * b;
+          if (atomicCAS(sl, -1, x) == -1 || atomicCAS(sl + 1, -1, x) == -1 ||
+              atomicCAS(sl + 2, -1, x) == -1 || atomicCAS(sl + 3, -1, x) == -1)
+            break;
+          b = (b + 1) & mask;
+        }
+      }
+      __syncthreads();
+      acc = stream_others<I, true, U>(R, col, others, wb, we, seg_end_all[w], seg_base_all[w],
+                                      tbl, shift, mask, cb, dc);
+      __syncthreads();  // the table is rebuilt for the next item
+    } else {
+      acc = stream_others<I, false, U>(R, col, others, wb, we, seg_end_all[w], seg_base_all[w],
+                                       tbl, 0, 0, cb, dc);
+    }
+    if (graph_of) {
+      acc = wave_sum(acc);
+      if (lane == 0 && acc) atomicAdd(&count[graph_of[c]], (unsigned long long)acc);
+    } else {
+      acc_total += acc;
+    }
+  }
+  if (!graph_of) {
+    acc_total = wave_sum(acc_total);
+    if (lane == 0 && acc_total) atomicAdd(count, (unsigned long long)acc_total);
+  }
+}
+
+template <class I, class O>
+static auto heavy_big_kernel(int unroll) -> decltype(&k_heavy_big<I, 4, O>) {
+  switch (unroll) {
+    case 1: return &k_heavy_big<I, 1, O>;
+    case 2: return &k_heavy_big<I, 2, O>;
+    case 8: return &k_heavy_big<I, 8, O>;
+    default: return &k_heavy_big<I, 4, O>;
   }
 }
 
@@ -545,13 +634,14 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   TC_TRY(iptr.alloc(sizeof(int64_t) * (max_runs + 1), s));
   const int64_t max_items = max_runs + E / chunk + 1;
   TC_TRY(items.alloc(sizeof(int2) * max_items, s));
-  TC_TRY(ctr.alloc(2 * sizeof(unsigned long long), s));
+  TC_TRY(ctr.alloc(3 * sizeof(unsigned long long), s));
   unsigned long long *next_item = ctr.as<unsigned long long>();
   unsigned long long *n_heavy = next_item + 1;
 
   const unsigned gr = grid_for(max_runs + 1, kBlock, 16384);
   const unsigned ge = grid_for(E, kBlock, 16384);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, next_item, 2, 0ull);
+  unsigned long long *next_big = next_item + 2;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, next_item, 3, 0ull);
   TC_DISPATCH_ROWS(row_ptr_64, I, {
     Rows<I> R{(const I *)row_ptr};
     {
@@ -591,13 +681,20 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
+      const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
+      const auto kbig = heavy_big_kernel<I, OthersPacked>(unroll);
+      hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
+                         (const int64_t *)run_ptr.as<int64_t>(),
+                         OthersPacked{(const uint64_t *)sorted}, (const int2 *)items.as<int2>(),
+                         (const int64_t *)iptr.as<int64_t>(), n_first, chunk, next_big, graph_of,
+                         count);
       const auto kfn = heavy_kernel<I, OthersPacked>(unroll);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R,
                          col, (const int64_t *)run_ptr.as<int64_t>(),
                          OthersPacked{(const uint64_t *)sorted}, (const int2 *)items.as<int2>(),
                          (const int64_t *)(iptr.as<int64_t>() + max_runs),
-                         (const int64_t *)iptr.as<int64_t>(), n_big_runs, chunk, next_item,
-                         graph_of, count);
+                         (const int64_t *)iptr.as<int64_t>(), n_first, n_big_runs, chunk,
+                         next_item, graph_of, count);
     }
     TC_HIP(hipGetLastError());
   });
@@ -641,12 +738,12 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
   TC_TRY(iptr.alloc(sizeof(int64_t) * (n + 1), s));
   const int64_t max_items = n + E / chunk + 1;
   TC_TRY(items.alloc(sizeof(int2) * max_items, s));
-  TC_TRY(ctr.alloc(sizeof(unsigned long long), s));
+  TC_TRY(ctr.alloc(2 * sizeof(unsigned long long), s));
 
   const unsigned gn = grid_for(n, kBlock, 16384);
   const unsigned ge = grid_for(E, kBlock, 16384);
   hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(kBlock), 0, s, cnt.as<int32_t>(), n, 0);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr.as<unsigned long long>(), 1, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr.as<unsigned long long>(), 2, 0ull);
   TC_DISPATCH_ROWS(row_ptr_64, I, {
     Rows<I> R{(const I *)row_ptr};
     {
@@ -681,12 +778,19 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
+      const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
+      const auto kbig = heavy_big_kernel<I, OthersIds>(unroll);
+      hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
+                         (const int64_t *)cptr.as<int64_t>(),
+                         OthersIds{(const int32_t *)others.as<int32_t>()},
+                         (const int2 *)items.as<int2>(), (const int64_t *)iptr.as<int64_t>(),
+                         n_first, chunk, ctr.as<unsigned long long>() + 1, graph_of, count);
       const auto kfn = heavy_kernel<I, OthersIds>(unroll);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)cptr.as<int64_t>(),
                          OthersIds{(const int32_t *)others.as<int32_t>()},
                          (const int2 *)items.as<int2>(), (const int64_t *)(iptr.as<int64_t>() + n),
-                         (const int64_t *)iptr.as<int64_t>(), n_big_cent, chunk,
+                         (const int64_t *)iptr.as<int64_t>(), n_first, n_big_cent, chunk,
                          ctr.as<unsigned long long>(), graph_of, count);
     }
     TC_HIP(hipGetLastError());

@@ -227,9 +227,12 @@ def mlp_route_bf16(params, norm, feats, threshold: float, cand=None, route=None,
     return out
 
 
-def train(feats, target, epochs: int = 6, batch: int = 65536, lr: float = 3e-3,
+def train(feats, target, epochs: int = 12, batch: int = 8192, lr: float = 1e-2,
           pos_weight: Optional[float] = None, seed: int = 0, norm=None):
-    """Train the network on the GPU; returns (params float32 [513] host, norm, loss history)."""
+    """Train the network on the GPU; returns (params float32 [513] host, norm, loss history).
+
+    Adam with the learning rate halved after every quarter of the epochs.
+    """
     t = _lib.require_device()
     f = _lib.as_device(feats, t.float32)
     tg = _lib.as_device(target, t.float32)
@@ -247,13 +250,14 @@ def train(feats, target, epochs: int = 6, batch: int = 65536, lr: float = 3e-3,
     gen.manual_seed(seed)
     history = []
     step = 0
-    for _ in range(epochs):
+    for ep in range(epochs):
         order = t.randperm(n, device="cuda", generator=gen)
+        lr_ep = lr * 0.5 ** (4 * ep // max(epochs, 1))
         for s in range(0, n, batch):
             idx = order[s:s + batch]
             g, loss = mlp_grad(params, nm, f[idx], tg[idx], pos_weight)
             step += 1
-            mlp_adam(params, g, m, v, step, lr)
+            mlp_adam(params, g, m, v, step, lr_ep)
         _, loss = mlp_grad(params, nm, f, tg, pos_weight)
         history.append(float(loss.item()))
     return params.cpu().numpy(), np.asarray(norm, np.float32), history
@@ -289,7 +293,14 @@ def routed_count(model: RouterModel, csr, sk: Optional[Sketches] = None, stream=
     """Hybrid count with the learned router; returns (estimate, stats dict)."""
     t = _lib.require_device()
     if sk is None:
-        sk = build_sketches(csr, model.theta, model.p)
+        # only vertices that can be an endpoint of a candidate need a sketch
+        need = max(model.theta, model.min_cost_degree)
+        if int(csr.degrees.max().item() if csr.n_nodes else 0) < need:
+            count = gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
+                                           stream=stream)
+            return float(count.item()), {"n_approx": 0, "n_exact": csr.n_oriented,
+                                         "n_clamped": 0, "n_candidates": 0, "n_sketched": 0}
+        sk = build_sketches(csr, need, model.p)
     r, n_cand = route(model, csr, sk, stream)
     c, s, fb = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
                                        csr.degrees, sk.regs, sk.has, 0, model.p, route=r,
