@@ -77,6 +77,22 @@ def cpu_baseline(rp, col, truth_hint=None, budget_s=20.0):
             "sample": f"every {stride}th oriented edge ({n_s} of {E}), u-major order"}
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per launch of k_heavy from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a calibration copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"exact_{workload}117M_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    traffic = hit = None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch":
+            traffic = float(parts[2])
+        if len(parts) == 3 and parts[0] == "l2_hit_rate":
+            hit = float(parts[2])
+    return traffic, hit
+
+
 def _timed(fn, reps):
     import torch
 
@@ -342,6 +358,8 @@ def main():
     heavy_ms, heavy_n = stages["heavy"]
     heavy_avg_ms = heavy_ms / max(heavy_n, 1)
     achieved = B_heavy / (heavy_avg_ms * 1e-3) / 1e9 if heavy_n else None
+    traffic, l2_hit = (measured_traffic(args.workload) if not args.scale_down
+                       and args.workload == "chunglu" else (None, None))
 
     cpu = None
     parity = None
@@ -392,7 +410,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/exact_chunglu117M_hbm_traffic.md (rocprofv3 --pmc "
+                              "FETCH_SIZE x2.0 + WRITE_SIZE x1.0, calibrated on a 1 GiB copy)"
+                              if traffic else None,
+            "traffic_gbs": traffic / (heavy_avg_ms * 1e-3) / 1e9 if traffic and heavy_n else None,
+            "l2_hit_rate": l2_hit,
             "algorithmic_bytes_per_launch": B_heavy,
             "heavy_edges": n_heavy,
             "avg_launch_ms": heavy_avg_ms,

@@ -1,7 +1,8 @@
 // (a) Device CSR build with (degree, id) orientation, and the degree-bucket sort.
 //
-// Pipeline (all on the device, no sort of the full edge list):
-//   canonical (min,max) keys -> hash-set dedup (64-bit CAS, open addressing)
+// Pipeline (all on the device):
+//   canonical (min,max) keys packed in 2 x bit_length(max id) bits -> LSD
+//   radix sort -> adjacent-unique
 //   -> presence flags + scan = dense relabel by sorted unique endpoint
 //   -> undirected degrees -> orientation -> forward degrees -> scan = row_ptr
 //   -> scatter into rows -> per-row sort (wave bitonic / LDS bitonic / global
@@ -17,49 +18,53 @@ namespace tc {
 
 constexpr uint64_t kEmptyKey = ~0ull;
 
-__device__ __forceinline__ uint64_t key_hash(uint64_t k) {
-  k ^= k >> 33;
-  k *= 0xff51afd7ed558ccdull;
-  k ^= k >> 33;
-  k *= 0xc4ceb9fe1a85ec53ull;
-  k ^= k >> 33;
-  return k;
-}
-
-// Canonical keys, self-loops dropped, then inserted into the hash set.  A key
-// seen for the first time is appended to `uniq` through an atomic cursor.
-__global__ void k_dedup_insert(const int32_t *__restrict__ pairs, int64_t m,
-                               unsigned long long *__restrict__ table, uint64_t tmask,
-                               uint64_t *__restrict__ uniq,
-                               unsigned long long *__restrict__ n_uniq,
-                               int *__restrict__ max_id, int *__restrict__ bad) {
+// Largest endpoint of the canonical non-loop pairs; flags negative ids.
+__global__ void k_canon_max(const int32_t *__restrict__ pairs, int64_t m, int *__restrict__ max_id,
+                            int *__restrict__ bad) {
   int local_max = -1;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int32_t x = pairs[2 * i], y = pairs[2 * i + 1];
+    const int32_t x = pairs[2 * i], y = pairs[2 * i + 1];
     if (x < 0 || y < 0) {
       atomicOr(bad, 1);
       continue;
     }
-    if (x == y) continue;
-    uint32_t a = (uint32_t)min(x, y), b = (uint32_t)max(x, y);
-    local_max = max(local_max, (int)b);
-    uint64_t key = ((uint64_t)a << 32) | b;
-    uint64_t slot = key_hash(key) & tmask;
-    while (true) {
-      unsigned long long prev = atomicCAS(&table[slot], (unsigned long long)kEmptyKey,
-                                          (unsigned long long)key);
-      if (prev == kEmptyKey) {
-        uniq[atomicAdd(n_uniq, 1ull)] = key;
-        break;
-      }
-      if (prev == key) break;
-      slot = (slot + 1) & tmask;
-    }
+    if (x != y) local_max = max(local_max, max(x, y));
   }
-  // one atomic per wave
   for (int off = 32; off > 0; off >>= 1) local_max = max(local_max, __shfl_xor(local_max, off));
   if (lane_id() == 0 && local_max >= 0) atomicMax(max_id, local_max);
+}
+
+// (min << B | max) for a non-loop pair, all-ones (sorts last) for a loop.
+__global__ void k_canon_pack(const int32_t *__restrict__ pairs, int64_t m, int B,
+                             uint64_t *__restrict__ key) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t x = pairs[2 * i], y = pairs[2 * i + 1];
+    key[i] = (x == y) ? kEmptyKey
+                      : (((uint64_t)(uint32_t)min(x, y) << B) | (uint64_t)(uint32_t)max(x, y));
+  }
+}
+
+// First of each run of equal sorted keys (loops excluded).
+__global__ void k_unique_flags(const uint64_t *__restrict__ k, int64_t m, int B,
+                               uint8_t *__restrict__ flag) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x)
+    flag[i] = (k[i] != kEmptyKey && (i == 0 || k[i] != k[i - 1])) ? 1 : 0;
+}
+
+// Unique canonical pairs, ascending by (min, max), as (min << 32 | max).
+__global__ void k_unpack_unique(const uint64_t *__restrict__ k, const int32_t *__restrict__ pos,
+                                const int64_t *__restrict__ n_p, int B,
+                                uint64_t *__restrict__ out) {
+  const int64_t n = *n_p;
+  const uint64_t lo_mask = (1ull << B) - 1;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t w = k[pos[j]];
+    out[j] = ((w >> B) << 32) | (w & lo_mask);
+  }
 }
 
 __global__ void k_mark_present(const uint64_t *__restrict__ uniq, int64_t nu,
@@ -334,40 +339,25 @@ extern "C" int tc_csr_build_begin(const int32_t *pairs, int64_t m, void *stream,
     return rc;
   };
 
-  uint64_t tsize = 1024;
-  while (tsize < 2 * (uint64_t)m) tsize <<= 1;
-  Scratch table, counters;
-  int rc = table.alloc(sizeof(uint64_t) * tsize, s);
+  // 1. largest id among canonical non-loop pairs (sizes the packed keys)
+  Scratch counters;
+  int rc = counters.alloc(64, s);
   if (rc) return fail(rc);
-  if (hipMalloc(&b->keys, sizeof(uint64_t) * std::max<int64_t>(m, 1)) != hipSuccess) {
-    set_error(TC_ERR_OOM, "cannot allocate %lld edge keys", (long long)m);
-    return fail(TC_ERR_OOM);
-  }
-  rc = counters.alloc(64, s);
-  if (rc) return fail(rc);
-  if (hipMemsetAsync(table.ptr, 0xff, sizeof(uint64_t) * tsize, s) != hipSuccess ||
-      hipMemsetAsync(counters.ptr, 0, 64, s) != hipSuccess) {
+  if (hipMemsetAsync(counters.ptr, 0, 64, s) != hipSuccess) {
     set_error(TC_ERR_HIP, "memset failed");
     return fail(TC_ERR_HIP);
   }
-  unsigned long long *n_uniq = (unsigned long long *)counters.ptr;
+  unsigned long long *n_uniq_d = (unsigned long long *)counters.ptr;
   int *max_id = (int *)((char *)counters.ptr + 8);
   int *bad = (int *)((char *)counters.ptr + 12);
-  // max_id starts at -1
   int minus1 = -1;
   if (hipMemcpyAsync(max_id, &minus1, sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess) {
     set_error(TC_ERR_HIP, "memcpy failed");
     return fail(TC_ERR_HIP);
   }
-  if (m > 0) {
-    hipLaunchKernelGGL(k_dedup_insert, dim3(grid_for(m, kBlock, 8192)), dim3(kBlock), 0, s,
-                       pairs, m, (unsigned long long *)table.ptr, tsize - 1, b->keys, n_uniq,
-                       max_id, bad);
-    if (hipGetLastError() != hipSuccess) {
-      set_error(TC_ERR_HIP, "k_dedup_insert launch failed");
-      return fail(TC_ERR_HIP);
-    }
-  }
+  if (m > 0)
+    hipLaunchKernelGGL(k_canon_max, dim3(grid_for(m, kBlock, 8192)), dim3(kBlock), 0, s, pairs,
+                       m, max_id, bad);
   struct {
     unsigned long long nu;
     int max_id;
@@ -375,15 +365,50 @@ extern "C" int tc_csr_build_begin(const int32_t *pairs, int64_t m, void *stream,
   } h;
   if (hipMemcpyAsync(&h, counters.ptr, 16, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
-    set_error(TC_ERR_HIP, "dedup readback failed");
+    set_error(TC_ERR_HIP, "id scan failed: %s", hipGetErrorString(hipGetLastError()));
     return fail(TC_ERR_HIP);
   }
   if (h.bad) {
     set_error(TC_ERR_INVALID, "edge list holds negative node ids");
     return fail(TC_ERR_INVALID);
   }
-  const int64_t nu = (int64_t)h.nu;
-  const int64_t span = (int64_t)h.max_id + 1;  // 0 when no edges survive
+  // 2. canonical pairs packed as (min << B | max), B = bit length of the
+  //    largest id, loops as all-ones; radix sort; keep the first of each run
+  int B = 1;
+  while (B < 32 && (int64_t(1) << B) <= (int64_t)h.max_id) ++B;
+  Scratch packed, tmp, flags, pos;
+  if ((rc = packed.alloc(sizeof(uint64_t) * std::max<int64_t>(m, 1), s)) ||
+      (rc = tmp.alloc(sizeof(uint64_t) * std::max<int64_t>(m, 1), s)) ||
+      (rc = flags.alloc(std::max<int64_t>(m, 1), s)) ||
+      (rc = pos.alloc(sizeof(int32_t) * std::max<int64_t>(m, 1), s)))
+    return fail(rc);
+  if (hipMalloc(&b->keys, sizeof(uint64_t) * std::max<int64_t>(m, 1)) != hipSuccess) {
+    set_error(TC_ERR_OOM, "cannot allocate %lld edge keys", (long long)m);
+    return fail(TC_ERR_OOM);
+  }
+  int64_t nu = 0;
+  if (m > 0 && h.max_id >= 0) {
+    hipLaunchKernelGGL(k_canon_pack, dim3(grid_for(m, kBlock, 8192)), dim3(kBlock), 0, s, pairs,
+                       m, B, packed.as<uint64_t>());
+    uint64_t *sorted = nullptr;
+    if ((rc = radix_sort_u64(packed.as<uint64_t>(), tmp.as<uint64_t>(), m, 0, 2 * B, &sorted, s)))
+      return fail(rc);
+    hipLaunchKernelGGL(k_unique_flags, dim3(grid_for(m, kBlock, 8192)), dim3(kBlock), 0, s,
+                       (const uint64_t *)sorted, m, B, flags.as<uint8_t>());
+    if ((rc = compact_indices(flags.as<uint8_t>(), m, pos.as<int32_t>(), (int64_t *)n_uniq_d, s)))
+      return fail(rc);
+    hipLaunchKernelGGL(k_unpack_unique, dim3(grid_for(m, kBlock, 8192)), dim3(kBlock), 0, s,
+                       (const uint64_t *)sorted, (const int32_t *)pos.as<int32_t>(),
+                       (const int64_t *)n_uniq_d, B, b->keys);
+    int64_t nu_h = 0;
+    if (hipMemcpyAsync(&nu_h, n_uniq_d, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error(TC_ERR_HIP, "dedup failed: %s", hipGetErrorString(hipGetLastError()));
+      return fail(TC_ERR_HIP);
+    }
+    nu = nu_h;
+  }
+  const int64_t span = nu > 0 ? (int64_t)h.max_id + 1 : 0;  // 0 when no edges survive
   b->E = nu;
 
   Scratch present, nidx;
