@@ -4,9 +4,8 @@
 //   h = splitmix64 finaliser(id); j = h >> (64 - p); w = low 64 - p bits;
 //   rho = (64 - p) - bit_length(w) + 1, or 65 - p when w == 0;
 //   reg[j] = max(reg[j], rho)                       (src/hll.py:81-91)
-// The estimator is the FP64 one of src/hll.py:37-54.  Its sum
-// Z = sum_j 2^-r_j is accumulated exactly as a 128-bit fixed-point integer
-// (unit 2^-64) plus a count of zero registers, so Z carries one rounding.
+// The estimator is the FP64 one of src/hll.py:37-54; its sum Z = sum_j 2^-r_j
+// is accumulated in FP64 in a fixed order (deterministic, ~1e-14 relative).
 #include <algorithm>
 
 #include "tc_common.hpp"
@@ -85,11 +84,10 @@ __device__ __forceinline__ double hll_alpha(int m) {
   return 0.7213 / (1.0 + 1.079 / (double)m);
 }
 
-// Estimate from the merged register statistics (src/hll.py:37-54).
-// Z = zeros + (hi * 2^64 + lo) * 2^-64.
-__device__ __forceinline__ double hll_estimate(int m, uint64_t zeros, uint64_t hi, uint64_t lo) {
+// Estimate from the merged register statistics (src/hll.py:37-54):
+// Z = sum_j 2^-r_j (zero registers included), zeros = #{j : r_j = 0}.
+__device__ __forceinline__ double hll_estimate(int m, uint64_t zeros, double Z) {
   const double md = (double)m;
-  const double Z = (double)(zeros + hi) + (double)lo * 0x1p-64;
   double E = hll_alpha(m) * md * md / Z;
   if (E <= 2.5 * md && zeros > 0) E = md * log(md / (double)zeros);
   const double two64 = 18446744073709551616.0;
@@ -97,25 +95,28 @@ __device__ __forceinline__ double hll_estimate(int m, uint64_t zeros, uint64_t h
   return E;
 }
 
-__device__ __forceinline__ void acc_reg(uint32_t r, uint64_t &zeros, uint64_t &hi, uint64_t &lo) {
-  if (r == 0) {
-    ++zeros;
-  } else {
-    const uint64_t t = 1ull << (64 - r);
-    lo += t;
-    hi += (lo < t);
-  }
+// 2^-r as a double built from its exponent bits (r <= 61).
+__device__ __forceinline__ double pow2_neg(uint32_t r) {
+  return __hiloint2double((int)((1023u - r) << 20), 0);
+}
+
+__device__ __forceinline__ void acc_reg(uint32_t r, uint32_t &zeros, double &z) {
+  zeros += (r == 0);
+  z += pow2_neg(r);
 }
 
 // Wave-cooperative register merge: every lane sees the totals on return.
+// Each lane sums its registers in a fixed order, then a fixed butterfly adds
+// the lanes, so the result is deterministic; the FP64 sum of at most 2^16
+// powers of two is within ~1e-14 relative of the exact value.
 __device__ __forceinline__ void wave_union_stats(const uint8_t *__restrict__ ra,
                                                  const uint8_t *__restrict__ rb, int m,
-                                                 uint64_t &zeros, uint64_t &hi, uint64_t &lo) {
+                                                 uint64_t &zeros_out, double &z_out) {
   const int lane = lane_id();
-  zeros = 0;
-  hi = 0;
-  lo = 0;
+  uint32_t zeros = 0;
+  double z = 0.0;
   if (m >= 16 * kWave) {
+#pragma unroll 4
     for (int j = lane * 16; j < m; j += 16 * kWave) {
       const uint4 a = *(const uint4 *)(ra + j);
       const uint4 b = *(const uint4 *)(rb + j);
@@ -124,25 +125,24 @@ __device__ __forceinline__ void wave_union_stats(const uint8_t *__restrict__ ra,
       for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int s = 0; s < 32; s += 8)
-          acc_reg(max((aw[q] >> s) & 0xffu, (bw[q] >> s) & 0xffu), zeros, hi, lo);
+          acc_reg(max((aw[q] >> s) & 0xffu, (bw[q] >> s) & 0xffu), zeros, z);
     }
   } else if (m >= 4 * kWave) {
     for (int j = lane * 4; j < m; j += 4 * kWave) {
       const uint32_t a = *(const uint32_t *)(ra + j), b = *(const uint32_t *)(rb + j);
 #pragma unroll
-      for (int s = 0; s < 32; s += 8) acc_reg(max((a >> s) & 0xffu, (b >> s) & 0xffu), zeros, hi, lo);
+      for (int s = 0; s < 32; s += 8) acc_reg(max((a >> s) & 0xffu, (b >> s) & 0xffu), zeros, z);
     }
   } else {
-    for (int j = lane; j < m; j += kWave) acc_reg(max((uint32_t)ra[j], (uint32_t)rb[j]), zeros, hi, lo);
+    for (int j = lane; j < m; j += kWave) acc_reg(max((uint32_t)ra[j], (uint32_t)rb[j]), zeros, z);
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     zeros += __shfl_xor(zeros, off);
-    const uint64_t olo = __shfl_xor(lo, off), ohi = __shfl_xor(hi, off);
-    const uint64_t nlo = lo + olo;
-    hi = hi + ohi + (nlo < lo);
-    lo = nlo;
+    z += __shfl_xor(z, off);
   }
+  zeros_out = zeros;
+  z_out = z;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -155,9 +155,10 @@ k_union_estimate(const uint8_t *__restrict__ regs, const int32_t *__restrict__ s
   for (int64_t i = wid; i < k; i += nw) {
     const int64_t a = slot ? slot[us[i]] : us[i];
     const int64_t b = slot ? slot[vs[i]] : vs[i];
-    uint64_t zeros, hi, lo;
-    wave_union_stats(regs + a * m, regs + b * m, m, zeros, hi, lo);
-    if (lane_id() == 0) out[i] = hll_estimate(m, zeros, hi, lo);
+    uint64_t zeros;
+    double z;
+    wave_union_stats(regs + a * m, regs + b * m, m, zeros, z);
+    if (lane_id() == 0) out[i] = hll_estimate(m, zeros, z);
   }
 }
 
@@ -186,10 +187,11 @@ k_router_features(const int32_t *__restrict__ degrees, const int32_t *__restrict
     const int32_t u = eu[e], v = ev[e];
     const int64_t a = slot ? slot[u] : u;
     const int64_t b = slot ? slot[v] : v;
-    uint64_t zeros, hi, lo;
-    wave_union_stats(regs + a * m, regs + b * m, ms, zeros, hi, lo);
+    uint64_t zeros;
+    double z;
+    wave_union_stats(regs + a * m, regs + b * m, ms, zeros, z);
     if (lane_id() == 0) {
-      const double us = hll_estimate(ms, zeros, hi, lo) * scale;
+      const double us = hll_estimate(ms, zeros, z) * scale;
       const float f0 = log2f(1.0f + (float)degrees[u]);
       const float f1 = log2f(1.0f + (float)degrees[v]);
       float *f = feats + i * 6;
@@ -266,10 +268,11 @@ k_hll_edges(Degrees<I> D, const int32_t *__restrict__ eu, const int32_t *__restr
     const int32_t u = eu[e], v = ev[e];
     const int64_t a = slot ? slot[u] : u;
     const int64_t b = slot ? slot[v] : v;
-    uint64_t zeros, hi, lo;
-    wave_union_stats(regs + a * m, regs + b * m, m, zeros, hi, lo);
+    uint64_t zeros;
+    double z;
+    wave_union_stats(regs + a * m, regs + b * m, m, zeros, z);
     if (lane_id() == 0) {
-      const double E = hll_estimate(m, zeros, hi, lo);
+      const double E = hll_estimate(m, zeros, z);
       const int64_t du = D(u), dv = D(v);
       double s = (double)du + (double)dv - E;
       const double upper = (double)min(du, dv);
