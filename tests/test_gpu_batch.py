@@ -49,6 +49,31 @@ def test_batch_counts_bit_exact(device, plan):
     assert np.array_equal(plain, truth) and secs > 0
 
 
+@pytest.mark.parametrize("plan", ["sorted", "atomic"])
+def test_batch_with_workgroup_centers(device, plan):
+    # cliques whose forward rows reach 1024..4095 ids (joined workgroup table)
+    # and >= 4096 ids (128 KiB workgroup kernel) next to small R-MAT graphs:
+    # each graph's count lands in its own slot
+    import math
+
+    import torch
+
+    parts = [np.stack(np.triu_indices(1500, 1), axis=1), rmat(10, 16, seed=2),
+             np.stack(np.triu_indices(4300, 1), axis=1), rmat(9, 8, seed=3)]
+    n_ids = np.array([1500, 1 << 10, 4300, 1 << 9], dtype=np.int64)
+    spec = batch.BatchSpec(kind=np.zeros(4, np.uint8), param=np.zeros(4, np.int32), n_ids=n_ids,
+                           n_pairs=np.array([len(p) for p in parts], dtype=np.int64),
+                           keys=np.zeros(4, np.uint64), seed=0)
+    pairs = np.concatenate([p.astype(np.int64) + off
+                            for p, off in zip(parts, spec.id_off[:-1])]).astype(np.int32)
+    truth = np.array([math.comb(1500, 3), cbind.exact_count(*O.csr_from_pairs(parts[1])[:2]),
+                      math.comb(4300, 3), cbind.exact_count(*O.csr_from_pairs(parts[3])[:2])],
+                     dtype=np.int64)
+    b = batch.build_batch(spec, pairs=torch.from_numpy(pairs).cuda())
+    got = batch.batch_exact_count(b, {"plan": plan}).cpu().numpy()
+    assert np.array_equal(got, truth)
+
+
 def test_graph_replays_match_eager(device):
     import torch
 

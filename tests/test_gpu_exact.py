@@ -38,8 +38,8 @@ def test_complete_graph_closed_form(device, n):
 
 
 def test_clique_beyond_lds_table(device):
-    # K_1400: forward rows up to 1399 ids exceed the per-wave LDS table, so the
-    # global binary-search path runs; C(1400, 3) triangles.
+    # K_1400: forward rows of 1024..1399 ids exceed the per-wave LDS table and
+    # are taken by whole workgroups over the joined table; C(1400, 3) triangles.
     n = 1400
     pairs = np.stack(np.triu_indices(n, 1), axis=1).astype(np.int32)
     csr = gpu_graph.build_csr_gpu(pairs)
@@ -47,10 +47,12 @@ def test_clique_beyond_lds_table(device):
 
 
 @pytest.mark.parametrize("plan", ["sorted", "atomic"])
-def test_big_centers_mixed_with_small(device, plan):
-    # a K_2100 block (forward rows of 1024..2099 ids take the workgroup kernel)
-    # plus an R-MAT graph on disjoint ids (wavefront kernel): counts add up
-    n = 2100
+@pytest.mark.parametrize("n", [2100, 4400])
+def test_big_centers_mixed_with_small(device, plan, n):
+    # a K_n block plus an R-MAT graph on disjoint ids: forward rows of
+    # 1024..4095 ids take the joined workgroup table, rows of 4096 ids and up
+    # (n = 4400) the 128 KiB workgroup kernel, the R-MAT rows one wavefront
+    # each; counts add up
     clique = np.stack(np.triu_indices(n, 1), axis=1)
     other = rmat(12, 16, seed=8).astype(np.int64) + n
     pairs = np.concatenate([clique, other]).astype(np.int32)

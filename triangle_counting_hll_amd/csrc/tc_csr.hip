@@ -78,16 +78,36 @@ __global__ void k_mark_present(const uint64_t *__restrict__ uniq, int64_t nu,
 }
 
 // Relabel to dense ids in place (key -> dense a<<32 | dense b) and count
-// undirected degrees.
+// undirected degrees.  The keys are sorted, so equal first ids sit in
+// consecutive lanes: one atomic per run of them (hub rows would otherwise
+// serialise thousands of same-address atomics); the second ids are spread out
+// and get one atomic each.
 __global__ void k_relabel_degree(uint64_t *__restrict__ uniq, int64_t nu,
                                  const int64_t *__restrict__ nidx, int *__restrict__ udeg) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nu;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t k = uniq[i];
-    uint32_t a = (uint32_t)nidx[k >> 32], b = (uint32_t)nidx[k & 0xffffffffu];
-    uniq[i] = ((uint64_t)a << 32) | b;
-    atomicAdd(&udeg[a], 1);
-    atomicAdd(&udeg[b], 1);
+  const int lane = lane_id();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // the loop bound is uniform per wavefront, so every lane joins the ballot
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1));
+       base < nu; base += stride) {
+    const int64_t i = base + lane;
+    const bool ok = i < nu;
+    uint32_t a = 0xffffffffu, b = 0;
+    if (ok) {
+      const uint64_t k = uniq[i];
+      a = (uint32_t)nidx[k >> 32];
+      b = (uint32_t)nidx[k & 0xffffffffu];
+      uniq[i] = ((uint64_t)a << 32) | b;
+    }
+    const uint32_t prev = (uint32_t)__shfl_up((int)a, 1);
+    const bool head = ok && (lane == 0 || prev != a);
+    const uint64_t heads = __ballot(head);
+    if (head) {
+      const uint64_t above = heads & ~((2ull << lane) - 1);  // lane 63: no lanes above
+      const int n_valid = (int)min<int64_t>(kWave, nu - base);
+      const int next = above ? (__ffsll((unsigned long long)above) - 1) : n_valid;
+      atomicAdd(&udeg[a], next - lane);
+    }
+    if (ok) atomicAdd(&udeg[b], 1);
   }
 }
 

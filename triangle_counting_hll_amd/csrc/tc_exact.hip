@@ -24,8 +24,10 @@
 namespace tc {
 
 constexpr int kTblInts = 2048;    // per-wave LDS hash table: 512 buckets x 4 slots (8 KiB)
-constexpr int kTblMaxDeg = 1023;  // longer center rows go to the workgroup kernel
-constexpr int kBigMinBits = 11;   // d+ >= 1024 (bit length >= 11): workgroup kernel
+constexpr int kTblMaxDeg = 1023;  // longer center rows: one table per workgroup
+constexpr int kBlkMinBits = 11;   // d+ in [1024, 4095]: the workgroup's four wave tables
+constexpr int kBlkTblInts = 4 * kTblInts;  // joined into one (2048 buckets, 32 KiB)
+constexpr int kBigMinBits = 13;   // d+ >= 4096: the 128 KiB workgroup kernel
 constexpr int kBigTblInts = 32768;  // block-shared table: 8192 buckets x 4 slots (128 KiB)
 constexpr int kBigMaxDeg = kBigTblInts / 2 - 1;  // longer rows: global binary search
 constexpr int kDefaultUnroll = 8; // streamed ids in flight per lane
@@ -173,17 +175,21 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t x, int shift) {
   return (x * 0x9E3779B1u) >> shift;
 }
 
+// Builds the set in `tbl` (kInts ints) with the lanes of one wavefront
+// (kBlockWide false) or all threads of the workgroup (true).
+template <int kInts, bool kBlockWide>
 __device__ __forceinline__ void table_build(int *__restrict__ tbl, const int32_t *__restrict__ row,
                                             int dc, int &shift, uint32_t &mask) {
-  const int lane = lane_id();
+  const int t0 = kBlockWide ? (int)threadIdx.x : lane_id();
+  const int stride = kBlockWide ? (int)blockDim.x : kWave;
   int logNB = 2;
-  while ((1 << logNB) < dc && (4 << logNB) < kTblInts) ++logNB;
+  while ((1 << logNB) < dc && (4 << logNB) < kInts) ++logNB;
   const int NB = 1 << logNB;
   shift = 32 - logNB;
   mask = (uint32_t)NB - 1;
-  for (int i = lane; i < 4 * NB; i += kWave) tbl[i] = -1;
-  wave_lds_sync();
-  for (int i = lane; i < dc; i += kWave) {
+  for (int i = t0; i < 4 * NB; i += stride) tbl[i] = -1;
+  if constexpr (kBlockWide) __syncthreads(); else wave_lds_sync();
+  for (int i = t0; i < dc; i += stride) {
     const int x = row[i];
     uint32_t b = bucket_of((uint32_t)x, shift);
     while (true) {
@@ -194,7 +200,7 @@ __device__ __forceinline__ void table_build(int *__restrict__ tbl, const int32_t
       b = (b + 1) & mask;
     }
   }
-  wave_lds_sync();
+  if constexpr (kBlockWide) __syncthreads(); else wave_lds_sync();
 }
 
 __device__ __forceinline__ uint32_t table_probe(const int4 *__restrict__ tbl, int x, int shift,
@@ -303,23 +309,62 @@ __global__ void __launch_bounds__(kBlock)
 k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
         O others, const int2 *__restrict__ items,
         const int64_t *__restrict__ n_items_p, const int64_t *__restrict__ iptr,
-        const int64_t *__restrict__ n_first_runs_p, const int64_t *__restrict__ n_big_centers_p,
-        int chunk, unsigned long long *__restrict__ next_item,
+        const int64_t *__restrict__ n_huge_runs_p, const int64_t *__restrict__ n_first_runs_p,
+        const int64_t *__restrict__ n_big_centers_p, int chunk,
+        unsigned long long *__restrict__ next_item, unsigned long long *__restrict__ next_block,
         const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count) {
   __shared__ int4 tbl_all[kWavesPerBlock][kTblInts / 4];
   __shared__ int seg_end_all[kWavesPerBlock][kWave];
   __shared__ int64_t seg_base_all[kWavesPerBlock][kWave];
+  __shared__ int64_t s_ticket;
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   int4 *tbl = tbl_all[w];
   int *seg_end = seg_end_all[w];
   int64_t *seg_base = seg_base_all[w];
   const int64_t n_items = *n_items_p;
-  // items [0, first) belong to k_heavy_big; then the items of the long
-  // centers, one per queue ticket; past those a ticket covers kSmallBatch items
+  // items [0, huge) belong to k_heavy_big; [huge, first) are centers with
+  // d+ in [1024, 4095], taken by whole workgroups over one joined table; then
+  // the items of the long centers, one per wavefront ticket; past those a
+  // ticket covers kSmallBatch items
+  const int64_t huge = iptr[*n_huge_runs_p];
   const int64_t first = iptr[*n_first_runs_p];
   const int64_t n_big = iptr[*n_big_centers_p];
   uint64_t acc_total = 0;
+
+  // Workgroup phase.  Every thread sees the same ticket, so the loop exits
+  // together; the barrier at its top keeps the table and the ticket in place
+  // until all wavefronts are done with the previous item.
+  if (first > huge) {
+    int *tbl_blk = reinterpret_cast<int *>(&tbl_all[0][0]);
+    while (true) {
+      __syncthreads();
+      if (threadIdx.x == 0) s_ticket = (int64_t)atomicAdd(next_block, 1ull);
+      __syncthreads();
+      const int64_t bit = huge + s_ticket;
+      if (bit >= first) break;
+      const int2 item = items[bit];
+      const int64_t ob = cptr[item.x] + (int64_t)item.y * chunk;
+      const int64_t oe = min(cptr[item.x + 1], ob + (int64_t)chunk);
+      const int32_t c = others.center(item.x, ob);
+      const int64_t cb = R.begin(c);
+      const int dc = (int)(R.end(c) - cb);
+      const int64_t part = (oe - ob + kWavesPerBlock - 1) / kWavesPerBlock;
+      const int64_t wb = min(oe, ob + w * part), we = min(oe, wb + part);
+      int shift;
+      uint32_t mask;
+      table_build<kBlkTblInts, true>(tbl_blk, col + cb, dc, shift, mask);
+      uint64_t acc = stream_others<I, true, U>(R, col, others, wb, we, seg_end, seg_base,
+                                               &tbl_all[0][0], shift, mask, cb, dc);
+      if (graph_of) {
+        acc = wave_sum(acc);
+        if (lane == 0 && acc) atomicAdd(&count[graph_of[c]], (unsigned long long)acc);
+      } else {
+        acc_total += acc;
+      }
+    }
+  }
+
   int64_t it = 0, it_end = 0;
 
   while (true) {
@@ -346,7 +391,7 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
     if (dc <= kTblMaxDeg) {
       int shift;
       uint32_t mask;
-      table_build(reinterpret_cast<int *>(tbl), col + cb, dc, shift, mask);
+      table_build<kTblInts, false>(reinterpret_cast<int *>(tbl), col + cb, dc, shift, mask);
       acc = stream_others<I, true, U>(R, col, others, ob, oe, seg_end, seg_base, tbl, shift, mask,
                                       cb, dc);
     } else {
@@ -393,7 +438,7 @@ __global__ void k_edge_supports(Rows<I> R, const int32_t *__restrict__ col,
   }
 }
 
-// Centers with d+ >= kBigMinDeg: one workgroup per work item, the center row
+// Centers with d+ >= 4096: one workgroup per work item, the center row
 // in one block-shared LDS table (up to kBigMaxDeg ids at load <= 1/2), the
 // four wavefronts streaming disjoint quarters of the item's others.  Items of
 // these centers come first in the heavy-first order; this kernel takes items
@@ -430,24 +475,9 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restric
     const int64_t wb = min(oe, ob + w * part), we = min(oe, wb + part);
     uint64_t acc;
     if (dc <= kBigMaxDeg) {
-      int logNB = 2;
-      while ((1 << logNB) < dc && (4 << logNB) < kBigTblInts) ++logNB;
-      const int NB = 1 << logNB, shift = 32 - logNB;
-      const uint32_t mask = (uint32_t)NB - 1;
-      for (int i = threadIdx.x; i < 4 * NB; i += blockDim.x) tbl_i[i] = -1;
-      __syncthreads();
-      for (int i = threadIdx.x; i < dc; i += blockDim.x) {
-        const int x = col[cb + i];
-        uint32_t b = bucket_of((uint32_t)x, shift);
-        while (true) {
-          int *sl = tbl_i + 4 * b;
-          if (atomicCAS(sl, -1, x) == -1 || atomicCAS(sl + 1, -1, x) == -1 ||
-              atomicCAS(sl + 2, -1, x) == -1 || atomicCAS(sl + 3, -1, x) == -1)
-            break;
-          b = (b + 1) & mask;
-        }
-      }
-      __syncthreads();
+      int shift;
+      uint32_t mask;
+      table_build<kBigTblInts, true>(tbl_i, col + cb, dc, shift, mask);
       acc = stream_others<I, true, U>(R, col, others, wb, we, seg_end_all[w], seg_base_all[w],
                                       tbl, shift, mask, cb, dc);
       __syncthreads();  // the table is rebuilt for the next item
@@ -634,14 +664,15 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   TC_TRY(iptr.alloc(sizeof(int64_t) * (max_runs + 1), s));
   const int64_t max_items = max_runs + E / chunk + 1;
   TC_TRY(items.alloc(sizeof(int2) * max_items, s));
-  TC_TRY(ctr.alloc(3 * sizeof(unsigned long long), s));
+  TC_TRY(ctr.alloc(4 * sizeof(unsigned long long), s));
   unsigned long long *next_item = ctr.as<unsigned long long>();
   unsigned long long *n_heavy = next_item + 1;
 
   const unsigned gr = grid_for(max_runs + 1, kBlock, 16384);
   const unsigned ge = grid_for(E, kBlock, 16384);
   unsigned long long *next_big = next_item + 2;
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, next_item, 3, 0ull);
+  unsigned long long *next_blk = next_item + 3;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, next_item, 4, 0ull);
   TC_DISPATCH_ROWS(row_ptr_64, I, {
     Rows<I> R{(const I *)row_ptr};
     {
@@ -681,20 +712,21 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
-      const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
+      const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
+      const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - kBlkMinBits + 1);
       const auto kbig = heavy_big_kernel<I, OthersPacked>(unroll);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)run_ptr.as<int64_t>(),
                          OthersPacked{(const uint64_t *)sorted}, (const int2 *)items.as<int2>(),
-                         (const int64_t *)iptr.as<int64_t>(), n_first, chunk, next_big, graph_of,
+                         (const int64_t *)iptr.as<int64_t>(), n_huge, chunk, next_big, graph_of,
                          count);
       const auto kfn = heavy_kernel<I, OthersPacked>(unroll);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R,
                          col, (const int64_t *)run_ptr.as<int64_t>(),
                          OthersPacked{(const uint64_t *)sorted}, (const int2 *)items.as<int2>(),
                          (const int64_t *)(iptr.as<int64_t>() + max_runs),
-                         (const int64_t *)iptr.as<int64_t>(), n_first, n_big_runs, chunk,
-                         next_item, graph_of, count);
+                         (const int64_t *)iptr.as<int64_t>(), n_huge, n_first, n_big_runs, chunk,
+                         next_item, next_blk, graph_of, count);
     }
     TC_HIP(hipGetLastError());
   });
@@ -738,12 +770,12 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
   TC_TRY(iptr.alloc(sizeof(int64_t) * (n + 1), s));
   const int64_t max_items = n + E / chunk + 1;
   TC_TRY(items.alloc(sizeof(int2) * max_items, s));
-  TC_TRY(ctr.alloc(2 * sizeof(unsigned long long), s));
+  TC_TRY(ctr.alloc(3 * sizeof(unsigned long long), s));
 
   const unsigned gn = grid_for(n, kBlock, 16384);
   const unsigned ge = grid_for(E, kBlock, 16384);
   hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(kBlock), 0, s, cnt.as<int32_t>(), n, 0);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr.as<unsigned long long>(), 2, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr.as<unsigned long long>(), 3, 0ull);
   TC_DISPATCH_ROWS(row_ptr_64, I, {
     Rows<I> R{(const I *)row_ptr};
     {
@@ -778,20 +810,22 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
-      const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
+      const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
+      const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - kBlkMinBits + 1);
       const auto kbig = heavy_big_kernel<I, OthersIds>(unroll);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)cptr.as<int64_t>(),
                          OthersIds{(const int32_t *)others.as<int32_t>()},
                          (const int2 *)items.as<int2>(), (const int64_t *)iptr.as<int64_t>(),
-                         n_first, chunk, ctr.as<unsigned long long>() + 1, graph_of, count);
+                         n_huge, chunk, ctr.as<unsigned long long>() + 1, graph_of, count);
       const auto kfn = heavy_kernel<I, OthersIds>(unroll);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)cptr.as<int64_t>(),
                          OthersIds{(const int32_t *)others.as<int32_t>()},
                          (const int2 *)items.as<int2>(), (const int64_t *)(iptr.as<int64_t>() + n),
-                         (const int64_t *)iptr.as<int64_t>(), n_first, n_big_cent, chunk,
-                         ctr.as<unsigned long long>(), graph_of, count);
+                         (const int64_t *)iptr.as<int64_t>(), n_huge, n_first, n_big_cent, chunk,
+                         ctr.as<unsigned long long>(), ctr.as<unsigned long long>() + 2, graph_of,
+                         count);
     }
     TC_HIP(hipGetLastError());
   });
