@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+RMAT_WORKLOADS = {"rmat14": (14, 8, 0), "rmat20": (20, 16, 5), "rmat23": (23, 16, 4)}
 
 
 def _log(msg):
@@ -77,10 +78,16 @@ def cpu_baseline(rp, col, truth_hint=None, budget_s=20.0):
             "sample": f"every {stride}th oriented edge ({n_s} of {E}), u-major order"}
 
 
+TRAFFIC_PROFILES = {"chunglu": "profiles/exact_chunglu117M_hbm_traffic.md",
+                    "rmat23": "profiles/exact_rmat23_hbm_traffic.md"}
+
+
 def measured_traffic(workload: str):
     """HBM bytes per launch of k_heavy from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE scaled by a calibration copy), or None."""
-    path = os.path.join(ROOT, "profiles", f"exact_{workload}117M_hbm_traffic.md")
+    if workload not in TRAFFIC_PROFILES:
+        return None, None
+    path = os.path.join(ROOT, TRAFFIC_PROFILES[workload])
     if not os.path.exists(path):
         return None, None
     traffic = hit = None
@@ -313,12 +320,17 @@ def main():
         _log("calibration copy: 1 GiB read + 1 GiB written")
 
     t0 = time.perf_counter()
-    pairs = config_pairs(args.workload, args.scale_down)
-    t_gen = time.perf_counter() - t0
-    _log(f"generated {len(pairs)} raw pairs in {t_gen:.1f}s")
+    if args.workload.startswith("rmat"):
+        # device twin of the seeded host generator (same pairs, bit for bit)
+        from triangle_counting_hll_amd import batch
 
-    pairs_d = torch.from_numpy(pairs).cuda()
+        scale, ef, seed = RMAT_WORKLOADS[args.workload]
+        pairs_d = batch.rmat_gpu(scale - args.scale_down, ef, seed)
+    else:
+        pairs_d = torch.from_numpy(config_pairs(args.workload, args.scale_down)).cuda()
     torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    _log(f"generated {pairs_d.shape[0]} raw pairs in {t_gen:.1f}s")
     t0 = time.perf_counter()
     csr = gpu_graph.build_csr_gpu(pairs_d, row_ptr_dtype="int64")
     torch.cuda.synchronize()
@@ -359,7 +371,7 @@ def main():
     heavy_avg_ms = heavy_ms / max(heavy_n, 1)
     achieved = B_heavy / (heavy_avg_ms * 1e-3) / 1e9 if heavy_n else None
     traffic, l2_hit = (measured_traffic(args.workload) if not args.scale_down
-                       and args.workload == "chunglu" else (None, None))
+                       else (None, None))
 
     cpu = None
     parity = None
@@ -411,8 +423,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
             "traffic": traffic,
-            "traffic_source": "profiles/exact_chunglu117M_hbm_traffic.md (rocprofv3 --pmc "
-                              "FETCH_SIZE x2.0 + WRITE_SIZE x1.0, calibrated on a 1 GiB copy)"
+            "traffic_source": f"{TRAFFIC_PROFILES[args.workload]} (rocprofv3 --pmc "
+                              "FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy's counts)"
                               if traffic else None,
             "traffic_gbs": traffic / (heavy_avg_ms * 1e-3) / 1e9 if traffic and heavy_n else None,
             "l2_hit_rate": l2_hit,

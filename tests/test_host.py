@@ -88,3 +88,13 @@ def test_alloc_sketch_arrays_and_reference_constants():
     assert gpu_hybrid._THREADS == 256
     gpu_hybrid._require_supported_block_size()
     assert gpu_hybrid.relative_error(0, 0) == 0.0
+
+
+def test_bench_rmat_parameters_match_named_workloads():
+    # bench.py generates the R-MAT workloads with the device twin; its
+    # (scale, edge factor, seed) table must name the same graphs
+    import bench
+
+    for name, (scale, ef, seed) in bench.RMAT_WORKLOADS.items():
+        sd = scale - 9
+        assert np.array_equal(graphgen.config_pairs(name, sd), graphgen.rmat(9, ef, seed=seed))
