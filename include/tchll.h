@@ -70,6 +70,34 @@ typedef struct tc_csr_build tc_csr_build;
 int tc_csr_build_begin(const int32_t *pairs, int64_t m, void *stream,
                        tc_csr_build **out, int64_t *n_nodes, int64_t *n_edges);
 
+/* Edge-list text -> raw pairs (int32 [m][2], file order, self-loops dropped),
+ * the input of tc_csr_build_begin.  `text` (device, n_bytes) holds whole
+ * lines of a SNAP-style edge list; `comment_chars` (host, NUL-terminated,
+ * NULL = "#%") lists the bytes that start a comment line.  Lines, fields and
+ * skips follow experiment/exp_gpu.py:131-143 (_iter_edges_raw): breaks at
+ * '\n' / '\r', leading whitespace skipped, comment and one-field lines
+ * skipped, the first two whitespace-separated fields read as Python int(),
+ * further fields ignored.  A field int() rejects, a negative id or one above
+ * INT32_MAX is an error: TC_ERR_INVALID with *bad_offset (host, optional) =
+ * byte offset of the first such line and *bad_kind (host, optional) = 2 (not
+ * an integer), 3 (negative) or 4 (too large).  *n_pairs (host) = number of
+ * pairs; with pairs == NULL only that count is computed.  Output larger than
+ * max_pairs is an error.  Synchronises `stream` after the counting pass; the
+ * pairs are written asynchronously.  At most (n_bytes + 1) / 4 pairs fit in
+ * any text.  Replaces the two Python passes of experiment/exp_gpu.py:291-313. */
+int tc_parse_edge_list(const uint8_t *text, int64_t n_bytes, const char *comment_chars,
+                       int32_t *pairs, int64_t max_pairs, int64_t *n_pairs,
+                       int64_t *bad_offset, int *bad_kind, void *stream);
+
+/* Raw pairs (int32 [m][2], ids >= 0) -> edge-list text, one
+ * "u<sep>v<newline>" line per pair in order (`sep`, `newline`: host strings
+ * of 1 to 4 bytes; NULL = "\t" and "\n").  *n_bytes (host) = text length;
+ * with text == NULL only that length is computed.  Synchronises `stream`
+ * after the sizing pass.  No reference counterpart: it stages synthetic edge
+ * lists in HBM for tc_parse_edge_list and its round-trip tests. */
+int tc_format_edge_list(const int32_t *pairs, int64_t m, const char *sep, const char *newline,
+                        uint8_t *text, int64_t max_bytes, int64_t *n_bytes, void *stream);
+
 /* Phase 2: writes the forward CSR into caller-owned arrays:
  *   row_ptr  [n+1]  int32 (row_ptr_64 = 0) or int64 (row_ptr_64 = 1)
  *   col_idx  [E]    int32, rows sorted ascending

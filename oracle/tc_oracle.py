@@ -14,6 +14,7 @@ which the tests bound at 1e-12 relative instead.
 
 from __future__ import annotations
 
+import io
 import math
 from collections import defaultdict
 from typing import Dict, List, Optional, Set, Tuple
@@ -203,6 +204,46 @@ def to_csr_numpy(forward_adj: Dict[int, np.ndarray]):
             edge_u[s + k] = i
             edge_v[s + k] = j
     return row_ptr, col_idx, degrees, node_ids, edge_u, edge_v
+
+
+# ---------------------------------------------------------------------------
+# Edge-list text  (experiment/exp_gpu.py:93, 131-143, 291-313)
+# ---------------------------------------------------------------------------
+
+COMMENT_CHARS = "#%"  # experiment/exp_gpu.py:93
+
+
+def iter_edges_raw(text: bytes, comment_chars: str = COMMENT_CHARS):
+    """``_iter_edges_raw`` (experiment/exp_gpu.py:131-143) over an in-memory file.
+
+    The text is read as the reference's ``open(path)`` reads a file: UTF-8 with
+    universal newlines, one ``line`` per iteration.
+    """
+    with io.TextIOWrapper(io.BytesIO(text), encoding="utf-8", newline=None) as f:
+        for line in f:
+            line = line.strip()
+            if not line or line[0] in comment_chars:
+                continue
+            parts = line.split()
+            if len(parts) < 2:
+                continue
+            u, v = int(parts[0]), int(parts[1])
+            if u != v:
+                yield u, v
+
+
+def parse_edge_list(text: bytes, comment_chars: str = COMMENT_CHARS) -> np.ndarray:
+    """Raw pairs int64 [m, 2] in file order (the edge buffer of exp_gpu.py:299-313,
+    before canonicalisation).  Ids must fit the reference's int32 buffer
+    (exp_gpu.py:300): negative ids raise ValueError, larger ones OverflowError."""
+    out = []
+    for u, v in iter_edges_raw(text, comment_chars):
+        if u < 0 or v < 0:
+            raise ValueError("negative node id")
+        if u > 2**31 - 1 or v > 2**31 - 1:
+            raise OverflowError("node id does not fit int32")
+        out.append((u, v))
+    return np.array(out, dtype=np.int64).reshape(-1, 2)
 
 
 def csr_from_pairs(pairs: np.ndarray):

@@ -1,0 +1,138 @@
+"""Edge-list ingest on the GPU against the text oracle (pairs and CSR bit-exact)."""
+
+import gzip
+
+import numpy as np
+import pytest
+
+from oracle import tc_oracle as O
+from triangle_counting_hll_amd import gpu_graph, graphgen, ingest
+
+pytestmark = pytest.mark.gpu
+
+TRICKY = (b"# comment\n\n   \t \n1 2\n  3\t4  extra fields x y\n5\nabc\n% other comment\n"
+          b"  # indented comment\n6 6\n+7 1_0\n-0 9\r\n11 12\r13\x0b14\x0c\n\x1c15\x1f16\x1e\n"
+          b"17 18")
+
+
+def _parse(text, **kw):
+    return ingest.parse_edge_list_gpu(text, **kw).cpu().numpy().astype(np.int64)
+
+
+def test_tricky_lines_match_oracle(device):
+    assert np.array_equal(_parse(TRICKY), O.parse_edge_list(TRICKY))
+    assert _parse(b"").shape == (0, 2)
+    assert _parse(b"# only a comment\n\n").shape == (0, 2)
+    assert np.array_equal(_parse(b"1 2\n; 3 4\n", comment_chars=";"), [[1, 2]])
+    assert ingest.count_edge_list_gpu(TRICKY) == len(O.parse_edge_list(TRICKY))
+
+
+@pytest.mark.parametrize("sep,newline", [("\t", "\n"), (" ", "\r\n"), ("\t", "\r")])
+def test_rmat_text_round_trip(device, sep, newline):
+    pairs = graphgen.rmat(14, 8, seed=0)
+    pairs = np.concatenate([pairs, [[7, 7], [0, 2147483647]]])
+    text = graphgen.edge_list_text(pairs, sep=sep, newline=newline, header="# SNAP\n% x\n")
+    got = _parse(text)
+    assert np.array_equal(got, O.parse_edge_list(text))
+    # the device writer produces the same bytes as the host writer
+    dev_text = ingest.format_edge_list_gpu(pairs[2:], sep=sep, newline=newline)
+    assert bytes(dev_text.cpu().numpy()) == graphgen.edge_list_text(pairs[2:], sep, newline)
+
+
+def test_lines_across_tiles_and_long_lines(device):
+    # line lengths chosen so that lines start on every byte of a 4 KiB tile and
+    # some run past the 1 KiB staging halo (long trailing fields)
+    rng = np.random.default_rng(5)
+    lines = []
+    want = []
+    for i in range(6000):
+        u, v = int(rng.integers(0, 10**rng.integers(1, 10))), int(rng.integers(0, 1000))
+        pad = " " * int(rng.integers(0, 40))
+        tail = ("\t" + "x" * int(rng.integers(900, 2500))) if i % 97 == 0 else ""
+        lines.append(f"{pad}{u}{pad} {v}{tail}")
+        if u != v:
+            want.append((u, v))
+    text = "\n".join(lines).encode()
+    got = _parse(text)
+    assert np.array_equal(got, np.array(want, dtype=np.int64).reshape(-1, 2))
+    assert np.array_equal(got, O.parse_edge_list(text))
+    # the same bytes at an odd device address take the unaligned staging path
+    import torch
+
+    buf = torch.zeros(len(text) + 3, dtype=torch.uint8, device="cuda")
+    buf[3:] = torch.from_numpy(np.frombuffer(text, np.uint8).copy()).cuda()
+    assert np.array_equal(_parse(buf[3:]), got)
+
+
+@pytest.mark.parametrize("bad,exc,kind", [(b"1 x\n", ValueError, 2), (b"-3 4\n", ValueError, 3),
+                                          (b"1 2147483648\n", OverflowError, 4),
+                                          (b"1__2 5\n", ValueError, 2)])
+def test_bad_line_reports_its_line(device, bad, exc, kind):
+    text = b"# h\n1 2\n3 4\r\n" + bad + b"5 6\n" + b"oops oops\n"
+    with pytest.raises(exc, match="line 4"):
+        ingest.parse_edge_list_gpu(text)
+    if exc is not OverflowError:
+        try:
+            ingest.parse_edge_list_gpu(text)
+        except ingest.EdgeListError as e:
+            assert e.kind == kind and e.offset == text.index(bad)
+
+
+def test_file_loader_chunks_and_gzip(device, tmp_path):
+    pairs = graphgen.rmat(13, 8, seed=4)
+    text = graphgen.edge_list_text(pairs, sep=" ", newline="\r\n", header="# nodes 8192\n")
+    want = O.parse_edge_list(text)
+    path = tmp_path / "g.txt"
+    path.write_bytes(text)
+    for chunk in (1 << 12, 1 << 16, 1 << 28):
+        got = ingest.load_edge_list_gpu(str(path), chunk_bytes=chunk).cpu().numpy()
+        assert np.array_equal(got, want)
+    gz = tmp_path / "g.txt.gz"
+    gz.write_bytes(gzip.compress(text))
+    assert np.array_equal(ingest.load_edge_list_gpu(str(gz), chunk_bytes=1 << 14).cpu().numpy(),
+                          want)
+    bad = tmp_path / "bad.txt"
+    bad.write_bytes(text + b"1 2\nx y\n")
+    n_lines = text.count(b"\n") + 2
+    with pytest.raises(ValueError, match=f"bad.txt:{n_lines}:"):
+        ingest.load_edge_list_gpu(str(bad), chunk_bytes=1 << 12)
+
+
+def test_load_graph_matches_oracle_loader(device, tmp_path):
+    pairs = graphgen.rmat(12, 16, seed=6)
+    pairs = np.concatenate([pairs, pairs[:100, ::-1], [[3, 3]]])  # duplicates, reversed, loop
+    path = tmp_path / "g.txt"
+    path.write_bytes(graphgen.edge_list_text(pairs, header="# FromNodeId\tToNodeId\n"))
+    (row_ptr, col, fdeg, eu, ev, fadj, edges_keep, node_ids, n, E,
+     t_load) = ingest.load_graph_gpu(str(path), need_forward_adj=True)
+    rp_o, col_o, fdeg_o, nid_o, eu_o, ev_o, _ = O.csr_from_pairs(pairs)
+    assert row_ptr.dtype.itemsize == 4 and n == len(nid_o) and E == len(col_o) and t_load > 0
+    assert np.array_equal(row_ptr.cpu().numpy(), rp_o)
+    assert np.array_equal(col.cpu().numpy(), col_o)
+    assert np.array_equal(fdeg.cpu().numpy(), fdeg_o)
+    assert np.array_equal(eu.cpu().numpy(), eu_o) and np.array_equal(ev.cpu().numpy(), ev_o)
+    assert np.array_equal(node_ids.cpu().numpy(), nid_o)
+    canon = np.unique(np.sort(pairs[pairs[:, 0] != pairs[:, 1]], axis=1), axis=0)
+    assert np.array_equal(edges_keep.cpu().numpy(), canon)
+    assert all(np.array_equal(fadj[i], col_o[rp_o[i]:rp_o[i + 1]]) for i in range(0, n, 97))
+
+
+def test_chunglu_device_round_trip(device):
+    # full-size Chung-Lu pairs: device writer -> device parser gives the pairs
+    # back, and the CSR built from the text equals the CSR built from the pairs
+    import torch
+
+    pairs = torch.from_numpy(graphgen.config_pairs("chunglu")).cuda()
+    text = ingest.format_edge_list_gpu(pairs)
+    back = ingest.parse_edge_list_gpu(text)
+    keep = pairs[:, 0] != pairs[:, 1]
+    assert torch.equal(back, pairs[keep])
+    a = gpu_graph.build_csr_gpu(back)
+    b = gpu_graph.build_csr_gpu(pairs)
+    assert torch.equal(a.row_ptr, b.row_ptr) and torch.equal(a.col_idx, b.col_idx)
+
+
+def test_writer_rejects_negative_ids(device):
+    with pytest.raises(ValueError, match="negative"):
+        ingest.format_edge_list_gpu(np.array([[1, 2], [-1, 3]], dtype=np.int32))
+    assert ingest.format_edge_list_gpu(np.zeros((0, 2), np.int32)).numel() == 0

@@ -1,0 +1,73 @@
+"""Edge-list text oracle (experiment/exp_gpu.py:131-143) pinned by round trips,
+hand-derived cases and numpy.loadtxt."""
+
+import io
+
+import numpy as np
+import pytest
+
+from oracle import tc_oracle as O
+from triangle_counting_hll_amd import graphgen
+
+
+def _no_loops(p):
+    p = np.asarray(p, dtype=np.int64).reshape(-1, 2)
+    return p[p[:, 0] != p[:, 1]]
+
+
+@pytest.mark.parametrize("sep,newline", [("\t", "\n"), (" ", "\n"), (" ", "\r\n"), ("\t", "\r"),
+                                         ("  \t ", "\n")])
+def test_round_trip_of_written_pairs(sep, newline):
+    pairs = graphgen.rmat(9, 8, seed=3)
+    pairs = np.concatenate([pairs, [[5, 5], [0, 2147483647]]])
+    text = graphgen.edge_list_text(pairs, sep=sep, newline=newline,
+                                   header="# Directed graph\n# Nodes: 512\n% konect\n")
+    assert np.array_equal(O.parse_edge_list(text), _no_loops(pairs))
+
+
+def test_writer_matches_numpy_formatting():
+    pairs = np.array([[0, 1], [10, 2], [123456789, 7], [2147483647, 99]])
+    want = "".join(f"{a}\t{b}\n" for a, b in pairs).encode()
+    assert graphgen.edge_list_text(pairs) == want
+    assert graphgen.edge_list_text(np.zeros((0, 2), np.int64)) == b""
+
+
+def test_agrees_with_numpy_loadtxt():
+    pairs = graphgen.erdos_renyi(300, 0.05, seed=2)
+    text = graphgen.edge_list_text(pairs, sep=" ", header="# a\n% b\n")
+    want = np.loadtxt(io.BytesIO(text), dtype=np.int64, comments=("#", "%")).reshape(-1, 2)
+    assert np.array_equal(O.parse_edge_list(text), _no_loops(want))
+
+
+def test_line_rules_hand_derived():
+    text = (b"# comment\n"
+            b"\n"
+            b"   \t \n"           # whitespace only
+            b"1 2\n"
+            b"  3\t4  extra fields x y\n"
+            b"5\n"                # one field: skipped
+            b"abc\n"              # one field, not an integer: skipped, no error
+            b"% other comment\n"
+            b"  # indented comment\n"
+            b"6 6\n"              # self-loop
+            b"+7 1_0\n"           # int() accepts a sign and single underscores
+            b"-0 9\r\n"
+            b"11 12\r"
+            b"13\x0b14\x0c\n"     # vertical tab and form feed separate fields
+            b"15 16")             # no final break
+    want = [[1, 2], [3, 4], [7, 10], [0, 9], [11, 12], [13, 14], [15, 16]]
+    assert O.parse_edge_list(text).tolist() == want
+    assert O.parse_edge_list(b"1 2\n; 3 4\n", comment_chars=";").tolist() == [[1, 2]]
+    with pytest.raises(ValueError):  # '#' is data when it is not a comment byte
+        O.parse_edge_list(b"1 2\n# 3 4\n", comment_chars="%")
+    assert O.parse_edge_list(b"").shape == (0, 2)
+
+
+@pytest.mark.parametrize("text,exc", [(b"1 x\n", ValueError), (b"x 1\n", ValueError),
+                                      (b"1 2.0\n", ValueError), (b"0x1 2\n", ValueError),
+                                      (b"1__2 3\n", ValueError), (b"_1 2\n", ValueError),
+                                      (b"1_ 2\n", ValueError), (b"-1 2\n", ValueError),
+                                      (b"1 2147483648\n", OverflowError)])
+def test_bad_lines_raise(text, exc):
+    with pytest.raises(exc):
+        O.parse_edge_list(b"1 2\n" + text)

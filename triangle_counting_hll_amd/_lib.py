@@ -41,6 +41,8 @@ _SIGNATURES = {
     "tc_csr_build_begin": (I32, [P, I64, P, P, P, P]),
     "tc_csr_build_finish": (I32, [P, P, I32, P, P, P, P, P, P, P]),
     "tc_csr_build_free": (None, [P]),
+    "tc_parse_edge_list": (I32, [P, I64, ctypes.c_char_p, P, I64, P, P, P, P]),
+    "tc_format_edge_list": (I32, [P, I64, ctypes.c_char_p, ctypes.c_char_p, P, I64, P, P]),
     "tc_degree_buckets": (I32, [P, I32, I64, P, P, P]),
     "tc_exact_count": (I32, [P, I32, P, I64, P, P, I64, P, P, P]),
     "tc_edge_supports": (I32, [P, I32, P, P, P, I64, P, P]),

@@ -1,0 +1,378 @@
+// Edge-list text ingest on gfx950: SNAP-style "u v" lines -> raw int32 pairs.
+//
+// Line rules (those of the reference's streaming reader,
+// experiment/exp_gpu.py:131-143, _iter_edges_raw):
+//   * a line ends at '\n' or '\r' (universal newlines: "\r\n" is one break,
+//     the empty line it would leave is skipped anyway);
+//   * leading whitespace is skipped; an empty line, or one whose first byte is
+//     a comment byte ('#' and '%' by default), is skipped;
+//   * the line is split on whitespace; fewer than two fields -> skipped;
+//   * the first two fields must be base-10 integers as Python's int() reads
+//     them ([+-]digits, single '_' between digits); more fields are ignored;
+//   * u == v (self-loop) -> skipped.
+// A field that is not an integer is an error (the reference raises
+// ValueError there).  Negative ids and ids above INT32_MAX are errors too:
+// the reference stores ids in an int32 array (exp_gpu.py:300), where they
+// would overflow or index from the end.
+//
+// Layout of the work: the text is cut into 4 KiB tiles, one workgroup per
+// tile.  The workgroup stages its tile plus a 1 KiB halo in LDS with 16-byte
+// loads; each thread owns 16 bytes and parses every line that STARTS in
+// them (reading past the halo from global memory for very long lines).
+// Pass 1 counts the pairs of each tile and records the first bad line; a
+// scan over the tile counts gives each tile its output offset; pass 2
+// parses again and writes the pairs in file order.
+#include "tc_common.hpp"
+#include "tc_exact.hpp"
+
+namespace tc {
+
+constexpr int kTileBytes = 4096;
+constexpr int kSegBytes = kTileBytes / kBlock;  // bytes owned by one thread
+constexpr int kHalo = 1024;                     // staged bytes past the tile
+constexpr int kPre = 16;                        // staged bytes before the tile
+constexpr int kStage = kPre + kTileBytes + kHalo;
+
+enum : int {
+  kLineSkip = 0,
+  kLinePair = 1,
+  kLineBadInt = 2,    // a field that int() rejects
+  kLineNegative = 3,  // an id below zero
+  kLineRange = 4,     // an id above INT32_MAX
+};
+
+struct CommentSet {
+  uint32_t bits[8];  // 256-bit membership set over byte values
+};
+
+__device__ __forceinline__ bool is_break(int c) { return c == '\n' || c == '\r'; }
+// str.split() whitespace among single bytes, minus the two line breaks
+__device__ __forceinline__ bool is_space(int c) {
+  return c == ' ' || c == '\t' || c == 0x0b || c == 0x0c || (c >= 0x1c && c <= 0x1f);
+}
+
+// Byte p of the text: from the LDS stage when staged, else from global memory;
+// -1 past the end.
+struct TextBytes {
+  const uint8_t *stage;  // stage[i] = text[t0 - kPre + i]
+  const uint8_t *text;
+  int64_t n, t0;
+  __device__ __forceinline__ int at(int64_t p) const {
+    if (p >= n) return -1;
+    const int64_t i = p - t0 + kPre;
+    return (i >= 0 && i < kStage) ? (int)stage[i] : (int)text[p];
+  }
+};
+
+// One field starting at p (a byte that is neither space nor break); p ends on
+// the byte after the field.
+__device__ __forceinline__ int parse_field(const TextBytes &T, int64_t &p, int32_t &out) {
+  int c = T.at(p);
+  bool neg = false;
+  if (c == '+' || c == '-') {
+    neg = (c == '-');
+    c = T.at(++p);
+  }
+  int64_t v = 0;
+  bool digits = false, ok = true, big = false, under = false;
+  while (c >= 0 && !is_break(c) && !is_space(c)) {
+    if (c >= '0' && c <= '9') {
+      if (!big) {
+        v = v * 10 + (c - '0');
+        big = v > 0x7fffffffLL;
+      }
+      digits = true;
+      under = false;
+    } else if (c == '_' && digits && !under) {
+      under = true;
+    } else {
+      ok = false;
+    }
+    c = T.at(++p);
+  }
+  if (!digits || under || !ok) return kLineBadInt;
+  if (neg && (v != 0 || big)) return kLineNegative;
+  if (big) return kLineRange;
+  out = (int32_t)v;
+  return kLinePair;
+}
+
+__device__ __forceinline__ int parse_line(const TextBytes &T, int64_t p, const CommentSet &cs,
+                                          int32_t &u, int32_t &v) {
+  int c = T.at(p);
+  while (c >= 0 && is_space(c)) c = T.at(++p);
+  if (c < 0 || is_break(c)) return kLineSkip;
+  if ((cs.bits[c >> 5] >> (c & 31)) & 1u) return kLineSkip;
+  const int s1 = parse_field(T, p, u);
+  c = T.at(p);
+  while (c >= 0 && is_space(c)) c = T.at(++p);
+  if (c < 0 || is_break(c)) return kLineSkip;  // one field only
+  const int s2 = parse_field(T, p, v);
+  if (s1 != kLinePair) return s1;  // int(parts[0]) is evaluated first
+  if (s2 != kLinePair) return s2;
+  return u == v ? kLineSkip : kLinePair;
+}
+
+// Exclusive scan of one int per thread over the workgroup; *total gets the sum.
+__device__ __forceinline__ int block_exclusive_scan(int x, int *wave_tot, int *total) {
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int inc = wave_inclusive_scan(x);
+  if (lane == kWave - 1) wave_tot[w] = inc;
+  __syncthreads();
+  int before = 0, sum = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / kWave; ++i) {
+    const int t = wave_tot[i];
+    before += (i < w) ? t : 0;
+    sum += t;
+  }
+  *total = sum;
+  return before + inc - x;
+}
+
+template <bool kWrite>
+__global__ void __launch_bounds__(kBlock)
+k_parse_tiles(const uint8_t *__restrict__ text, int64_t n, int aligned, CommentSet cs,
+              int32_t *__restrict__ tile_count, const int64_t *__restrict__ tile_off,
+              int2 *__restrict__ pairs, unsigned long long *__restrict__ bad) {
+  __shared__ __align__(16) uint8_t stage[kStage];
+  __shared__ int wave_tot[kBlock / kWave];
+  const int64_t t0 = (int64_t)blockIdx.x * kTileBytes;
+  const int64_t s0 = t0 - kPre;  // text offset of stage[0]
+  // stage [t0 - kPre, t0 + kTileBytes + kHalo); outside the text reads as '\n'
+  if (aligned) {
+    uint4 *st4 = reinterpret_cast<uint4 *>(stage);
+    for (int i = threadIdx.x; i < kStage / 16; i += kBlock) {
+      const int64_t p = s0 + 16 * (int64_t)i;
+      if (p >= 0 && p + 16 <= n) {
+        st4[i] = *reinterpret_cast<const uint4 *>(text + p);
+      } else {
+        for (int b = 0; b < 16; ++b) {
+          const int64_t q = p + b;
+          stage[16 * i + b] = (q >= 0 && q < n) ? text[q] : (uint8_t)'\n';
+        }
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < kStage; i += kBlock) {
+      const int64_t q = s0 + i;
+      stage[i] = (q >= 0 && q < n) ? text[q] : (uint8_t)'\n';
+    }
+  }
+  __syncthreads();
+
+  const TextBytes T{stage, text, n, t0};
+  const int64_t seg0 = t0 + (int64_t)threadIdx.x * kSegBytes;
+  // lines starting in this thread's bytes: a non-break byte after a break
+  int cnt = 0;
+  for (int q = 0; q < kSegBytes; ++q) {
+    const int64_t p = seg0 + q;
+    if (p >= n) break;
+    const int i = (int)(p - s0);
+    if (is_break(stage[i]) || !is_break(stage[i - 1])) continue;
+    int32_t u, v;
+    const int st = parse_line(T, p, cs, u, v);
+    if (st == kLinePair) {
+      ++cnt;
+    } else if (!kWrite && st != kLineSkip) {
+      atomicMin(bad, ((unsigned long long)p << 3) | (unsigned long long)st);
+    }
+  }
+  int total;
+  const int before = block_exclusive_scan(cnt, wave_tot, &total);
+  if constexpr (!kWrite) {
+    if (threadIdx.x == 0) tile_count[blockIdx.x] = total;
+  } else {
+    if (cnt == 0) return;
+    int64_t o = tile_off[blockIdx.x] + before;
+    for (int q = 0; q < kSegBytes; ++q) {
+      const int64_t p = seg0 + q;
+      if (p >= n) break;
+      const int i = (int)(p - s0);
+      if (is_break(stage[i]) || !is_break(stage[i - 1])) continue;
+      int32_t u, v;
+      if (parse_line(T, p, cs, u, v) == kLinePair) pairs[o++] = make_int2(u, v);
+    }
+  }
+}
+
+}  // namespace tc
+
+extern "C" int tc_parse_edge_list(const uint8_t *text, int64_t n_bytes, const char *comment_chars,
+                                  int32_t *pairs, int64_t max_pairs, int64_t *n_pairs,
+                                  int64_t *bad_offset, int *bad_kind, void *stream) {
+  using namespace tc;
+  TC_CHECK(n_pairs, TC_ERR_INVALID, "n_pairs is required");
+  TC_CHECK(n_bytes >= 0 && max_pairs >= 0, TC_ERR_INVALID, "negative size");
+  TC_CHECK(n_bytes == 0 || text, TC_ERR_INVALID, "text is required");
+  TC_CHECK(((uintptr_t)pairs & 7) == 0, TC_ERR_INVALID, "pairs must be 8-byte aligned");
+  const int64_t n_tiles = (n_bytes + kTileBytes - 1) / kTileBytes;
+  TC_CHECK(n_tiles < INT32_MAX, TC_ERR_INVALID, "text of %lld bytes is too long for one call",
+           (long long)n_bytes);
+  TC_TRY(ensure_init());
+  *n_pairs = 0;
+  if (bad_offset) *bad_offset = -1;
+  if (bad_kind) *bad_kind = 0;
+  if (n_bytes == 0) return TC_OK;
+  hipStream_t s = (hipStream_t)stream;
+
+  CommentSet cs{};
+  const char *cc = comment_chars ? comment_chars : "#%";
+  for (const unsigned char *c = (const unsigned char *)cc; *c; ++c)
+    cs.bits[*c >> 5] |= 1u << (*c & 31);
+  const int aligned = ((uintptr_t)text & 15) == 0;
+
+  Scratch cnt, off, bad;
+  TC_TRY(cnt.alloc(sizeof(int32_t) * n_tiles, s));
+  TC_TRY(off.alloc(sizeof(int64_t) * (n_tiles + 1), s));
+  TC_TRY(bad.alloc(sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, bad.as<unsigned long long>(), 1,
+                     ~0ull);
+  hipLaunchKernelGGL(k_parse_tiles<false>, dim3((unsigned)n_tiles), dim3(kBlock), 0, s, text,
+                     n_bytes, aligned, cs, cnt.as<int32_t>(), (const int64_t *)nullptr,
+                     (int2 *)nullptr, bad.as<unsigned long long>());
+  TC_HIP(hipGetLastError());
+  TC_TRY(exclusive_scan<int32_t>(cnt.as<int32_t>(), off.as<int64_t>(), n_tiles, nullptr, s));
+  int64_t total = 0;
+  unsigned long long first_bad = 0;
+  if (hipMemcpyAsync(&total, off.as<int64_t>() + n_tiles, sizeof(int64_t), hipMemcpyDeviceToHost,
+                     s) != hipSuccess ||
+      hipMemcpyAsync(&first_bad, bad.as<unsigned long long>(), sizeof(first_bad),
+                     hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error(TC_ERR_HIP, "edge-list count pass failed: %s",
+              hipGetErrorString(hipGetLastError()));
+    return TC_ERR_HIP;
+  }
+  if (first_bad != ~0ull) {
+    const int64_t at = (int64_t)(first_bad >> 3);
+    const int kind = (int)(first_bad & 7);
+    if (bad_offset) *bad_offset = at;
+    if (bad_kind) *bad_kind = kind;
+    set_error(TC_ERR_INVALID, "line at byte %lld: %s", (long long)at,
+              kind == kLineBadInt     ? "expected integer node ids"
+              : kind == kLineNegative ? "negative node id"
+                                      : "node id does not fit int32");
+    return TC_ERR_INVALID;
+  }
+  *n_pairs = total;
+  if (!pairs) return TC_OK;  // sizing query
+  if (total > max_pairs) {
+    set_error(TC_ERR_INVALID, "%lld pairs do not fit the %lld-pair output", (long long)total,
+              (long long)max_pairs);
+    return TC_ERR_INVALID;
+  }
+  if (total > 0) {
+    hipLaunchKernelGGL(k_parse_tiles<true>, dim3((unsigned)n_tiles), dim3(kBlock), 0, s, text,
+                       n_bytes, aligned, cs, cnt.as<int32_t>(), (const int64_t *)off.as<int64_t>(),
+                       reinterpret_cast<int2 *>(pairs), (unsigned long long *)nullptr);
+    TC_HIP(hipGetLastError());
+  }
+  return TC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The inverse: raw pairs -> SNAP-style text ("u<sep>v<newline>" per pair).
+// No reference counterpart; used to stage large synthetic edge lists in HBM
+// and for encode -> parse round trips.
+// ---------------------------------------------------------------------------
+namespace tc {
+
+struct Delims {
+  uint8_t sep[4], nl[4];
+  int n_sep, n_nl;
+};
+
+__device__ __forceinline__ int dec_width(uint32_t x) {
+  int w = 1;
+  while (x >= 10u) {
+    x /= 10u;
+    ++w;
+  }
+  return w;
+}
+
+__global__ void k_line_bytes(const int2 *__restrict__ pairs, int64_t m, Delims d,
+                             int32_t *__restrict__ len, unsigned int *__restrict__ negative) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int2 e = pairs[i];
+    if ((e.x | e.y) < 0) atomicOr(negative, 1u);
+    len[i] = dec_width((uint32_t)e.x) + dec_width((uint32_t)e.y) + d.n_sep + d.n_nl;
+  }
+}
+
+__device__ __forceinline__ int64_t put_dec(uint8_t *__restrict__ out, int64_t o, uint32_t x) {
+  const int w = dec_width(x);
+  for (int k = w - 1; k >= 0; --k) {
+    out[o + k] = (uint8_t)('0' + x % 10u);
+    x /= 10u;
+  }
+  return o + w;
+}
+
+__global__ void k_write_lines(const int2 *__restrict__ pairs, int64_t m, Delims d,
+                              const int64_t *__restrict__ off, uint8_t *__restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int2 e = pairs[i];
+    int64_t o = put_dec(out, off[i], (uint32_t)e.x);
+    for (int j = 0; j < d.n_sep; ++j) out[o++] = d.sep[j];
+    o = put_dec(out, o, (uint32_t)e.y);
+    for (int j = 0; j < d.n_nl; ++j) out[o++] = d.nl[j];
+  }
+}
+
+}  // namespace tc
+
+extern "C" int tc_format_edge_list(const int32_t *pairs, int64_t m, const char *sep,
+                                   const char *newline, uint8_t *text, int64_t max_bytes,
+                                   int64_t *n_bytes, void *stream) {
+  using namespace tc;
+  TC_CHECK(n_bytes, TC_ERR_INVALID, "n_bytes is required");
+  TC_CHECK(m >= 0 && max_bytes >= 0, TC_ERR_INVALID, "negative size");
+  TC_CHECK(m == 0 || pairs, TC_ERR_INVALID, "pairs is required");
+  TC_CHECK(((uintptr_t)pairs & 7) == 0, TC_ERR_INVALID, "pairs must be 8-byte aligned");
+  Delims d{};
+  const char *sp = sep ? sep : "\t", *nl = newline ? newline : "\n";
+  d.n_sep = (int)strlen(sp);
+  d.n_nl = (int)strlen(nl);
+  TC_CHECK(d.n_sep >= 1 && d.n_sep <= 4 && d.n_nl >= 1 && d.n_nl <= 4, TC_ERR_INVALID,
+           "separator and newline must be 1 to 4 bytes");
+  memcpy(d.sep, sp, d.n_sep);
+  memcpy(d.nl, nl, d.n_nl);
+  TC_TRY(ensure_init());
+  *n_bytes = 0;
+  if (m == 0) return TC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Scratch len, off, neg;
+  TC_TRY(len.alloc(sizeof(int32_t) * m, s));
+  TC_TRY(off.alloc(sizeof(int64_t) * (m + 1), s));
+  TC_TRY(neg.alloc(sizeof(unsigned int), s));
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, neg.as<int32_t>(), 1, 0);
+  const int2 *pp = reinterpret_cast<const int2 *>(pairs);
+  hipLaunchKernelGGL(k_line_bytes, dim3(grid_for(m)), dim3(kBlock), 0, s, pp, m, d,
+                     len.as<int32_t>(), neg.as<unsigned int>());
+  TC_HIP(hipGetLastError());
+  TC_TRY(exclusive_scan<int32_t>(len.as<int32_t>(), off.as<int64_t>(), m, nullptr, s));
+  int64_t total = 0;
+  unsigned int negative = 0;
+  if (hipMemcpyAsync(&total, off.as<int64_t>() + m, sizeof(int64_t), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipMemcpyAsync(&negative, neg.as<unsigned int>(), sizeof(negative), hipMemcpyDeviceToHost,
+                     s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error(TC_ERR_HIP, "edge-list sizing pass failed: %s",
+              hipGetErrorString(hipGetLastError()));
+    return TC_ERR_HIP;
+  }
+  TC_CHECK(!negative, TC_ERR_INVALID, "pairs hold a negative id");
+  *n_bytes = total;
+  if (!text) return TC_OK;  // sizing query
+  TC_CHECK(total <= max_bytes, TC_ERR_INVALID, "%lld bytes of text do not fit %lld",
+           (long long)total, (long long)max_bytes);
+  hipLaunchKernelGGL(k_write_lines, dim3(grid_for(m)), dim3(kBlock), 0, s, pp, m, d,
+                     (const int64_t *)off.as<int64_t>(), text);
+  TC_HIP(hipGetLastError());
+  return TC_OK;
+}
