@@ -158,6 +158,90 @@ def run_hybrid_extra(csr, truth, reps=3):
     return out
 
 
+def workload_pairs(workload: str, scale_down: int = 0):
+    """Raw pairs of a named workload on the device."""
+    import torch
+
+    from triangle_counting_hll_amd.graphgen import config_pairs
+
+    if workload.startswith("rmat"):
+        # device twin of the seeded host generator (same pairs, bit for bit)
+        from triangle_counting_hll_amd import batch
+
+        scale, ef, seed = RMAT_WORKLOADS[workload]
+        return batch.rmat_gpu(scale - scale_down, ef, seed)
+    return torch.from_numpy(config_pairs(workload, scale_down)).cuda()
+
+
+def run_ingest_extra(workload, ref_csr, reps=5):
+    """Edge-list text -> raw pairs -> CSR on the device (experiment/exp_gpu.py:264-389).
+
+    The workload's raw pairs are written as a tab-separated SNAP-style text in
+    HBM (``tc_format_edge_list``), then parsed back (``tc_parse_edge_list``)
+    and built into the CSR.  Checks: the parsed pairs equal the input pairs
+    minus self-loops, and the CSR equals the one built from the pairs.  The
+    PCIe-inclusive rate copies the text from pinned host memory first.  The
+    CPU figure is the oracle's restatement of the reference's Python reader
+    (_iter_edges_raw, exp_gpu.py:131-143) on the first 2M lines, one thread.
+    """
+    import torch
+
+    from oracle import tc_oracle as O
+    from triangle_counting_hll_amd import gpu_graph, ingest
+
+    pairs = workload_pairs(workload)
+    text = ingest.format_edge_list_gpu(pairs)
+    nb = text.numel()
+    out = torch.empty((ingest.max_pairs(nb), 2), dtype=torch.int32, device="cuda")
+
+    def parse():
+        return ingest._parse_device(text, nb, ingest.COMMENT_CHARS, out)
+
+    (n, bad, _), parse_ms = _timed(parse, reps)
+    keep = pairs[:, 0] != pairs[:, 1]
+    round_trip = bad < 0 and torch.equal(out[:n], pairs[keep])
+
+    def text_to_csr():
+        k, _, _ = ingest._parse_device(text, nb, ingest.COMMENT_CHARS, out)
+        return gpu_graph.build_csr_gpu(out[:k], row_ptr_dtype="int64")
+
+    csr, csr_ms = _timed(text_to_csr, 2)
+    same_csr = (torch.equal(csr.row_ptr, ref_csr.row_ptr)
+                and torch.equal(csr.col_idx, ref_csr.col_idx))
+    del csr
+    host = torch.empty(nb, dtype=torch.uint8, pin_memory=True)
+    host.copy_(text)
+    text_h = text  # reuse the device buffer as the copy target
+
+    def h2d_parse():
+        text_h.copy_(host, non_blocking=True)
+        return ingest._parse_device(text_h, nb, ingest.COMMENT_CHARS, out)
+
+    _, h2d_ms = _timed(h2d_parse, 2)
+    sample = bytes(host[: min(nb, 1 << 26)].numpy())
+    cut = 0
+    for _ in range(2_000_000):
+        nxt = sample.find(b"\n", cut)
+        if nxt < 0:
+            break
+        cut = nxt + 1
+    t0 = time.perf_counter()
+    n_cpu = sum(1 for _ in O.iter_edges_raw(sample[:cut]))
+    cpu_s = time.perf_counter() - t0
+    lines = int(sample[:cut].count(b"\n"))
+    return {
+        "text_bytes": nb, "lines": int(pairs.shape[0]), "pairs": int(n),
+        "parse_ms": parse_ms, "parse_gbs": nb / (parse_ms * 1e-3) / 1e9,
+        "parse_lines_per_s": pairs.shape[0] / (parse_ms * 1e-3),
+        "text_to_csr_ms": csr_ms, "h2d_plus_parse_ms": h2d_ms,
+        "h2d_plus_parse_gbs": nb / (h2d_ms * 1e-3) / 1e9,
+        "round_trip_equal": bool(round_trip), "csr_equal_to_pairs_build": bool(same_csr),
+        "cpu_reference_reader": {"lines": lines, "pairs": n_cpu, "seconds": cpu_s,
+                                 "lines_per_s": lines / cpu_s, "threads": 1,
+                                 "sample": "first 2M lines of the same text"},
+    }
+
+
 def run_batch_extra(args, reps=10):
     """4,096 graphs (half ER, half R-MAT, seed 3) in one HIP-graph launch vs per-graph calls."""
     import torch
@@ -303,7 +387,6 @@ def main():
 
     from oracle import cbind
     from triangle_counting_hll_amd import _lib, gpu_graph, gpu_hybrid
-    from triangle_counting_hll_amd.graphgen import config_pairs
 
     if not torch.cuda.is_available():
         print("no ROCm GPU visible", file=sys.stderr)
@@ -320,14 +403,7 @@ def main():
         _log("calibration copy: 1 GiB read + 1 GiB written")
 
     t0 = time.perf_counter()
-    if args.workload.startswith("rmat"):
-        # device twin of the seeded host generator (same pairs, bit for bit)
-        from triangle_counting_hll_amd import batch
-
-        scale, ef, seed = RMAT_WORKLOADS[args.workload]
-        pairs_d = batch.rmat_gpu(scale - args.scale_down, ef, seed)
-    else:
-        pairs_d = torch.from_numpy(config_pairs(args.workload, args.scale_down)).cuda()
+    pairs_d = workload_pairs(args.workload, args.scale_down)
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t0
     _log(f"generated {pairs_d.shape[0]} raw pairs in {t_gen:.1f}s")
@@ -386,6 +462,7 @@ def main():
     extras = {}
     if not args.no_extras and not args.scale_down:
         for name, fn in (("hybrid_fixed_threshold", lambda: run_hybrid_extra(csr, gpu_count)),
+                         ("ingest_text", lambda: run_ingest_extra(args.workload, csr)),
                          ("batch4096", lambda: run_batch_extra(args)),
                          ("learned_router", lambda: run_router_extra())):
             try:
