@@ -371,6 +371,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=256)
     ap.add_argument("--waves-per-cu", type=int, default=16)
     ap.add_argument("--unroll", type=int, default=8)
+    ap.add_argument("--extras", default="all",
+                    help="comma-separated extra measurements to run (hybrid_fixed_threshold, "
+                         "ingest_text, batch4096, learned_router) or 'all'")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the hybrid and many-graph measurements added to the line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -465,6 +468,8 @@ def main():
                          ("ingest_text", lambda: run_ingest_extra(args.workload, csr)),
                          ("batch4096", lambda: run_batch_extra(args)),
                          ("learned_router", lambda: run_router_extra())):
+            if args.extras != "all" and name not in args.extras.split(","):
+                continue
             try:
                 t0 = time.perf_counter()
                 extras[name] = fn()
