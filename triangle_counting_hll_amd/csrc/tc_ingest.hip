@@ -51,27 +51,38 @@ __device__ __forceinline__ bool is_space(int c) {
   return c == ' ' || c == '\t' || c == 0x0b || c == 0x0c || (c >= 0x1c && c <= 0x1f);
 }
 
-// Byte p of the text: from the LDS stage when staged, else from global memory;
-// -1 past the end.
-struct TextBytes {
-  const uint8_t *stage;  // stage[i] = text[t0 - kPre + i]
-  const uint8_t *text;
-  int64_t n, t0;
-  __device__ __forceinline__ int at(int64_t p) const {
+// Byte sources for the line parser.  A source returns the byte at text
+// offset p, -1 past the end of the text, or -2 where it cannot see (the LDS
+// source past its staged bytes); a line that reaches -2 is parsed again from
+// global memory.
+struct LdsBytes {
+  const uint8_t *stage;  // __shared__ stage; stage[i] = text[s0 + i]
+  int64_t s0, n;
+  __device__ __forceinline__ int operator()(int64_t p) const {
     if (p >= n) return -1;
-    const int64_t i = p - t0 + kPre;
-    return (i >= 0 && i < kStage) ? (int)stage[i] : (int)text[p];
+    const int64_t i = p - s0;
+    return i < kStage ? (int)stage[i] : -2;
+  }
+};
+struct GlobalBytes {
+  const uint8_t *text;
+  int64_t n;
+  __device__ __forceinline__ int operator()(int64_t p) const {
+    return p < n ? (int)text[p] : -1;
   }
 };
 
+enum : int { kLineOffStage = 7 };
+
 // One field starting at p (a byte that is neither space nor break); p ends on
 // the byte after the field.
-__device__ __forceinline__ int parse_field(const TextBytes &T, int64_t &p, int32_t &out) {
-  int c = T.at(p);
+template <class Src>
+__device__ __forceinline__ int parse_field(const Src &at, int64_t &p, int32_t &out) {
+  int c = at(p);
   bool neg = false;
   if (c == '+' || c == '-') {
     neg = (c == '-');
-    c = T.at(++p);
+    c = at(++p);
   }
   int64_t v = 0;
   bool digits = false, ok = true, big = false, under = false;
@@ -88,8 +99,9 @@ __device__ __forceinline__ int parse_field(const TextBytes &T, int64_t &p, int32
     } else {
       ok = false;
     }
-    c = T.at(++p);
+    c = at(++p);
   }
+  if (c == -2) return kLineOffStage;
   if (!digits || under || !ok) return kLineBadInt;
   if (neg && (v != 0 || big)) return kLineNegative;
   if (big) return kLineRange;
@@ -97,23 +109,41 @@ __device__ __forceinline__ int parse_field(const TextBytes &T, int64_t &p, int32
   return kLinePair;
 }
 
-__device__ __forceinline__ int parse_line(const TextBytes &T, int64_t p, const CommentSet &cs,
+template <class Src>
+__device__ __forceinline__ int parse_line(const Src &at, int64_t p, const CommentSet &cs,
                                           int32_t &u, int32_t &v) {
-  int c = T.at(p);
-  while (c >= 0 && is_space(c)) c = T.at(++p);
+  int c = at(p);
+  while (c >= 0 && is_space(c)) c = at(++p);
+  if (c == -2) return kLineOffStage;
   if (c < 0 || is_break(c)) return kLineSkip;
   if ((cs.bits[c >> 5] >> (c & 31)) & 1u) return kLineSkip;
-  const int s1 = parse_field(T, p, u);
-  c = T.at(p);
-  while (c >= 0 && is_space(c)) c = T.at(++p);
+  const int s1 = parse_field(at, p, u);
+  if (s1 == kLineOffStage) return s1;
+  c = at(p);
+  while (c >= 0 && is_space(c)) c = at(++p);
+  if (c == -2) return kLineOffStage;
   if (c < 0 || is_break(c)) return kLineSkip;  // one field only
-  const int s2 = parse_field(T, p, v);
+  const int s2 = parse_field(at, p, v);
+  if (s2 == kLineOffStage) return s2;
   if (s1 != kLinePair) return s1;  // int(parts[0]) is evaluated first
   if (s2 != kLinePair) return s2;
   return u == v ? kLineSkip : kLinePair;
 }
 
+// Lines longer than the staged halo (rare): parsed from global memory.
+struct LineResult {
+  int st;
+  int32_t u, v;
+};
+__device__ __noinline__ LineResult parse_line_global(const uint8_t *text, int64_t n, int64_t p,
+                                                     CommentSet cs) {
+  LineResult r{0, 0, 0};
+  r.st = parse_line(GlobalBytes{text, n}, p, cs, r.u, r.v);
+  return r;
+}
+
 // Exclusive scan of one int per thread over the workgroup; *total gets the sum.
+// Ends with a barrier, so wave_tot may be reused right after.
 __device__ __forceinline__ int block_exclusive_scan(int x, int *wave_tot, int *total) {
   const int w = threadIdx.x / kWave, lane = lane_id();
   const int inc = wave_inclusive_scan(x);
@@ -127,15 +157,21 @@ __device__ __forceinline__ int block_exclusive_scan(int x, int *wave_tot, int *t
     sum += t;
   }
   *total = sum;
+  __syncthreads();
   return before + inc - x;
 }
 
+// One workgroup per 4 KiB tile.  The tile's line starts are found bytewise
+// (16 bytes per thread) and compacted into an LDS list; then thread j parses
+// lines j, j + 256, ... so that every lane has a line of its own.  Pairs are
+// written in line order: each round of 256 lines is ranked by a block scan.
 template <bool kWrite>
 __global__ void __launch_bounds__(kBlock)
 k_parse_tiles(const uint8_t *__restrict__ text, int64_t n, int aligned, CommentSet cs,
               int32_t *__restrict__ tile_count, const int64_t *__restrict__ tile_off,
               int2 *__restrict__ pairs, unsigned long long *__restrict__ bad) {
   __shared__ __align__(16) uint8_t stage[kStage];
+  __shared__ uint16_t starts[kTileBytes / 2];  // at most one line start per two bytes
   __shared__ int wave_tot[kBlock / kWave];
   const int64_t t0 = (int64_t)blockIdx.x * kTileBytes;
   const int64_t s0 = t0 - kPre;  // text offset of stage[0]
@@ -161,38 +197,59 @@ k_parse_tiles(const uint8_t *__restrict__ text, int64_t n, int aligned, CommentS
   }
   __syncthreads();
 
-  const TextBytes T{stage, text, n, t0};
-  const int64_t seg0 = t0 + (int64_t)threadIdx.x * kSegBytes;
-  // lines starting in this thread's bytes: a non-break byte after a break
-  int cnt = 0;
+  // line starts in this thread's 16 bytes: a non-break byte after a break
+  const int b0 = kPre + (int)threadIdx.x * kSegBytes;
+  uint32_t mask = 0;
+  int prev = stage[b0 - 1];
+#pragma unroll
   for (int q = 0; q < kSegBytes; ++q) {
-    const int64_t p = seg0 + q;
-    if (p >= n) break;
-    const int i = (int)(p - s0);
-    if (is_break(stage[i]) || !is_break(stage[i - 1])) continue;
-    int32_t u, v;
-    const int st = parse_line(T, p, cs, u, v);
-    if (st == kLinePair) {
-      ++cnt;
-    } else if (!kWrite && st != kLineSkip) {
-      atomicMin(bad, ((unsigned long long)p << 3) | (unsigned long long)st);
+    const int c = stage[b0 + q];
+    if (!is_break(c) && is_break(prev) && t0 + (int64_t)(b0 - kPre + q) < n) mask |= 1u << q;
+    prev = c;
+  }
+  int n_lines;
+  int at = block_exclusive_scan(__popc(mask), wave_tot, &n_lines);
+  while (mask) {
+    const int q = __ffs(mask) - 1;
+    mask &= mask - 1;
+    starts[at++] = (uint16_t)(b0 - kPre + q);
+  }
+  __syncthreads();
+
+  const LdsBytes lds{stage, s0, n};
+  int64_t out = kWrite ? tile_off[blockIdx.x] : 0;
+  int cnt = 0;
+  for (int r0 = 0; r0 < n_lines; r0 += kBlock) {  // uniform trip count
+    const int k = r0 + (int)threadIdx.x;
+    int st = kLineSkip;
+    int32_t u = 0, v = 0;
+    int64_t p = 0;
+    if (k < n_lines) {
+      p = t0 + starts[k];
+      st = parse_line(lds, p, cs, u, v);
+      if (st == kLineOffStage) {
+        const LineResult r = parse_line_global(text, n, p, cs);
+        st = r.st;
+        u = r.u;
+        v = r.v;
+      }
+    }
+    const int is_pair = st == kLinePair;
+    if constexpr (kWrite) {
+      int round;
+      const int rank = block_exclusive_scan(is_pair, wave_tot, &round);
+      if (is_pair) pairs[out + rank] = make_int2(u, v);
+      out += round;
+    } else {
+      cnt += is_pair;
+      if (st != kLinePair && st != kLineSkip)
+        atomicMin(bad, ((unsigned long long)p << 3) | (unsigned long long)st);
     }
   }
-  int total;
-  const int before = block_exclusive_scan(cnt, wave_tot, &total);
   if constexpr (!kWrite) {
+    int total;
+    (void)block_exclusive_scan(cnt, wave_tot, &total);
     if (threadIdx.x == 0) tile_count[blockIdx.x] = total;
-  } else {
-    if (cnt == 0) return;
-    int64_t o = tile_off[blockIdx.x] + before;
-    for (int q = 0; q < kSegBytes; ++q) {
-      const int64_t p = seg0 + q;
-      if (p >= n) break;
-      const int i = (int)(p - s0);
-      if (is_break(stage[i]) || !is_break(stage[i - 1])) continue;
-      int32_t u, v;
-      if (parse_line(T, p, cs, u, v) == kLinePair) pairs[o++] = make_int2(u, v);
-    }
   }
 }
 
