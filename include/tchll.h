@@ -73,7 +73,7 @@ int tc_csr_build_begin(const int32_t *pairs, int64_t m, void *stream,
 /* Edge-list text -> raw pairs (int32 [m][2], file order, self-loops dropped),
  * the input of tc_csr_build_begin.  `text` (device, n_bytes) holds whole
  * lines of a SNAP-style edge list; `comment_chars` (host, NUL-terminated,
- * NULL = "#%") lists the bytes that start a comment line.  Lines, fields and
+ * at most 8 bytes, NULL = "#%") lists the bytes that start a comment line.  Lines, fields and
  * skips follow experiment/exp_gpu.py:131-143 (_iter_edges_raw): breaks at
  * '\n' / '\r', leading whitespace skipped, comment and one-field lines
  * skipped, the first two whitespace-separated fields read as Python int(),

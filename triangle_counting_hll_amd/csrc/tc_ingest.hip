@@ -41,8 +41,19 @@ enum : int {
   kLineRange = 4,     // an id above INT32_MAX
 };
 
+// Up to 8 comment bytes packed little-endian; unused bytes are 0 (a C string
+// cannot hold a NUL, so 0 never matches a real byte of the set).
 struct CommentSet {
-  uint32_t bits[8];  // 256-bit membership set over byte values
+  uint64_t bytes;
+  __device__ __forceinline__ bool has(int c) const {
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int b = (int)((bytes >> (8 * k)) & 0xffu);
+      hit |= (b != 0) & (b == c);
+    }
+    return hit;
+  }
 };
 
 __device__ __forceinline__ bool is_break(int c) { return c == '\n' || c == '\r'; }
@@ -51,16 +62,15 @@ __device__ __forceinline__ bool is_space(int c) {
   return c == ' ' || c == '\t' || c == 0x0b || c == 0x0c || (c >= 0x1c && c <= 0x1f);
 }
 
-// Byte sources for the line parser.  A source returns the byte at text
-// offset p, -1 past the end of the text, or -2 where it cannot see (the LDS
-// source past its staged bytes); a line that reaches -2 is parsed again from
-// global memory.
+// Byte sources for the line parser, indexed by an Index (int for the LDS
+// stage, int64 text offset for global memory).  A source returns the byte,
+// -1 past the end of the text, or -2 where it cannot see (the LDS source past
+// its staged bytes); a line that reaches -2 is parsed again from global
+// memory.  Staged bytes past the end of the text read as '\n', which ends a
+// line exactly as the end of the text does.
 struct LdsBytes {
-  const uint8_t *stage;  // __shared__ stage; stage[i] = text[s0 + i]
-  int64_t s0, n;
-  __device__ __forceinline__ int operator()(int64_t p) const {
-    if (p >= n) return -1;
-    const int64_t i = p - s0;
+  const uint8_t *stage;  // the __shared__ stage
+  __device__ __forceinline__ int operator()(int i) const {
     return i < kStage ? (int)stage[i] : -2;
   }
 };
@@ -76,8 +86,8 @@ enum : int { kLineOffStage = 7 };
 
 // One field starting at p (a byte that is neither space nor break); p ends on
 // the byte after the field.
-template <class Src>
-__device__ __forceinline__ int parse_field(const Src &at, int64_t &p, int32_t &out) {
+template <class Src, class Index>
+__device__ __forceinline__ int parse_field(const Src &at, Index &p, int32_t &out) {
   int c = at(p);
   bool neg = false;
   if (c == '+' || c == '-') {
@@ -109,14 +119,14 @@ __device__ __forceinline__ int parse_field(const Src &at, int64_t &p, int32_t &o
   return kLinePair;
 }
 
-template <class Src>
-__device__ __forceinline__ int parse_line(const Src &at, int64_t p, const CommentSet &cs,
+template <class Src, class Index>
+__device__ __forceinline__ int parse_line(const Src &at, Index p, const CommentSet &cs,
                                           int32_t &u, int32_t &v) {
   int c = at(p);
   while (c >= 0 && is_space(c)) c = at(++p);
   if (c == -2) return kLineOffStage;
   if (c < 0 || is_break(c)) return kLineSkip;
-  if ((cs.bits[c >> 5] >> (c & 31)) & 1u) return kLineSkip;
+  if (cs.has(c)) return kLineSkip;
   const int s1 = parse_field(at, p, u);
   if (s1 == kLineOffStage) return s1;
   c = at(p);
@@ -216,7 +226,7 @@ k_parse_tiles(const uint8_t *__restrict__ text, int64_t n, int aligned, CommentS
   }
   __syncthreads();
 
-  const LdsBytes lds{stage, s0, n};
+  const LdsBytes lds{stage};
   int64_t out = kWrite ? tile_off[blockIdx.x] : 0;
   int cnt = 0;
   for (int r0 = 0; r0 < n_lines; r0 += kBlock) {  // uniform trip count
@@ -225,8 +235,9 @@ k_parse_tiles(const uint8_t *__restrict__ text, int64_t n, int aligned, CommentS
     int32_t u = 0, v = 0;
     int64_t p = 0;
     if (k < n_lines) {
-      p = t0 + starts[k];
-      st = parse_line(lds, p, cs, u, v);
+      const int i = kPre + starts[k];
+      p = t0 + (i - kPre);
+      st = parse_line(lds, i, cs, u, v);
       if (st == kLineOffStage) {
         const LineResult r = parse_line_global(text, n, p, cs);
         st = r.st;
@@ -273,10 +284,10 @@ extern "C" int tc_parse_edge_list(const uint8_t *text, int64_t n_bytes, const ch
   if (n_bytes == 0) return TC_OK;
   hipStream_t s = (hipStream_t)stream;
 
-  CommentSet cs{};
+  CommentSet cs{0};
   const char *cc = comment_chars ? comment_chars : "#%";
-  for (const unsigned char *c = (const unsigned char *)cc; *c; ++c)
-    cs.bits[*c >> 5] |= 1u << (*c & 31);
+  TC_CHECK(strlen(cc) <= 8, TC_ERR_INVALID, "at most 8 comment characters");
+  for (int k = 0; cc[k]; ++k) cs.bytes |= (uint64_t)(unsigned char)cc[k] << (8 * k);
   const int aligned = ((uintptr_t)text & 15) == 0;
 
   Scratch cnt, off, bad;
