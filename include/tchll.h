@@ -294,6 +294,52 @@ int tc_generate_pairs(const uint8_t *kind, const int32_t *param, const int64_t *
                       uint32_t t_ab, uint32_t t_abc, int32_t *pairs, int64_t n_pairs,
                       void *stream);
 
+/* ---- DOULION edge sampling ---------------------------------------------- */
+
+/* keep[i] = (numpy PCG64 draw first + i, as random()) < q for i in [0, n):
+ * the stream of numpy.random.default_rng(seed).random(...), whose initial
+ * 128-bit state and increment the caller reads from numpy
+ * (bit_generator.state).  Draw i uses the generator state after first + i + 1
+ * steps of s <- s * 0x2360ED051FC65DA44385DF649FCCF645 + inc, output XSL-RR,
+ * double (x >> 11) * 2^-53.  Replaces `rng.random(n_fwd) < q` of
+ * experiment/exp_gpu.py:414 (_doulion_forward) and src/doulion.py:38. */
+int tc_pcg64_below(uint64_t state_hi, uint64_t state_lo, uint64_t inc_hi, uint64_t inc_lo,
+                   uint64_t first, int64_t n, double q, uint8_t *keep, void *stream);
+
+/* Edge-induced subgraph of a forward CSR: keeps edge e (CSR order) iff
+ * keep[e] != 0, rows stay sorted.  out_row_ptr [n+1] int64; out_col holds up
+ * to n_edges ids; *n_kept (host) = kept edges.  Synchronises `stream`.
+ * Replaces the sample-to-forward-adjacency step of experiment/exp_gpu.py:415-433
+ * (the sample keeps the original orientation). */
+int tc_edge_subgraph(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                     int64_t n_nodes, int64_t n_edges, const uint8_t *keep,
+                     int64_t *out_row_ptr, int32_t *out_col, int64_t *n_kept, void *stream);
+
+/* ---- degree statistics ----------------------------------------------------- */
+
+/* degree[u] += 1 and degree[v] += 1 for every edge (u, v) = (src[e * stride],
+ * dst[e * stride]), e < n_edges (stride 1: two arrays; stride 2 with
+ * dst = src + 1: interleaved pairs).  Ids must lie in [0, n_slots), else
+ * TC_ERR_INVALID (checked; synchronises `stream`).
+ * Replaces degree_count_kernel, experiment/gpu_graph_stats.py:89-97. */
+int tc_degree_count(const int32_t *src, const int32_t *dst, int64_t stride, int64_t n_edges,
+                    int32_t *degree, int64_t n_slots, void *stream);
+
+/* Over the positive entries of degree[n_slots]: sums (device int64[3]) =
+ * {count, sum, sum of squares}, minmax (device int32[2]) = {max, min}; both
+ * overwritten ({0, 0, 0} and {0, INT32_MAX} when no entry is positive).
+ * Replaces degree_stats_kernel, experiment/gpu_graph_stats.py:100-117. */
+int tc_degree_summary(const int32_t *degree, int64_t n_slots, int64_t *sums, int32_t *minmax,
+                      void *stream);
+
+/* Histograms of the positive entries of degree[n_slots] (device outputs,
+ * overwritten): hist[d] (n_hist = max degree + 1 entries) and
+ * log_hist[min(floor(log2 d), n_log_buckets - 1)] (n_log_buckets <= 64).
+ * A degree >= n_hist is TC_ERR_INVALID (checked; synchronises `stream`).
+ * Replaces degree_hist_kernel, experiment/gpu_graph_stats.py:120-140. */
+int tc_degree_histogram(const int32_t *degree, int64_t n_slots, int64_t *hist, int64_t n_hist,
+                        int64_t *log_hist, int n_log_buckets, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

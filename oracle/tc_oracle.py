@@ -435,6 +435,52 @@ def hybrid_count(row_ptr, col_idx, edge_u, edge_v, regs: np.ndarray, has: np.nda
     }
 
 
+# ---------------------------------------------------------------------------
+# DOULION  (experiment/exp_gpu.py:396-450; src/doulion.py:11-84)
+# ---------------------------------------------------------------------------
+
+PCG64_MULT = 0x2360ED051FC65DA44385DF649FCCF645
+
+
+def pcg64_doubles(state: int, inc: int, n: int) -> np.ndarray:
+    """n doubles of numpy's PCG64 from (state, inc), restated with Python ints:
+    s <- s * PCG64_MULT + inc (mod 2^128), x = rotr64(hi ^ lo, hi >> 58),
+    u = (x >> 11) * 2^-53 (O'Neill's PCG XSL-RR 128/64; numpy's next_double)."""
+    m128, m64 = (1 << 128) - 1, (1 << 64) - 1
+    out = np.empty(n, dtype=np.float64)
+    for i in range(n):
+        state = (state * PCG64_MULT + inc) & m128
+        hi, lo = state >> 64, state & m64
+        x, r = hi ^ lo, hi >> 58
+        x = ((x >> r) | (x << ((64 - r) & 63))) & m64
+        out[i] = (x >> 11) * (1.0 / 9007199254740992.0)
+    return out
+
+
+def doulion_forward(row_ptr, col_idx, q: float, n_seeds: int = 10, base_seed: int = 42):
+    """``_doulion_forward`` (experiment/exp_gpu.py:396-450) on a forward CSR.
+
+    Edge e (CSR u-major order, the reference's edge_u/edge_v order) is kept
+    iff ``np.random.default_rng(seed).random(E)[e] < q``; the sample keeps the
+    orientation and is counted exactly; estimate = count / q^3.  Returns
+    (estimates, sampled_counts).
+    """
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    col_idx = np.asarray(col_idx)
+    E = len(col_idx)
+    eu = np.repeat(np.arange(len(row_ptr) - 1, dtype=np.int64), np.diff(row_ptr))
+    estimates, counts = [], []
+    for seed in range(base_seed, base_seed + n_seeds):
+        mask = np.random.default_rng(seed).random(E) < q
+        su, sv = eu[mask], col_idx[mask].astype(np.int64)
+        rp = np.zeros(len(row_ptr), dtype=np.int64)
+        np.cumsum(np.bincount(su, minlength=len(row_ptr) - 1), out=rp[1:])
+        c = exact_count_csr(rp, sv) if len(sv) else 0
+        counts.append(int(c))
+        estimates.append(c / (q ** 3))
+    return estimates, counts
+
+
 def relative_error(estimate: float, truth: int) -> float:
     """``|est - truth| / truth`` with the zero conventions of src/utils.py:36-43."""
     if truth == 0:

@@ -63,6 +63,12 @@ _SIGNATURES = {
     "tc_batch_graph_create": (I32, [P, I32, P, I64, P, P, I64, P, I32, P, P, P, P, P]),
     "tc_batch_graph_launch": (I32, [P, P]),
     "tc_batch_graph_free": (None, [P]),
+    "tc_pcg64_below": (I32, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                             ctypes.c_uint64, I64, ctypes.c_double, P, P]),
+    "tc_edge_subgraph": (I32, [P, I32, P, I64, I64, P, P, P, P, P]),
+    "tc_degree_count": (I32, [P, P, I64, I64, P, I64, P]),
+    "tc_degree_summary": (I32, [P, I64, P, P, P]),
+    "tc_degree_histogram": (I32, [P, I64, P, I64, P, I32, P]),
     "tc_generate_pairs": (I32, [P, P, P, P, P, I32, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_uint32, P, I64, P]),
 }
