@@ -161,6 +161,19 @@ int tc_edge_supports(const void *row_ptr, int row_ptr_64, const int32_t *col_idx
  * src/gpu_hll.py:12-17 (= src/hll.py:13-19) for testing. */
 int tc_hash64(const uint64_t *x, uint64_t *out, int64_t n, void *stream);
 
+/* Hash and estimator variants of the HLL entry points (hash_mode):
+ *   TC_HLL_GPU      h = splitmix64 finaliser(id), index = top p bits, rank
+ *                   from the low 64 - p bits; estimator of src/hll.py:37-54
+ *                   (LinearCounting, then the 2^64 large-range correction).
+ *                   The reference's GPU and src/ path (src/hll.py:13-19, 81-91).
+ *   TC_HLL_CPU_EXP  h = splitmix64 finaliser(id + seed + 0x9E3779B97F4A7C15),
+ *                   index = low p bits, rank from the high 64 - p bits;
+ *                   estimator returning LinearCounting or the 2^32 large-range
+ *                   correction of the raw value (cpu_exp/triangle_hll_experiment.py:
+ *                   347-398, 615-635).  `seed` is used by this mode only. */
+#define TC_HLL_GPU 0
+#define TC_HLL_CPU_EXP 1
+
 /* Builds HLL registers of N+(u) for every u with deg+(u) >= theta and sets
  * has_sketch[u] (uint8, every u).  Row of u: regs + slot[u] * 2^p when `slot`
  * (int32 [n], -1 = no row) is given, else regs + u * 2^p (dense layout of
@@ -171,7 +184,7 @@ int tc_hash64(const uint64_t *x, uint64_t *out, int64_t n, void *stream);
 int tc_hll_build(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
                  int64_t n_nodes, int64_t theta, int p, uint8_t *regs,
                  const int32_t *slot, uint8_t *has_sketch, int accumulate,
-                 void *stream);
+                 int hash_mode, uint64_t seed, void *stream);
 
 /* ---- (d) union / intersection estimation (FP64) -------------------------- */
 
@@ -181,7 +194,7 @@ int tc_hll_build(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
  * the FP64 estimator of src/hll.py:37-54, 112-120. */
 int tc_union_estimate(const uint8_t *regs, const int32_t *slot, int p,
                       const int32_t *us, const int32_t *vs, int64_t k,
-                      double *out, void *stream);
+                      double *out, int hash_mode, void *stream);
 
 /* ---- (e) hybrid router and count ----------------------------------------- */
 
@@ -209,7 +222,7 @@ int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
                     int64_t n_edges, const int32_t *degrees, const uint8_t *regs,
                     const int32_t *slot, const uint8_t *has_sketch, int64_t gamma, int p,
                     const uint8_t *route, int part, double *counts, int64_t *stats,
-                    int64_t *fallback, double *per_edge, void *stream);
+                    int64_t *fallback, double *per_edge, int hash_mode, void *stream);
 
 /* ---- learned exact-vs-sketch router -------------------------------------- */
 /* No reference counterpart: the reference routes by a fixed degree threshold

@@ -207,6 +207,125 @@ def to_csr_numpy(forward_adj: Dict[int, np.ndarray]):
 
 
 # ---------------------------------------------------------------------------
+# The CPU experiment's HLL variant  (cpu_exp/triangle_hll_experiment.py:45-48,
+# 347-398, 615-709)
+# ---------------------------------------------------------------------------
+
+SM64_GOLDEN = 0x9E3779B97F4A7C15
+
+
+def splitmix64_seeded_int(x: int) -> int:
+    """splitmix64 with the golden increment (cpu_exp:369-373) on a Python int."""
+    return hash64_int((x + SM64_GOLDEN) & MASK64)
+
+
+def rho_high_int(h: int, p: int) -> int:
+    """Rank from the high 64 - p bits (cpu_exp:376-381)."""
+    bits = 64 - p
+    w = h >> p
+    return bits + 1 if w == 0 else bits - w.bit_length() + 1
+
+
+def build_registers_cpu_exp_py(row_ptr, col_idx, p: int, seed: int) -> np.ndarray:
+    """``build_hll_registers_py`` (cpu_exp:384-398): uint8 [n, 2^p], one Python
+    step per element (small graphs only)."""
+    n = len(row_ptr) - 1
+    m = 1 << p
+    regs = np.zeros((n, m), dtype=np.uint8)
+    seed &= MASK64
+    for u in range(n):
+        for k in range(int(row_ptr[u]), int(row_ptr[u + 1])):
+            h = splitmix64_seeded_int((int(col_idx[k]) + seed) & MASK64)
+            j = h & (m - 1)
+            r = rho_high_int(h, p)
+            if r > regs[u, j]:
+                regs[u, j] = r
+    return regs
+
+
+def build_registers_cpu_exp(row_ptr, col_idx, p: int, seed: int) -> np.ndarray:
+    """Vectorised ``build_registers_cpu_exp_py``: uint8 [n, 2^p]."""
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    n = len(row_ptr) - 1
+    m = 1 << p
+    x = np.asarray(col_idx).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        h = hash64(x + np.uint64(seed & MASK64) + np.uint64(SM64_GOLDEN))
+    j = (h & np.uint64(m - 1)).astype(np.int64)
+    w = h >> np.uint64(p)
+    bl = np.zeros(w.shape, dtype=np.int64)
+    for b in range(64 - p):
+        bl += (w >= np.uint64(1 << b)).astype(np.int64)
+    rho = (64 - p) - bl + 1  # w == 0 gives bl = 0 and rho = 65 - p
+    u = np.repeat(np.arange(n, dtype=np.int64), np.diff(row_ptr))
+    regs = np.zeros(n * m, dtype=np.uint8)
+    np.maximum.at(regs, u * m + j, rho.astype(np.uint8))
+    return regs.reshape(n, m)
+
+
+def estimate_cpu_exp(registers: np.ndarray) -> float:
+    """``estimate_hll_cardinality`` (cpu_exp:356-365)."""
+    r = np.asarray(registers)
+    m = int(r.size)
+    inv_sum = float(np.sum(np.exp2(-r.astype(np.float64))))
+    raw = alpha(m) * m * m / inv_sum
+    zeros = int(np.count_nonzero(r == 0))
+    if raw <= 2.5 * m and zeros > 0:
+        return m * math.log(m / zeros)
+    if raw > 4294967296.0 / 30.0:
+        return -4294967296.0 * math.log(1.0 - raw / 4294967296.0)
+    return raw
+
+
+def union_estimate_cpu_exp(ra: np.ndarray, rb: np.ndarray) -> float:
+    """``hll_union_estimate_numba`` (cpu_exp:615-635): register-wise max, then
+    2^-r summed left to right."""
+    r = np.maximum(np.asarray(ra), np.asarray(rb)).astype(np.int64)
+    m = int(r.size)
+    inv_sum = float(np.add.accumulate(np.ldexp(1.0, -r))[-1])
+    zeros = int(np.count_nonzero(r == 0))
+    raw = alpha(m) * m * m / inv_sum
+    if raw <= 2.5 * m and zeros > 0:
+        return m * math.log(m / zeros)
+    if raw > 4294967296.0 / 30.0:
+        return -4294967296.0 * math.log(1.0 - raw / 4294967296.0)
+    return raw
+
+
+def hll_support_cpu_exp(regs, out_degree, u: int, v: int) -> Tuple[float, bool]:
+    """(clamped support, clamped?) of one edge (cpu_exp:637-660)."""
+    est = float(out_degree[u] + out_degree[v]) - union_estimate_cpu_exp(regs[u], regs[v])
+    upper = float(min(out_degree[u], out_degree[v]))
+    if est < 0.0:
+        return 0.0, True
+    if est > upper:
+        return upper, True
+    return est, False
+
+
+def pure_hll_count_cpu_exp(edge_u, edge_v, out_degree, regs) -> Tuple[float, int]:
+    """``pure_hll_count`` (cpu_exp:662-684): (total, clamp_count)."""
+    total, clamps = [], 0
+    for u, v in zip(np.asarray(edge_u).tolist(), np.asarray(edge_v).tolist()):
+        s, c = hll_support_cpu_exp(regs, out_degree, u, v)
+        total.append(s)
+        clamps += c
+    return math.fsum(total), clamps
+
+
+def hybrid_count_cpu_exp(row_ptr, col_idx, edge_u, edge_v, out_degree, regs, threshold: int):
+    """``hybrid_count`` (cpu_exp:686-709): exact when min(out deg) <= threshold.
+    Returns (total, exact_edges, hll_edges, exact_sum, hll_sum, clamp_count)."""
+    eu = np.asarray(edge_u)
+    ev = np.asarray(edge_v)
+    cost = np.minimum(out_degree[eu], out_degree[ev])
+    ex = cost <= threshold
+    exact_sum = float(edge_supports(row_ptr, col_idx, eu[ex], ev[ex]).sum())
+    hll_sum, clamps = pure_hll_count_cpu_exp(eu[~ex], ev[~ex], out_degree, regs)
+    return (exact_sum + hll_sum, int(ex.sum()), int((~ex).sum()), exact_sum, hll_sum, clamps)
+
+
+# ---------------------------------------------------------------------------
 # Edge-list text  (experiment/exp_gpu.py:93, 131-143, 291-313)
 # ---------------------------------------------------------------------------
 

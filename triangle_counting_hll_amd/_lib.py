@@ -47,10 +47,10 @@ _SIGNATURES = {
     "tc_exact_count": (I32, [P, I32, P, I64, P, P, I64, P, P, P]),
     "tc_edge_supports": (I32, [P, I32, P, P, P, I64, P, P]),
     "tc_hash64": (I32, [P, P, I64, P]),
-    "tc_hll_build": (I32, [P, I32, P, I64, I64, I32, P, P, P, I32, P]),
-    "tc_union_estimate": (I32, [P, P, I32, P, P, I64, P, P]),
+    "tc_hll_build": (I32, [P, I32, P, I64, I64, I32, P, P, P, I32, I32, ctypes.c_uint64, P]),
+    "tc_union_estimate": (I32, [P, P, I32, P, P, I64, P, I32, P]),
     "tc_hybrid_count": (I32, [P, I32, P, I64, P, P, I64, P, P, P, P, I64, I32, P, I32, P, P, P, P,
-                              P]),
+                              I32, P]),
     "tc_router_features": (I32, [P, P, P, P, I64, P, P, I32, I32, P, P, P]),
     "tc_router_n_params": (I32, []),
     "tc_mlp_forward": (I32, [P, P, P, I64, P, P]),
@@ -85,6 +85,15 @@ class ExactParams(ctypes.Structure):
 
 
 PLANS = {"sorted": 0, "atomic": 1}
+
+# hash / estimator variants of the HLL entry points (TC_HLL_* in include/tchll.h)
+HASH_MODES = {"gpu": 0, "cpu_exp": 1}
+
+
+def hash_mode(name) -> int:
+    if name not in HASH_MODES:
+        raise ValueError(f"hash_mode must be one of {sorted(HASH_MODES)}, got {name!r}")
+    return HASH_MODES[name]
 
 
 _lock = threading.Lock()

@@ -6,6 +6,12 @@
 //   reg[j] = max(reg[j], rho)                       (src/hll.py:81-91)
 // The estimator is the FP64 one of src/hll.py:37-54; its sum Z = sum_j 2^-r_j
 // is accumulated in FP64 in a fixed order (deterministic, ~1e-14 relative).
+//
+// TC_HLL_CPU_EXP selects the CPU experiment's variant instead
+// (cpu_exp/triangle_hll_experiment.py:347-398, 615-635): h = seeded splitmix64
+// (id + seed + 0x9E3779B97F4A7C15, then the finaliser); j = low p bits;
+// rho from the high 64 - p bits; estimator with early returns and the 2^32
+// large-range constant.
 #include <algorithm>
 
 #include "tc_common.hpp"
@@ -22,6 +28,21 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 __device__ __forceinline__ uint32_t hll_rank(uint64_t h, int p) {
   const uint64_t w = h & ((~0ull) >> p);
   return w == 0 ? (uint32_t)(65 - p) : (uint32_t)(__clzll((long long)w) - p + 1);
+}
+
+// Register index and rank of one id under either hash mode.
+__device__ __forceinline__ void hll_slot(int64_t id, int p, int mode, uint64_t seed, uint32_t &j,
+                                         uint32_t &rho) {
+  if (mode == TC_HLL_CPU_EXP) {
+    const uint64_t h = mix64((uint64_t)id + seed + 0x9E3779B97F4A7C15ull);
+    j = (uint32_t)(h & ((1ull << p) - 1));
+    const uint64_t w = h >> p;  // the high 64 - p bits
+    rho = w == 0 ? (uint32_t)(65 - p) : (uint32_t)(__clzll((long long)w) - p + 1);
+  } else {
+    const uint64_t h = mix64((uint64_t)id);
+    j = (uint32_t)(h >> (64 - p));
+    rho = hll_rank(h, p);
+  }
 }
 
 __global__ void k_hash64(const uint64_t *__restrict__ x, uint64_t *__restrict__ out, int64_t n) {
@@ -43,12 +64,11 @@ template <class I>
 __global__ void __launch_bounds__(kBlock)
 k_hll_build(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict__ list,
             const int64_t *__restrict__ n_list_p, int p, uint8_t *__restrict__ regs,
-            const int32_t *__restrict__ slot, int accumulate) {
+            const int32_t *__restrict__ slot, int accumulate, int mode, uint64_t seed) {
   extern __shared__ uint32_t t32[];
   const int m = 1 << p;
   const int words = m >> 2;
   const int64_t n_list = *n_list_p;
-  const int shift = 64 - p;
   for (int64_t li = blockIdx.x; li < n_list; li += gridDim.x) {
     const int32_t u = list[li];
     const int64_t r = slot ? (int64_t)slot[u] : (int64_t)u;
@@ -58,9 +78,8 @@ k_hll_build(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restric
     __syncthreads();
     const int64_t b = R.begin(u), e = R.end(u);
     for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
-      const uint64_t h = mix64((uint64_t)(int64_t)col[k]);
-      const uint32_t j = (uint32_t)(h >> shift);
-      const uint32_t rho = hll_rank(h, p);
+      uint32_t j, rho;
+      hll_slot(col[k], p, mode, seed, j, rho);
       uint32_t *wp = &t32[j >> 2];
       const int sh = (j & 3) * 8;
       uint32_t old = *wp;
@@ -86,9 +105,18 @@ __device__ __forceinline__ double hll_alpha(int m) {
 
 // Estimate from the merged register statistics (src/hll.py:37-54):
 // Z = sum_j 2^-r_j (zero registers included), zeros = #{j : r_j = 0}.
-__device__ __forceinline__ double hll_estimate(int m, uint64_t zeros, double Z) {
+// TC_HLL_CPU_EXP (cpu_exp/triangle_hll_experiment.py:356-365): LinearCounting
+// or the 2^32 large-range correction of the raw estimate, each returned as is.
+__device__ __forceinline__ double hll_estimate(int m, uint64_t zeros, double Z,
+                                              int mode = TC_HLL_GPU) {
   const double md = (double)m;
   double E = hll_alpha(m) * md * md / Z;
+  if (mode == TC_HLL_CPU_EXP) {
+    if (E <= 2.5 * md && zeros > 0) return md * log(md / (double)zeros);
+    const double two32 = 4294967296.0;
+    if (E > two32 / 30.0) return -two32 * log(1.0 - E / two32);
+    return E;
+  }
   if (E <= 2.5 * md && zeros > 0) E = md * log(md / (double)zeros);
   const double two64 = 18446744073709551616.0;
   if (E > two64 / 30.0) E = -two64 * log(1.0 - E / two64);
@@ -148,7 +176,7 @@ __device__ __forceinline__ void wave_union_stats(const uint8_t *__restrict__ ra,
 __global__ void __launch_bounds__(kBlock)
 k_union_estimate(const uint8_t *__restrict__ regs, const int32_t *__restrict__ slot, int p,
                  const int32_t *__restrict__ us, const int32_t *__restrict__ vs, int64_t k,
-                 double *__restrict__ out) {
+                 double *__restrict__ out, int mode) {
   const int m = 1 << p;
   const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
@@ -158,7 +186,7 @@ k_union_estimate(const uint8_t *__restrict__ regs, const int32_t *__restrict__ s
     uint64_t zeros;
     double z;
     wave_union_stats(regs + a * m, regs + b * m, m, zeros, z);
-    if (lane_id() == 0) out[i] = hll_estimate(m, zeros, z);
+    if (lane_id() == 0) out[i] = hll_estimate(m, zeros, z, mode);
   }
 }
 
@@ -257,7 +285,7 @@ k_hll_edges(Degrees<I> D, const int32_t *__restrict__ eu, const int32_t *__restr
             const int32_t *__restrict__ list, const int64_t *__restrict__ n_list_p,
             const uint8_t *__restrict__ regs, const int32_t *__restrict__ slot, int p,
             double *__restrict__ s_out, double *__restrict__ per_edge,
-            unsigned long long *__restrict__ n_clamped) {
+            unsigned long long *__restrict__ n_clamped, int mode) {
   const int m = 1 << p;
   const int64_t n_list = *n_list_p;
   const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -272,7 +300,7 @@ k_hll_edges(Degrees<I> D, const int32_t *__restrict__ eu, const int32_t *__restr
     double z;
     wave_union_stats(regs + a * m, regs + b * m, m, zeros, z);
     if (lane_id() == 0) {
-      const double E = hll_estimate(m, zeros, z);
+      const double E = hll_estimate(m, zeros, z, mode);
       const int64_t du = D(u), dv = D(v);
       double s = (double)du + (double)dv - E;
       const double upper = (double)min(du, dv);
@@ -411,9 +439,11 @@ extern "C" int tc_hash64(const uint64_t *x, uint64_t *out, int64_t n, void *stre
 extern "C" int tc_hll_build(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
                             int64_t n_nodes, int64_t theta, int p, uint8_t *regs,
                             const int32_t *slot, uint8_t *has_sketch, int accumulate,
-                            void *stream) {
+                            int hash_mode, uint64_t seed, void *stream) {
   using namespace tc;
   TC_CHECK(p >= 4 && p <= 16, TC_ERR_INVALID, "p must be in [4, 16], got %d", p);
+  TC_CHECK(hash_mode == TC_HLL_GPU || hash_mode == TC_HLL_CPU_EXP, TC_ERR_INVALID,
+           "unknown hash mode %d", hash_mode);
   TC_CHECK(row_ptr && has_sketch && n_nodes >= 0, TC_ERR_INVALID,
            "row_ptr and has_sketch are required");
   TC_CHECK(n_nodes == 0 || regs, TC_ERR_INVALID, "regs is required");
@@ -435,7 +465,7 @@ extern "C" int tc_hll_build(const void *row_ptr, int row_ptr_64, const int32_t *
       StageTimer _timer(TC_STAGE_HLL_BUILD, s);
       hipLaunchKernelGGL(k_hll_build<I>, dim3(g), dim3(kBlock), (size_t)1 << p, s, R, col_idx,
                          (const int32_t *)list.as<int32_t>(), (const int64_t *)cnt.as<int64_t>(), p,
-                         regs, slot, accumulate);
+                         regs, slot, accumulate, hash_mode, seed);
     }
     TC_HIP(hipGetLastError());
   });
@@ -444,16 +474,18 @@ extern "C" int tc_hll_build(const void *row_ptr, int row_ptr_64, const int32_t *
 
 extern "C" int tc_union_estimate(const uint8_t *regs, const int32_t *slot, int p,
                                  const int32_t *us, const int32_t *vs, int64_t k, double *out,
-                                 void *stream) {
+                                 int hash_mode, void *stream) {
   using namespace tc;
   TC_CHECK(p >= 4 && p <= 16, TC_ERR_INVALID, "p must be in [4, 16], got %d", p);
+  TC_CHECK(hash_mode == TC_HLL_GPU || hash_mode == TC_HLL_CPU_EXP, TC_ERR_INVALID,
+           "unknown hash mode %d", hash_mode);
   TC_CHECK(k >= 0 && (k == 0 || (regs && us && vs && out)), TC_ERR_INVALID,
            "invalid arguments to tc_union_estimate");
   TC_CHECK(((uintptr_t)regs & 15) == 0, TC_ERR_INVALID, "regs must be 16-byte aligned");
   TC_TRY(ensure_init());
   if (k == 0) return TC_OK;
   hipLaunchKernelGGL(k_union_estimate, dim3(grid_for(k * kWave, kBlock, 65536)), dim3(kBlock), 0,
-                     (hipStream_t)stream, regs, slot, p, us, vs, k, out);
+                     (hipStream_t)stream, regs, slot, p, us, vs, k, out, hash_mode);
   TC_HIP(hipGetLastError());
   return TC_OK;
 }
@@ -483,8 +515,11 @@ extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_
                                int64_t n_edges, const int32_t *degrees, const uint8_t *regs,
                                const int32_t *slot, const uint8_t *has_sketch, int64_t gamma, int p,
                                const uint8_t *route, int part, double *counts, int64_t *stats,
-                               int64_t *fallback, double *per_edge, void *stream) {
+                               int64_t *fallback, double *per_edge, int hash_mode,
+                               void *stream) {
   using namespace tc;
+  TC_CHECK(hash_mode == TC_HLL_GPU || hash_mode == TC_HLL_CPU_EXP, TC_ERR_INVALID,
+           "unknown hash mode %d", hash_mode);
   TC_CHECK(part == TC_PART_BOTH || part == TC_PART_EXACT || part == TC_PART_HLL, TC_ERR_INVALID,
            "unknown part %d", part);
   TC_CHECK(part == TC_PART_EXACT || (p >= 4 && p <= 16), TC_ERR_INVALID,
@@ -537,7 +572,8 @@ extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_
           StageTimer _timer(TC_STAGE_HLL_EDGES, s);
           hipLaunchKernelGGL(k_hll_edges<I>, dim3(gw), dim3(kBlock), 0, s, D, edge_u, edge_v,
                              (const int32_t *)list.as<int32_t>(), (const int64_t *)nlist.as<int64_t>(),
-                             regs, slot, p, svals.as<double>(), per_edge, counters + 2);
+                             regs, slot, p, svals.as<double>(), per_edge, counters + 2,
+                             hash_mode);
         }
         hipLaunchKernelGGL(k_tile_sums_f64, dim3((unsigned)std::min<int64_t>(max_tiles, 65536)),
                            dim3(kBlock), 0, s, (const double *)svals.as<double>(),

@@ -65,8 +65,14 @@ def alloc_sketch_arrays_device(n_rows: int, p: int, n_nodes: int = None, device=
 
 
 def build_sketches(row_ptr, col_idx, n_nodes: int, theta: int, p: int, sketch_regs,
-                   has_sketch, slot=None, accumulate: bool = True, stream=None) -> None:
-    """Device sketch build in place (see ``tc_hll_build`` in include/tchll.h)."""
+                   has_sketch, slot=None, accumulate: bool = True, hash_mode: str = "gpu",
+                   seed: int = 0, stream=None) -> None:
+    """Device sketch build in place (see ``tc_hll_build`` in include/tchll.h).
+
+    ``hash_mode="cpu_exp"`` builds the CPU experiment's registers
+    (cpu_exp/triangle_hll_experiment.py:384-398: seeded splitmix64, low-p
+    index, rank from the high bits) with ``seed``.
+    """
     _check_p(p)
     t = _lib.require_device()
     rp, is64 = _lib.row_ptr_arg(row_ptr)
@@ -90,6 +96,7 @@ def build_sketches(row_ptr, col_idx, n_nodes: int, theta: int, p: int, sketch_re
     _lib.check(_lib.load().tc_hll_build(_lib.ptr(rp), is64, _lib.ptr(col), int(n_nodes),
                                         int(theta), int(p), _lib.ptr(sketch_regs), _lib.ptr(sl),
                                         _lib.ptr(has_sketch), int(bool(accumulate)),
+                                        _lib.hash_mode(hash_mode), int(seed) & ((1 << 64) - 1),
                                         _lib.stream_ptr(stream)))
 
 
@@ -123,10 +130,13 @@ class _BuildSketchesLauncher:
 build_sketches_kernel = _BuildSketchesLauncher()
 
 
-def union_estimate_pairs(sketch_regs, us, vs, p: int, slot=None, stream=None):
+def union_estimate_pairs(sketch_regs, us, vs, p: int, slot=None, hash_mode: str = "gpu",
+                         stream=None):
     """FP64 estimates of ``|N+(u) ∪ N+(v)|`` for each pair (device float64 tensor).
 
-    Pass ``us == vs`` for single-sketch estimates.
+    Pass ``us == vs`` for single-sketch estimates.  ``hash_mode="cpu_exp"`` uses
+    the CPU experiment's estimator (cpu_exp/triangle_hll_experiment.py:356-365,
+    615-635).
     """
     _check_p(p)
     t = _lib.require_device()
@@ -139,7 +149,7 @@ def union_estimate_pairs(sketch_regs, us, vs, p: int, slot=None, stream=None):
     out = t.empty(u.numel(), dtype=t.float64, device=regs.device)
     _lib.check(_lib.load().tc_union_estimate(_lib.ptr(regs), _lib.ptr(sl), int(p), _lib.ptr(u),
                                              _lib.ptr(v), u.numel(), _lib.ptr(out),
-                                             _lib.stream_ptr(stream)))
+                                             _lib.hash_mode(hash_mode), _lib.stream_ptr(stream)))
     return out
 
 

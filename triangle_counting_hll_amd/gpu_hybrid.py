@@ -104,7 +104,7 @@ def edge_supports(row_ptr, col_idx, edge_u, edge_v, stream=None):
 
 def hybrid_count(row_ptr, col_idx, edge_u, edge_v, degrees, sketch_regs, has_sketch, gamma,
                  p, part=_lib.TC_PART_BOTH, counts=None, stats=None, fallback=None,
-                 route=None, slot=None, per_edge=None, stream=None):
+                 route=None, slot=None, per_edge=None, hash_mode: str = "gpu", stream=None):
     """Enqueue one hybrid pass (``tc_hybrid_count``); returns ``(counts, stats, fallback)``."""
     t = _lib.require_device()
     eu = _lib.as_device(edge_u, t.int32, "edge_u")
@@ -134,7 +134,7 @@ def hybrid_count(row_ptr, col_idx, edge_u, edge_v, degrees, sketch_regs, has_ske
         _lib.ptr(rp), is64, _lib.ptr(col), n, _lib.ptr(eu), _lib.ptr(ev), eu.numel(),
         _lib.ptr(deg), _lib.ptr(regs), _lib.ptr(sl), _lib.ptr(has), int(gamma), int(p),
         _lib.ptr(rt), int(part), _lib.ptr(counts), _lib.ptr(stats), _lib.ptr(fallback),
-        _lib.ptr(per_edge), _lib.stream_ptr(stream)))
+        _lib.ptr(per_edge), _lib.hash_mode(hash_mode), _lib.stream_ptr(stream)))
     return counts, stats, fallback
 
 
@@ -216,7 +216,7 @@ def gpu_exact_triangle_count(row_ptr_d, col_idx_d, edge_u_d, edge_v_d, params=No
 def gpu_hybrid_triangle_count(row_ptr_d, col_idx_d, edge_u_d, edge_v_d, degrees_d,
                               sketch_regs_d, has_sketch_d, gamma: int, p: int,
                               split_kernels: bool = True, truth: Optional[int] = None,
-                              route=None, slot=None):
+                              route=None, slot=None, hash_mode: str = "gpu"):
     """Hybrid estimate, wall seconds and stats (src/gpu_hybrid.py:357-431).
 
     ``stats`` holds the reference keys ``n_approx, n_exact, n_clamped,
@@ -243,12 +243,12 @@ def gpu_hybrid_triangle_count(row_ptr_d, col_idx_d, edge_u_d, edge_v_d, degrees_
     t0 = time.perf_counter()
     if split_kernels:
         hybrid_count(rp, col, eu, ev, deg, None, has, gamma, p, _lib.TC_PART_EXACT, counts,
-                     stats_d, fb, route=route, slot=slot)
+                     stats_d, fb, route=route, slot=slot, hash_mode=hash_mode)
         hybrid_count(None, None, eu, ev, deg, regs, has, gamma, p, _lib.TC_PART_HLL, counts,
-                     stats_d, None, route=route, slot=slot)
+                     stats_d, None, route=route, slot=slot, hash_mode=hash_mode)
     else:
         hybrid_count(rp, col, eu, ev, deg, regs, has, gamma, p, _lib.TC_PART_BOTH, counts,
-                     stats_d, fb, route=route, slot=slot)
+                     stats_d, fb, route=route, slot=slot, hash_mode=hash_mode)
     _sync()
     elapsed = time.perf_counter() - t0
     estimate = float(counts.item())
@@ -269,6 +269,61 @@ def gpu_hybrid_triangle_count(row_ptr_d, col_idx_d, edge_u_d, edge_v_d, degrees_
     if truth is not None:
         stats["rel_err"] = relative_error(estimate, truth)
     return estimate, elapsed, stats
+
+
+def gpu_pure_hll_triangle_count(edge_u_d, edge_v_d, degrees_d, sketch_regs_d, p: int,
+                                slot=None, hash_mode: str = "gpu",
+                                truth: Optional[int] = None):
+    """Every edge by its HLL support: ``(estimate, seconds, stats)``.
+
+    The pure-HLL mode of cpu_exp/triangle_hll_experiment.py:637-660
+    (``pure_hll_count``): s = deg u + deg v - union, clamped to [0, min] with
+    the clamps counted; every vertex must have a register row.  ``stats`` has
+    ``n_approx``, ``n_clamped``, ``clamp_ratio`` and, with ``truth``, ``rel_err``.
+    """
+    t = _lib.require_device()
+    deg = _lib.as_device(degrees_d, t.int32, "degrees")
+    eu = _lib.as_device(edge_u_d, t.int32, "edge_u")
+    has = t.ones(deg.numel(), dtype=t.uint8, device=deg.device)
+    counts = t.zeros(1, dtype=t.float64, device=deg.device)
+    stats_d = t.zeros(3, dtype=t.int64, device=deg.device)
+    _sync()
+    t0 = time.perf_counter()
+    hybrid_count(None, None, eu, edge_v_d, deg, sketch_regs_d, has, 0, p, _lib.TC_PART_HLL,
+                 counts, stats_d, None, slot=slot, hash_mode=hash_mode)
+    _sync()
+    elapsed = time.perf_counter() - t0
+    est = float(counts.item())
+    s = stats_d.cpu().numpy()
+    stats = {"n_approx": int(s[0]), "n_clamped": int(s[2]),
+             "clamp_ratio": int(s[2]) / int(s[0]) if s[0] else 0.0}
+    if truth is not None:
+        stats["rel_err"] = relative_error(est, truth)
+    return est, elapsed, stats
+
+
+def cpu_exp_hybrid_count(row_ptr_d, col_idx_d, edge_u_d, edge_v_d, degrees_d, sketch_regs_d,
+                         p: int, threshold: int, slot=None, hash_mode: str = "cpu_exp"):
+    """The CPU experiment's hybrid (cpu_exp/triangle_hll_experiment.py:662-709):
+    exact when min(deg u, deg v) <= threshold, else HLL, every vertex sketched.
+
+    Returns ``(total, exact_edges, hll_edges, exact_sum, hll_sum, clamp_count)``.
+    """
+    t = _lib.require_device()
+    rp, _ = _lib.row_ptr_arg(row_ptr_d)
+    deg = _lib.as_device(degrees_d, t.int32, "degrees")
+    has = t.ones(deg.numel(), dtype=t.uint8, device=deg.device)
+    ex = t.zeros(1, dtype=t.float64, device=deg.device)
+    hl = t.zeros(1, dtype=t.float64, device=deg.device)
+    st = t.zeros(3, dtype=t.int64, device=deg.device)
+    gamma = int(threshold) + 1  # HLL iff min > threshold
+    hybrid_count(rp, col_idx_d, edge_u_d, edge_v_d, deg, None, has, gamma, p,
+                 _lib.TC_PART_EXACT, ex, st, None, slot=slot, hash_mode=hash_mode)
+    hybrid_count(None, None, edge_u_d, edge_v_d, deg, sketch_regs_d, has, gamma, p,
+                 _lib.TC_PART_HLL, hl, st, None, slot=slot, hash_mode=hash_mode)
+    exact_sum, hll_sum = float(ex.item()), float(hl.item())
+    s = st.cpu().numpy()
+    return (exact_sum + hll_sum, int(s[1]), int(s[0]), exact_sum, hll_sum, int(s[2]))
 
 
 def relative_error(estimate: float, truth: int) -> float:
