@@ -1,0 +1,30 @@
+"""Copies published result files of the reference into this directory as
+fixtures (data only):
+
+* orkut_graph_stats/: the degree histogram CSV, the log2-bucket CSV and the
+  summary JSON written by experiment/gpu_graph_stats.py:186-320 on
+  com-orkut.ungraph.txt;
+* exp_gpu_schema/ca-hepth_exp_gpu_full.json: one result file of
+  experiment/exp_gpu.py:641-933 (run_exp_gpu), the schema the experiment
+  harness writes.
+
+Run once in a checkout that has the reference at /root/reference; the tests
+read only the copies."""
+
+import os
+import shutil
+
+REF = "/root/reference/results"
+HERE = os.path.dirname(os.path.abspath(__file__))
+FILES = [
+    ("orkut_graph_stats/degree_distribution.csv", "orkut_graph_stats/degree_distribution.csv"),
+    ("orkut_graph_stats/degree_log2_buckets.csv", "orkut_graph_stats/degree_log2_buckets.csv"),
+    ("orkut_graph_stats/graph_stats.json", "orkut_graph_stats/graph_stats.json"),
+    ("ca-hepth/exp_gpu_full.json", "exp_gpu_schema/ca-hepth_exp_gpu_full.json"),
+]
+
+if __name__ == "__main__":
+    for src, dst in FILES:
+        os.makedirs(os.path.dirname(os.path.join(HERE, dst)), exist_ok=True)
+        shutil.copyfile(os.path.join(REF, src), os.path.join(HERE, dst))
+        print("copied", src, "->", dst)
