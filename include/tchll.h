@@ -130,7 +130,10 @@ typedef struct tc_exact_params {
   int waves_per_cu;  /* persistent waves per CU for the heavy kernel           */
   int unroll;        /* streamed ids in flight per lane: 1, 2, 4 or 8          */
   int plan;          /* TC_PLAN_SORTED (default) or TC_PLAN_ATOMIC             */
-  int reserved[3];
+  int block_min_bits; /* centers with bit_length(deg+) in [block_min_bits, 12]
+                         are taken by whole workgroups over one joined LDS
+                         table: 8..13 (13 = none), 0 = 11 (deg+ >= 1024)      */
+  int reserved[2];
 } tc_exact_params;
 
 /* How heavy edges are grouped by center before the intersection kernel:

@@ -73,16 +73,29 @@ def test_rmat14_config(device):
 
 
 @pytest.mark.parametrize("plan", ["sorted", "atomic"])
-@pytest.mark.parametrize("light_max,chunk,wpc,unroll", [(0, 1, 4, 1), (2, 7, 8, 2),
-                                                        (16, 256, 16, 4), (64, 32, 32, 8),
-                                                        (10**6, 256, 16, 4)])
-def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc, unroll, plan):
+@pytest.mark.parametrize("light_max,chunk,wpc,unroll,blk", [(0, 1, 4, 1, 8), (2, 7, 8, 2, 13),
+                                                            (16, 256, 16, 4, 0),
+                                                            (64, 32, 32, 8, 10),
+                                                            (10**6, 256, 16, 4, 9)])
+def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc, unroll, blk, plan):
     pairs = rmat(12, 16, seed=7)
     truth = _truth(pairs)
     csr = gpu_graph.build_csr_gpu(pairs)
     params = {"light_max": light_max, "chunk": chunk, "waves_per_cu": wpc, "unroll": unroll,
-              "plan": plan}
+              "plan": plan, "block_min_bits": blk}
     assert _count(csr, params=params) == truth
+
+
+@pytest.mark.parametrize("blk", [8, 10, 13])
+def test_block_phase_bounds_on_cliques(device, blk):
+    # K_1400 rows of 128..1399 ids under every workgroup-phase lower bound,
+    # including none (13: rows above 1023 then take the global search)
+    n = 1400
+    pairs = np.stack(np.triu_indices(n, 1), axis=1).astype(np.int32)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    assert _count(csr, params={"block_min_bits": blk}) == math.comb(n, 3)
+    with pytest.raises(ValueError, match="block_min_bits"):
+        _count(csr, params={"block_min_bits": 7})
 
 
 def test_rgg_small(device):

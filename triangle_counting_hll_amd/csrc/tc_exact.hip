@@ -25,7 +25,7 @@ namespace tc {
 
 constexpr int kTblInts = 2048;    // per-wave LDS hash table: 512 buckets x 4 slots (8 KiB)
 constexpr int kTblMaxDeg = 1023;  // longer center rows: one table per workgroup
-constexpr int kBlkMinBits = 11;   // d+ in [1024, 4095]: the workgroup's four wave tables
+constexpr int kBlkMinBits = 11;   // default: d+ in [1024, 4095] take the workgroup's four wave tables
 constexpr int kBlkTblInts = 4 * kTblInts;  // joined into one (2048 buckets, 32 KiB)
 constexpr int kBigMinBits = 13;   // d+ >= 4096: the 128 KiB workgroup kernel
 constexpr int kBigTblInts = 32768;  // block-shared table: 8192 buckets x 4 slots (128 KiB)
@@ -203,14 +203,22 @@ __device__ __forceinline__ void table_build(int *__restrict__ tbl, const int32_t
   if constexpr (kBlockWide) __syncthreads(); else wave_lds_sync();
 }
 
-__device__ __forceinline__ uint32_t table_probe(const int4 *__restrict__ tbl, int x, int shift,
-                                                uint32_t mask) {
-  uint32_t b = bucket_of((uint32_t)x, shift);
+// 0 iff x is one of the bucket's four slots (distance by xor, straight-line
+// VALU work with no per-slot branches).
+__device__ __forceinline__ uint32_t bucket_miss(int4 v, int x) {
+  return min(min((uint32_t)(v.x ^ x), (uint32_t)(v.y ^ x)),
+             min((uint32_t)(v.z ^ x), (uint32_t)(v.w ^ x)));
+}
+
+// Membership of x (not -1, the empty-slot marker) probing buckets b, b + 1, ...
+// until x or a bucket whose last slot is empty.
+__device__ __forceinline__ uint32_t table_probe_from(const int4 *__restrict__ tbl, int x,
+                                                     uint32_t b, uint32_t mask) {
   while (true) {
     const int4 v = tbl[b];  // one ds_read_b128
-    const bool hit = (v.x == x) | (v.y == x) | (v.z == x) | (v.w == x);
-    if (hit) return 1;
-    if (v.w == -1) return 0;
+    const uint32_t miss = bucket_miss(v, x);
+    if (miss == 0) return 1u;
+    if (v.w == -1) return 0u;
     b = (b + 1) & mask;
   }
 }
@@ -253,7 +261,7 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
                                                   const int4 *__restrict__ tbl, int shift,
                                                   uint32_t mask, int64_t cb, int dc) {
   const int lane = lane_id();
-  uint64_t acc = 0;
+  uint32_t acc = 0;  // at most chunk * kBigMaxDeg hits per call
   for (int64_t base = ob; base < oe; base += kWave) {
     const int64_t k = base + lane;
     int len = 0;
@@ -279,7 +287,7 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int t = t0 + u * kWave;
-        xs[u] = -1;
+        xs[u] = -2;  // past the end: matches no id and no empty slot
         if (t < total) {
           while (seg_hi <= t) {
             ++seg;
@@ -289,14 +297,34 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
           xs[u] = col[seg_off + t];
         }
       }
+      if constexpr (kLds) {
+        // all U home buckets are read before the first compare (one LDS
+        // wait), then the few lanes whose home bucket is full without the
+        // id continue along the probe sequence
+        uint32_t bk[U];
+        int4 vv[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (xs[u] >= 0) {
-          if constexpr (kLds)
-            acc += table_probe(tbl, xs[u], shift, mask);
-          else
-            acc += global_probe(col, cb, dc, xs[u]);
+        for (int u = 0; u < U; ++u) {
+          bk[u] = bucket_of((uint32_t)xs[u], shift);
+          vv[u] = tbl[bk[u]];
         }
+        bool more = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t miss = bucket_miss(vv[u], xs[u]);
+          acc += miss == 0 ? 1u : 0u;
+          more |= (miss != 0) & (vv[u].w != -1);
+        }
+        if (more) {
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (bucket_miss(vv[u], xs[u]) != 0 && vv[u].w != -1)
+              acc += table_probe_from(tbl, xs[u], (bk[u] + 1) & mask, mask);
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (xs[u] >= 0) acc += global_probe(col, cb, dc, xs[u]);
       }
     }
     wave_lds_sync();
@@ -324,7 +352,8 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
   int64_t *seg_base = seg_base_all[w];
   const int64_t n_items = *n_items_p;
   // items [0, huge) belong to k_heavy_big; [huge, first) are centers with
-  // d+ in [1024, 4095], taken by whole workgroups over one joined table; then
+  // d+ in [2^(block_min_bits - 1), 4095], taken by whole workgroups over one
+  // joined table; then
   // the items of the long centers, one per wavefront ticket; past those a
   // ticket covers kSmallBatch items
   const int64_t huge = iptr[*n_huge_runs_p];
@@ -634,7 +663,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              const int32_t *eu, const int32_t *ev, int64_t E,
                              const uint8_t *skip, const int32_t *graph_of,
                              unsigned long long *count, int light_max, int chunk, int wpc,
-                             int unroll, hipStream_t s) {
+                             int unroll, int blk_bits, hipStream_t s) {
   Scratch eu_buf;
   if (!eu) {
     TC_TRY(eu_buf.alloc(sizeof(int32_t) * E, s));
@@ -713,7 +742,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
       const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
-      const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - kBlkMinBits + 1);
+      const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - blk_bits + 1);
       const auto kbig = heavy_big_kernel<I, OthersPacked>(unroll);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)run_ptr.as<int64_t>(),
@@ -738,14 +767,19 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                      const int32_t *graph_of, unsigned long long *count,
                      const tc_exact_params *params, hipStream_t s) {
   const int light_max = (params && params->light_max > 0) ? params->light_max : 32;
-  const int chunk = (params && params->chunk > 0) ? params->chunk : 256;
+  // at most 65536 others per work item keeps each lane's 32-bit hit count exact
+  const int chunk = (params && params->chunk > 0) ? std::min(params->chunk, 65536) : 256;
   const int wpc = (params && params->waves_per_cu > 0) ? params->waves_per_cu : 16;
   const int unroll = (params && params->unroll > 0) ? params->unroll : kDefaultUnroll;
   const int plan = params ? params->plan : TC_PLAN_SORTED;
+  const int blk_bits = (params && params->block_min_bits > 0) ? params->block_min_bits
+                                                              : kBlkMinBits;
+  TC_CHECK(blk_bits >= kBigCenterBits && blk_bits <= kBigMinBits, TC_ERR_INVALID,
+           "block_min_bits must be in [%d, %d], got %d", kBigCenterBits, kBigMinBits, blk_bits);
   if (E <= 0 || n <= 0) return TC_OK;
   if (plan == TC_PLAN_SORTED)
     return exact_sorted_plan(row_ptr, row_ptr_64, col, n, eu, ev, E, skip, graph_of, count,
-                             light_max, chunk, wpc, unroll, s);
+                             light_max, chunk, wpc, unroll, blk_bits, s);
 
   Scratch eu_buf;
   if (!eu) {
@@ -811,7 +845,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
       const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
-      const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - kBlkMinBits + 1);
+      const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - blk_bits + 1);
       const auto kbig = heavy_big_kernel<I, OthersIds>(unroll);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)cptr.as<int64_t>(),
