@@ -132,7 +132,7 @@ typedef struct tc_exact_params {
   int plan;          /* TC_PLAN_SORTED (default) or TC_PLAN_ATOMIC             */
   int block_min_bits; /* centers with bit_length(deg+) in [block_min_bits, 12]
                          are taken by whole workgroups over one joined LDS
-                         table: 8..13 (13 = none), 0 = 11 (deg+ >= 1024)      */
+                         table: 8..13 (13 = none), 0 = 10 (deg+ >= 512)       */
   int reserved[2];
 } tc_exact_params;
 

@@ -25,7 +25,7 @@ namespace tc {
 
 constexpr int kTblInts = 2048;    // per-wave LDS hash table: 512 buckets x 4 slots (8 KiB)
 constexpr int kTblMaxDeg = 1023;  // longer center rows: one table per workgroup
-constexpr int kBlkMinBits = 11;   // default: d+ in [1024, 4095] take the workgroup's four wave tables
+constexpr int kBlkMinBits = 10;   // default: d+ in [512, 4095] take the workgroup's four wave tables
 constexpr int kBlkTblInts = 4 * kTblInts;  // joined into one (2048 buckets, 32 KiB)
 constexpr int kBigMinBits = 13;   // d+ >= 4096: the 128 KiB workgroup kernel
 constexpr int kBigTblInts = 32768;  // block-shared table: 8192 buckets x 4 slots (128 KiB)
