@@ -14,6 +14,24 @@ constexpr int kRadixDigits = 1 << kRadixBits;
 constexpr int kRadixRounds = 16;                              // keys per lane
 constexpr int kRadixWaveKeys = kWave * kRadixRounds;         // 1024
 constexpr int kRadixTile = kRadixWaveKeys * (kBlock / kWave);  // 4096
+static_assert(kBlock == kRadixDigits, "the scatter assigns one digit per thread");
+
+// Exclusive scan of one int per thread over the workgroup (*total = sum).
+__device__ __forceinline__ int block_exclusive_scan_int(int x, int *wave_tot, int *total) {
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int inc = wave_inclusive_scan(x);
+  if (lane == kWave - 1) wave_tot[w] = inc;
+  __syncthreads();
+  int before = 0, sum = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / kWave; ++i) {
+    const int t = wave_tot[i];
+    before += (i < w) ? t : 0;
+    sum += t;
+  }
+  *total = sum;
+  return before + inc - x;
+}
 
 __global__ void __launch_bounds__(kBlock)
 k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift, int dmask, int64_t ntiles,
@@ -34,17 +52,25 @@ k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift, int dmask,
     counts[(int64_t)d * ntiles + blockIdx.x] = hist[d];
 }
 
+// Scatter of one tile: keys are ranked (stable) inside the tile, staged in
+// LDS in digit order, then written out digit run by digit run, so that
+// consecutive threads write consecutive addresses of the output.
 __global__ void __launch_bounds__(kBlock)
 k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int64_t n, int shift,
                 int dmask, int64_t ntiles, const int64_t *__restrict__ offs) {
-  __shared__ int wcnt[kBlock / kWave][kRadixDigits];
-  __shared__ int64_t wpos[kBlock / kWave][kRadixDigits];
+  __shared__ int wpos[kBlock / kWave][kRadixDigits];  // counts, then running positions
+  __shared__ int tstart[kRadixDigits];                // first tile slot of each digit
+  __shared__ int64_t gstart[kRadixDigits];            // first output slot of each digit
+  __shared__ uint64_t stage[kRadixTile];
+  __shared__ int wave_tot[kBlock / kWave];
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int i = threadIdx.x; i < (kBlock / kWave) * kRadixDigits; i += kBlock)
-    (&wcnt[0][0])[i] = 0;
-  const int64_t base = (int64_t)blockIdx.x * kRadixTile + (int64_t)w * kRadixWaveKeys + lane;
+    (&wpos[0][0])[i] = 0;
+  const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
+  const int tile_n = (int)min<int64_t>(kRadixTile, n - t0);
+  const int64_t base = t0 + (int64_t)w * kRadixWaveKeys + lane;
   uint64_t k[kRadixRounds];
 #pragma unroll
   for (int r = 0; r < kRadixRounds; ++r) {
@@ -53,7 +79,8 @@ k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int
   }
   __syncthreads();
 
-  // per-wavefront digit counts
+  // per-wavefront digit counts; peers = lanes of this round with the same digit
+  uint64_t peer[kRadixRounds];
 #pragma unroll
   for (int r = 0; r < kRadixRounds; ++r) {
     const bool valid = base + (int64_t)r * kWave < n;
@@ -64,38 +91,50 @@ k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int
       const uint64_t m = __ballot((d >> bit) & 1);
       peers &= ((d >> bit) & 1) ? m : ~m;
     }
-    if (valid && (peers & lt) == 0) atomicAdd(&wcnt[w][d], __popcll(peers));
+    peer[r] = valid ? peers : 0;
+    if (valid && (peers & lt) == 0) atomicAdd(&wpos[w][d], __popcll(peers));
   }
   __syncthreads();
 
-  // start of each (wavefront, digit) run in the output
-  for (int d = threadIdx.x; d < kRadixDigits; d += kBlock) {
-    int64_t p = offs[(int64_t)d * ntiles + blockIdx.x];
+  // thread d: the tile's digit-d slots start after all smaller digits; the
+  // wavefronts' runs follow each other inside a digit (wavefront order is
+  // input order)
+  {
+    const int d = threadIdx.x;  // kBlock == kRadixDigits
+    int c[kBlock / kWave], tot = 0;
+#pragma unroll
+    for (int ww = 0; ww < kBlock / kWave; ++ww) {
+      c[ww] = wpos[ww][d];
+      tot += c[ww];
+    }
+    int dummy;
+    const int st = block_exclusive_scan_int(tot, wave_tot, &dummy);
+    tstart[d] = st;
+    gstart[d] = offs[(int64_t)d * ntiles + blockIdx.x];
+    int p = st;
 #pragma unroll
     for (int ww = 0; ww < kBlock / kWave; ++ww) {
       wpos[ww][d] = p;
-      p += wcnt[ww][d];
+      p += c[ww];
     }
   }
   __syncthreads();
 
 #pragma unroll
   for (int r = 0; r < kRadixRounds; ++r) {
-    const bool valid = base + (int64_t)r * kWave < n;
+    const uint64_t peers = peer[r];
     const int d = (int)(k[r] >> shift) & dmask;
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int bit = 0; bit < kRadixBits; ++bit) {
-      const uint64_t m = __ballot((d >> bit) & 1);
-      peers &= ((d >> bit) & 1) ? m : ~m;
-    }
-    if (valid) {
-      const int64_t pos = wpos[w][d] + __popcll(peers & lt);
-      out[pos] = k[r];
-    }
+    if (peers) stage[wpos[w][d] + __popcll(peers & lt)] = k[r];
     wave_lds_sync();
-    if (valid && (peers & lt) == 0) wpos[w][d] += __popcll(peers);
+    if (peers && (peers & lt) == 0) wpos[w][d] += __popcll(peers);
     wave_lds_sync();
+  }
+  __syncthreads();
+
+  for (int i = threadIdx.x; i < tile_n; i += kBlock) {
+    const uint64_t key = stage[i];
+    const int d = (int)(key >> shift) & dmask;
+    out[gstart[d] + (i - tstart[d])] = key;
   }
 }
 
