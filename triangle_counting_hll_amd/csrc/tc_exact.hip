@@ -282,19 +282,33 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
     int seg = 0;
     int seg_hi = seg_end[0];
     int64_t seg_off = seg_base[0];
-    for (int t0 = lane; t0 < total; t0 += kWave * U) {
+    for (int w0 = 0; w0 < total; w0 += kWave * U) {
+      const int t0 = w0 + lane;
       int xs[U];
+      if (w0 + kWave * U <= total) {  // uniform: every position of the step exists
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int t = t0 + u * kWave;
-        xs[u] = -2;  // past the end: matches no id and no empty slot
-        if (t < total) {
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + u * kWave;
           while (seg_hi <= t) {
             ++seg;
             seg_hi = seg_end[seg];
             seg_off = seg_base[seg];
           }
           xs[u] = col[seg_off + t];
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + u * kWave;
+          xs[u] = -2;  // past the end: matches no id and no empty slot
+          if (t < total) {
+            while (seg_hi <= t) {
+              ++seg;
+              seg_hi = seg_end[seg];
+              seg_off = seg_base[seg];
+            }
+            xs[u] = col[seg_off + t];
+          }
         }
       }
       if constexpr (kLds) {
