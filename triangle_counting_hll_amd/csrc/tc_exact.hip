@@ -315,25 +315,29 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
         // all U home buckets are read before the first compare (one LDS
         // wait), then the few lanes whose home bucket is full without the
         // id continue along the probe sequence
+        // groups of up to four reads keep the register footprint small
+        constexpr int G = U < 4 ? U : 4;
         uint32_t bk[U];
-        int4 vv[U];
+        uint32_t more = 0;  // bit u: the home bucket of xs[u] is full and lacks it
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          bk[u] = bucket_of((uint32_t)xs[u], shift);
-          vv[u] = tbl[bk[u]];
-        }
-        bool more = false;
+        for (int g = 0; g < U; g += G) {
+          int4 vv[G];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const uint32_t miss = bucket_miss(vv[u], xs[u]);
-          acc += miss == 0 ? 1u : 0u;
-          more |= (miss != 0) & (vv[u].w != -1);
+          for (int j = 0; j < G; ++j) {
+            bk[g + j] = bucket_of((uint32_t)xs[g + j], shift);
+            vv[j] = tbl[bk[g + j]];
+          }
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            const uint32_t miss = bucket_miss(vv[j], xs[g + j]);
+            acc += miss == 0 ? 1u : 0u;
+            more |= ((miss != 0) & (vv[j].w != -1)) ? (1u << (g + j)) : 0u;
+          }
         }
         if (more) {
 #pragma unroll
           for (int u = 0; u < U; ++u)
-            if (bucket_miss(vv[u], xs[u]) != 0 && vv[u].w != -1)
-              acc += table_probe_from(tbl, xs[u], (bk[u] + 1) & mask, mask);
+            if (more & (1u << u)) acc += table_probe_from(tbl, xs[u], (bk[u] + 1) & mask, mask);
         }
       } else {
 #pragma unroll
