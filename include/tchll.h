@@ -35,6 +35,16 @@ extern "C" {
 const char *tc_last_error(void);
 /* TC_ABI_VERSION of the loaded library. */
 int tc_abi_version(void);
+/* Scratch memory: an entry point that needs scratch takes it from a
+ * workspace kept per (device, stream), which grows on first use and is
+ * reused by every later call on that stream (the stream orders the calls), so
+ * repeated calls allocate nothing.  Calls that overlap on one stream from
+ * several host threads are not supported.  tc_release_workspaces
+ * synchronises the device and frees every workspace; call it only while no
+ * other call is in flight.  No reference counterpart (the reference allocates
+ * its accumulators per call, src/gpu_hybrid.py:343, 390-391). */
+int tc_release_workspaces(void);
+
 /* Device facts (host outputs).  Replaces the compute-capability probe of
  * src/gpu_hybrid.py:27-39 (_warn_if_float64_atomic_slow). */
 int tc_device_info(int *n_cu, int *wave_size, int64_t *lds_per_block,

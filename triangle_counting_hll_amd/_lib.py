@@ -36,6 +36,7 @@ _SIGNATURES = {
     "tc_last_error": (ctypes.c_char_p, []),
     "tc_abi_version": (I32, []),
     "tc_device_info": (I32, [P, P, P, P, ctypes.c_char_p, I32]),
+    "tc_release_workspaces": (I32, []),
     "tc_timing_enable": (I32, [I32]),
     "tc_timing_read": (I32, [P, P]),
     "tc_csr_build_begin": (I32, [P, I64, P, P, P, P]),
@@ -187,6 +188,11 @@ def row_ptr_arg(row_ptr):
         rp = rp.to(t.int64)
         return rp, 1
     return rp, 0
+
+
+def release_workspaces() -> None:
+    """Free the per-stream scratch workspaces (synchronises the device)."""
+    check(load().tc_release_workspaces())
 
 
 STAGES = ("classify", "heavy", "hll_build", "hll_edges")

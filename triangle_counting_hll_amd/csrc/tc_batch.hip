@@ -45,6 +45,7 @@ struct tc_batch_graph {
 extern "C" int tc_vertex_graph_ids(const int64_t *node_ids, int64_t n_nodes, const int64_t *id_off,
                                    int n_graphs, int32_t *graph_of, void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(n_nodes >= 0 && n_graphs > 0 && id_off && (n_nodes == 0 || (node_ids && graph_of)),
            TC_ERR_INVALID, "invalid arguments to tc_vertex_graph_ids");
   TC_TRY(ensure_init());
@@ -74,6 +75,7 @@ extern "C" int tc_batch_exact_count(const void *row_ptr, int row_ptr_64, const i
                                     uint64_t *counts, const tc_exact_params *params,
                                     void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_TRY(check_batch_args(row_ptr, col_idx, n_nodes, edge_u, edge_v, n_edges, graph_of, n_graphs,
                           counts));
   TC_TRY(ensure_init());

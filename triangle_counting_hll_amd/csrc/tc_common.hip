@@ -31,6 +31,98 @@ Arena *&current_arena() {
   return a;
 }
 
+// ---------------------------------------------------------------------------
+// Per-stream workspaces
+// ---------------------------------------------------------------------------
+namespace {
+struct Chunk {
+  char *base;
+  size_t cap;
+};
+struct Workspace {
+  std::vector<Chunk> chunks;
+  size_t chunk = 0, off = 0;  // bump position of the open frame
+};
+std::mutex g_ws_mu;
+std::vector<std::pair<std::pair<int, hipStream_t>, Workspace *>> g_ws;
+thread_local Workspace *t_ws = nullptr;  // workspace of this thread's open frame
+
+Workspace *workspace_for(hipStream_t s) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  for (auto &e : g_ws)
+    if (e.first.first == dev && e.first.second == s) return e.second;
+  Workspace *w = new Workspace();
+  g_ws.push_back({{dev, s}, w});
+  return w;
+}
+}  // namespace
+
+Frame::Frame(hipStream_t s) {
+  if (t_ws) return;  // nested: keep carving the open frame
+  Workspace *w = workspace_for(s);
+  if (!w) return;
+  w->chunk = 0;
+  w->off = 0;
+  t_ws = w;
+  outer = true;
+}
+
+Frame::~Frame() {
+  if (outer) t_ws = nullptr;
+}
+
+int workspace_alloc(size_t bytes, void **out) {
+  *out = nullptr;
+  Workspace *w = t_ws;
+  if (!w) return TC_OK;  // no open frame: the caller allocates from the pool
+  const size_t need = arena_round(bytes);
+  while (w->chunk < w->chunks.size()) {
+    Chunk &c = w->chunks[w->chunk];
+    if (w->off + need <= c.cap) {
+      *out = c.base + w->off;
+      w->off += need;
+      return TC_OK;
+    }
+    ++w->chunk;  // later pieces of this frame continue in the next chunk
+    w->off = 0;
+  }
+  // grow: a chunk at least twice the last one (64 MiB minimum)
+  size_t cap = std::max<size_t>(need, size_t(64) << 20);
+  if (!w->chunks.empty()) cap = std::max(cap, 2 * w->chunks.back().cap);
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, cap);
+  if (e != hipSuccess) {
+    cap = need;  // retry at the size needed
+    e = hipMalloc(&p, cap);
+  }
+  if (e != hipSuccess) {
+    set_error(e == hipErrorOutOfMemory ? TC_ERR_OOM : TC_ERR_HIP,
+              "workspace growth by %zu bytes failed: %s", cap, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? TC_ERR_OOM : TC_ERR_HIP;
+  }
+  w->chunks.push_back({static_cast<char *>(p), cap});
+  w->chunk = w->chunks.size() - 1;
+  w->off = need;
+  *out = p;
+  return TC_OK;
+}
+
+int release_workspaces() {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  if (!g_ws.empty() && hipDeviceSynchronize() != hipSuccess) {
+    set_error(TC_ERR_HIP, "device synchronisation failed");
+    return TC_ERR_HIP;
+  }
+  for (auto &e : g_ws) {
+    for (auto &c : e.second->chunks) (void)hipFree(c.base);
+    delete e.second;
+  }
+  g_ws.clear();
+  return TC_OK;
+}
+
 bool timing_on() { return g_timing && current_arena() == nullptr; }
 
 void timing_record(int stage, hipEvent_t a, hipEvent_t b) {
@@ -269,6 +361,8 @@ int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d
 extern "C" const char *tc_last_error(void) { return tc::g_msg; }
 
 extern "C" int tc_abi_version(void) { return TC_ABI_VERSION; }
+
+extern "C" int tc_release_workspaces(void) { return tc::release_workspaces(); }
 
 extern "C" int tc_timing_enable(int on) {
   std::lock_guard<std::mutex> lk(tc::g_timing_mu);

@@ -61,9 +61,26 @@ struct Arena {
 Arena *&current_arena();
 inline size_t arena_round(size_t b) { return (b + 255) & ~size_t(255); }
 
-// Stream-ordered scratch buffer from the device's default memory pool.  The
-// pool keeps freed blocks (release threshold raised at init), so repeated
-// calls do not go back to the driver.
+// Per-stream workspaces.  An entry point that allocates scratch opens a
+// Frame on its stream; while the frame is open, Scratch::alloc carves
+// consecutive pieces of that stream's persistent workspace (chunks that grow
+// on first use and are kept), and the next call on the same stream starts
+// again at the beginning: the stream orders the previous call's kernels
+// before the next call's, so no allocation or free happens per call.  Calls
+// that overlap on one stream from several host threads are not supported.
+int workspace_alloc(size_t bytes, void **out);
+int release_workspaces();
+struct Frame {
+  bool outer = false;
+  explicit Frame(hipStream_t s);
+  ~Frame();
+  Frame(const Frame &) = delete;
+  Frame &operator=(const Frame &) = delete;
+};
+
+// Scratch buffer: a piece of the graph-capture arena when one is installed,
+// else a piece of the current frame's workspace, else a stream-ordered
+// allocation from the device's default memory pool.
 struct Scratch {
   void *ptr = nullptr;
   hipStream_t stream = nullptr;
@@ -91,6 +108,16 @@ struct Scratch {
       return TC_OK;
     }
     if (ar) ar->total += arena_round(bytes);
+    if (!ar) {
+      void *w = nullptr;
+      const int rc = workspace_alloc(bytes, &w);
+      if (rc != TC_OK) return rc;
+      if (w) {
+        ptr = w;
+        owned = false;
+        return TC_OK;
+      }
+    }
     owned = true;
     hipError_t e = hipMallocAsync(&ptr, bytes, s);
     if (e != hipSuccess) {

@@ -346,6 +346,7 @@ extern "C" void tc_csr_build_free(tc_csr_build *b) { delete b; }
 extern "C" int tc_csr_build_begin(const int32_t *pairs, int64_t m, void *stream,
                                   tc_csr_build **out, int64_t *n_nodes, int64_t *n_edges) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(out && n_nodes && n_edges, TC_ERR_INVALID, "null output pointer");
   TC_CHECK(m >= 0 && (m == 0 || pairs), TC_ERR_INVALID, "invalid pair array (m=%lld)",
            (long long)m);
@@ -493,6 +494,7 @@ extern "C" int tc_csr_build_finish(tc_csr_build *b, void *row_ptr, int row_ptr_6
                                    int32_t *edge_u, int32_t *edge_v, int32_t *undirected_deg,
                                    void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(b, TC_ERR_INVALID, "null CSR build state");
   TC_CHECK(b->E == 0 || col_idx, TC_ERR_INVALID, "col_idx is required");
   TC_CHECK(row_ptr_64 || b->E <= INT32_MAX, TC_ERR_INVALID,
@@ -539,6 +541,7 @@ __global__ void k_degree_keys(const I *__restrict__ rp, int64_t n, uint8_t *__re
 extern "C" int tc_degree_buckets(const void *row_ptr, int row_ptr_64, int64_t n_nodes,
                                  int64_t *bucket_start, int32_t *perm, void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(row_ptr && bucket_start && perm && n_nodes >= 0, TC_ERR_INVALID,
            "invalid arguments to tc_degree_buckets");
   TC_TRY(ensure_init());

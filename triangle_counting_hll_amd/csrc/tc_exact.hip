@@ -624,11 +624,24 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
       acc += tri;
     }
   }
+  // one atomic per workgroup and counter: same-address atomics from every
+  // wavefront of a large grid serialise
+  __shared__ unsigned long long s_red[2][kBlock / kWave];
   nh = wave_sum(nh);
-  if (lane_id() == 0 && nh) atomicAdd(n_heavy, (unsigned long long)nh);
-  if (!graph_of) {
-    acc = wave_sum(acc);
-    if (lane_id() == 0 && acc) atomicAdd(count, (unsigned long long)acc);
+  acc = wave_sum(acc);
+  if (lane == 0) {
+    s_red[0][threadIdx.x / kWave] = nh;
+    s_red[1][threadIdx.x / kWave] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0, b = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      a += s_red[0][w];
+      b += s_red[1][w];
+    }
+    if (a) atomicAdd(n_heavy, a);
+    if (!graph_of && b) atomicAdd(count, b);
   }
 }
 
@@ -724,7 +737,8 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     Rows<I> R{(const I *)row_ptr};
     {
       StageTimer _timer(TC_STAGE_CLASSIFY, s);
-      hipLaunchKernelGGL(k_classify_pack<I>, dim3(ge), dim3(kBlock), 0, s, R, col, eu, ev, E,
+      hipLaunchKernelGGL(k_classify_pack<I>, dim3(grid_for(E, kBlock, num_cus() * 8)),
+                         dim3(kBlock), 0, s, R, col, eu, ev, E,
                          skip, light_max, (uint64_t)n << 32, packed.as<uint64_t>(), graph_of,
                          count, n_heavy);
     }
@@ -891,6 +905,7 @@ extern "C" int tc_exact_count(const void *row_ptr, int row_ptr_64, const int32_t
                               int64_t n_edges, uint64_t *count, const tc_exact_params *params,
                               void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(row_ptr && count, TC_ERR_INVALID, "row_ptr and count are required");
   TC_CHECK(n_nodes >= 0 && n_edges >= 0, TC_ERR_INVALID, "negative size");
   TC_CHECK((edge_u == nullptr) == (edge_v == nullptr), TC_ERR_INVALID,
@@ -908,6 +923,7 @@ extern "C" int tc_edge_supports(const void *row_ptr, int row_ptr_64, const int32
                                 const int32_t *edge_u, const int32_t *edge_v, int64_t n_edges,
                                 int32_t *out, void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(row_ptr && out && n_edges >= 0, TC_ERR_INVALID, "invalid arguments");
   TC_CHECK(n_edges == 0 || (col_idx && edge_u && edge_v), TC_ERR_INVALID,
            "col_idx, edge_u and edge_v are required");

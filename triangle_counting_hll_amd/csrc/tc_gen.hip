@@ -65,6 +65,7 @@ extern "C" int tc_generate_pairs(const uint8_t *kind, const int32_t *param, cons
                                  uint32_t t_a, uint32_t t_ab, uint32_t t_abc, int32_t *pairs,
                                  int64_t n_pairs, void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(n_graphs > 0 && kind && param && id_off && pair_off && key && n_pairs >= 0 &&
                (n_pairs == 0 || pairs),
            TC_ERR_INVALID, "invalid arguments to tc_generate_pairs");

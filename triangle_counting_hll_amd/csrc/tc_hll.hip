@@ -441,6 +441,7 @@ extern "C" int tc_hll_build(const void *row_ptr, int row_ptr_64, const int32_t *
                             const int32_t *slot, uint8_t *has_sketch, int accumulate,
                             int hash_mode, uint64_t seed, void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(p >= 4 && p <= 16, TC_ERR_INVALID, "p must be in [4, 16], got %d", p);
   TC_CHECK(hash_mode == TC_HLL_GPU || hash_mode == TC_HLL_CPU_EXP, TC_ERR_INVALID,
            "unknown hash mode %d", hash_mode);
@@ -476,6 +477,7 @@ extern "C" int tc_union_estimate(const uint8_t *regs, const int32_t *slot, int p
                                  const int32_t *us, const int32_t *vs, int64_t k, double *out,
                                  int hash_mode, void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(p >= 4 && p <= 16, TC_ERR_INVALID, "p must be in [4, 16], got %d", p);
   TC_CHECK(hash_mode == TC_HLL_GPU || hash_mode == TC_HLL_CPU_EXP, TC_ERR_INVALID,
            "unknown hash mode %d", hash_mode);
@@ -495,6 +497,7 @@ extern "C" int tc_router_features(const int32_t *degrees, const int32_t *edge_u,
                                   const uint8_t *regs, const int32_t *slot, int p, int sample_bits,
                                   const float *card, float *feats, void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(p >= 4 && p <= 16 && sample_bits >= 4 && sample_bits <= p, TC_ERR_INVALID,
            "need 4 <= sample_bits <= p <= 16 (got p=%d, sample_bits=%d)", p, sample_bits);
   TC_CHECK(n_cand >= 0 && (n_cand == 0 || (degrees && edge_u && edge_v && cand && regs && card &&
@@ -518,6 +521,7 @@ extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_
                                int64_t *fallback, double *per_edge, int hash_mode,
                                void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(hash_mode == TC_HLL_GPU || hash_mode == TC_HLL_CPU_EXP, TC_ERR_INVALID,
            "unknown hash mode %d", hash_mode);
   TC_CHECK(part == TC_PART_BOTH || part == TC_PART_EXACT || part == TC_PART_HLL, TC_ERR_INVALID,

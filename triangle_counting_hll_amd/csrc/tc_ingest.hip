@@ -270,6 +270,7 @@ extern "C" int tc_parse_edge_list(const uint8_t *text, int64_t n_bytes, const ch
                                   int32_t *pairs, int64_t max_pairs, int64_t *n_pairs,
                                   int64_t *bad_offset, int *bad_kind, void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(n_pairs, TC_ERR_INVALID, "n_pairs is required");
   TC_CHECK(n_bytes >= 0 && max_pairs >= 0, TC_ERR_INVALID, "negative size");
   TC_CHECK(n_bytes == 0 || text, TC_ERR_INVALID, "text is required");
@@ -397,6 +398,7 @@ extern "C" int tc_format_edge_list(const int32_t *pairs, int64_t m, const char *
                                    const char *newline, uint8_t *text, int64_t max_bytes,
                                    int64_t *n_bytes, void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(n_bytes, TC_ERR_INVALID, "n_bytes is required");
   TC_CHECK(m >= 0 && max_bytes >= 0, TC_ERR_INVALID, "negative size");
   TC_CHECK(m == 0 || pairs, TC_ERR_INVALID, "pairs is required");

@@ -233,6 +233,7 @@ extern "C" int tc_mlp_grad(const float *params, const float *norm, const float *
                            const float *target, int64_t n, float pos_weight, float *grad,
                            float *loss, void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(params && norm && grad && n > 0 && feats && target, TC_ERR_INVALID,
            "invalid arguments to tc_mlp_grad");
   TC_TRY(ensure_init());

@@ -118,6 +118,7 @@ extern "C" int tc_edge_subgraph(const void *row_ptr, int row_ptr_64, const int32
                                 int64_t *out_row_ptr, int32_t *out_col, int64_t *n_kept,
                                 void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(row_ptr && out_row_ptr && n_kept, TC_ERR_INVALID,
            "row_ptr, out_row_ptr and n_kept are required");
   TC_CHECK(n_nodes >= 0 && n_edges >= 0, TC_ERR_INVALID, "negative size");

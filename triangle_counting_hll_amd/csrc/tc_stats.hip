@@ -136,6 +136,7 @@ static int read_flag(unsigned int *d_flag, hipStream_t s, unsigned int *out) {
 extern "C" int tc_degree_count(const int32_t *src, const int32_t *dst, int64_t stride,
                                int64_t n_edges, int32_t *degree, int64_t n_slots, void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(n_edges >= 0 && n_slots >= 0 && stride >= 1, TC_ERR_INVALID, "bad size or stride");
   TC_CHECK(n_edges == 0 || (src && dst && degree), TC_ERR_INVALID,
            "src, dst and degree are required");
@@ -174,6 +175,7 @@ extern "C" int tc_degree_histogram(const int32_t *degree, int64_t n_slots, int64
                                    int64_t n_hist, int64_t *log_hist, int n_log_buckets,
                                    void *stream) {
   using namespace tc;
+  Frame _frame((hipStream_t)stream);
   TC_CHECK(n_slots >= 0 && n_hist >= 1, TC_ERR_INVALID, "bad size");
   TC_CHECK(n_log_buckets >= 1 && n_log_buckets <= kMaxLogBuckets, TC_ERR_INVALID,
            "n_log_buckets must be in [1, %d]", kMaxLogBuckets);
