@@ -350,6 +350,29 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
   return acc;
 }
 
+// Per-graph counts of a batch: a wavefront keeps per-lane partial sums for
+// the graph of its current work item and adds them to that graph's counter
+// (one wave reduction, one atomic) only when the graph changes and at the end.
+// Consecutive items mostly belong to one graph (graphs own contiguous ids).
+struct GraphAcc {
+  int g = -1;
+  uint64_t acc = 0;
+  __device__ __forceinline__ void flush(unsigned long long *__restrict__ count) {
+    const uint64_t t = wave_sum(acc);
+    if (lane_id() == 0 && t && g >= 0) atomicAdd(&count[g], (unsigned long long)t);
+    acc = 0;
+  }
+  // wave-uniform call: every lane passes the same graph
+  __device__ __forceinline__ void add(int graph, uint64_t lane_acc,
+                                      unsigned long long *__restrict__ count) {
+    if (graph != g) {
+      flush(count);
+      g = graph;
+    }
+    acc += lane_acc;
+  }
+};
+
 template <class I, int U, class O>
 __global__ void __launch_bounds__(kBlock)
 k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
@@ -378,6 +401,7 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
   const int64_t first = iptr[*n_first_runs_p];
   const int64_t n_big = iptr[*n_big_centers_p];
   uint64_t acc_total = 0;
+  GraphAcc gacc;
 
   // Workgroup phase.  Every thread sees the same ticket, so the loop exits
   // together; the barrier at its top keeps the table and the ticket in place
@@ -403,12 +427,10 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
       table_build<kBlkTblInts, true>(tbl_blk, col + cb, dc, shift, mask);
       uint64_t acc = stream_others<I, true, U>(R, col, others, wb, we, seg_end, seg_base,
                                                &tbl_all[0][0], shift, mask, cb, dc);
-      if (graph_of) {
-        acc = wave_sum(acc);
-        if (lane == 0 && acc) atomicAdd(&count[graph_of[c]], (unsigned long long)acc);
-      } else {
+      if (graph_of)
+        gacc.add(graph_of[c], acc, count);
+      else
         acc_total += acc;
-      }
     }
   }
 
@@ -445,14 +467,14 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
       acc = stream_others<I, false, U>(R, col, others, ob, oe, seg_end, seg_base, tbl, 0, 0, cb,
                                        dc);
     }
-    if (graph_of) {
-      acc = wave_sum(acc);
-      if (lane == 0 && acc) atomicAdd(&count[graph_of[c]], (unsigned long long)acc);
-    } else {
+    if (graph_of)
+      gacc.add(graph_of[c], acc, count);
+    else
       acc_total += acc;
-    }
   }
-  if (!graph_of) {
+  if (graph_of) {
+    gacc.flush(count);
+  } else {
     acc_total = wave_sum(acc_total);
     if (lane == 0 && acc_total) atomicAdd(count, (unsigned long long)acc_total);
   }
@@ -506,6 +528,7 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restric
   const int64_t n_first = iptr[*n_first_runs_p];
   int *tbl_i = reinterpret_cast<int *>(tbl);
   uint64_t acc_total = 0;
+  GraphAcc gacc;
   while (true) {
     if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(next_item, 1ull);
     __syncthreads();
@@ -532,14 +555,14 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restric
       acc = stream_others<I, false, U>(R, col, others, wb, we, seg_end_all[w], seg_base_all[w],
                                        tbl, 0, 0, cb, dc);
     }
-    if (graph_of) {
-      acc = wave_sum(acc);
-      if (lane == 0 && acc) atomicAdd(&count[graph_of[c]], (unsigned long long)acc);
-    } else {
+    if (graph_of)
+      gacc.add(graph_of[c], acc, count);
+    else
       acc_total += acc;
-    }
   }
-  if (!graph_of) {
+  if (graph_of) {
+    gacc.flush(count);
+  } else {
     acc_total = wave_sum(acc_total);
     if (lane == 0 && acc_total) atomicAdd(count, (unsigned long long)acc_total);
   }
