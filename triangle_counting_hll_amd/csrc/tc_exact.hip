@@ -760,8 +760,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     Rows<I> R{(const I *)row_ptr};
     {
       StageTimer _timer(TC_STAGE_CLASSIFY, s);
-      hipLaunchKernelGGL(k_classify_pack<I>, dim3(grid_for(E, kBlock, num_cus() * 8)),
-                         dim3(kBlock), 0, s, R, col, eu, ev, E,
+      hipLaunchKernelGGL(k_classify_pack<I>, dim3(ge), dim3(kBlock), 0, s, R, col, eu, ev, E,
                          skip, light_max, (uint64_t)n << 32, packed.as<uint64_t>(), graph_of,
                          count, n_heavy);
     }
