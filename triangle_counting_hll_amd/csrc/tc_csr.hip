@@ -5,8 +5,7 @@
 //   radix sort -> adjacent-unique
 //   -> presence flags + scan = dense relabel by sorted unique endpoint
 //   -> undirected degrees -> orientation -> forward degrees -> scan = row_ptr
-//   -> scatter into rows -> per-row sort (wave bitonic / LDS bitonic / global
-//   bitonic by row length).
+//   -> oriented (src, dst) keys radix-sorted: the rows in order, each sorted.
 // The output equals the lexsort-based loader of experiment/exp_gpu.py:301-365.
 #include <algorithm>
 #include <climits>
@@ -126,123 +125,28 @@ __global__ void k_orient(uint64_t *__restrict__ uniq, int64_t nu, const int *__r
   }
 }
 
-__global__ void k_scatter_rows(const uint64_t *__restrict__ keys, int64_t nu,
-                               const int64_t *__restrict__ row_ptr, int *__restrict__ cursor,
-                               int32_t *__restrict__ col, int32_t *__restrict__ edge_u) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nu;
+// Oriented keys (s << 32 | d) repacked as (s << B | d) for a radix sort on
+// 2B bits (dense ids are below 2^B).
+__global__ void k_repack_oriented(uint64_t *__restrict__ keys, int64_t E, int B) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
        i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t k = keys[i];
-    uint32_t s = (uint32_t)(k >> 32), d = (uint32_t)(k & 0xffffffffu);
-    int64_t pos = row_ptr[s] + atomicAdd(&cursor[s], 1);
-    col[pos] = (int32_t)d;
-    if (edge_u) edge_u[pos] = (int32_t)s;
+    const uint64_t k = keys[i];
+    keys[i] = ((k >> 32) << B) | (k & 0xffffffffu);
   }
 }
 
-// Rows of length <= 64: one wavefront per row, bitonic sort across lanes.
-__global__ void k_sort_rows_wave(const int64_t *__restrict__ row_ptr, int64_t n,
-                                 int32_t *__restrict__ col) {
-  const int lane = lane_id();
-  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
-  for (int64_t r = wid; r < n; r += nw) {
-    const int64_t b = row_ptr[r], len = row_ptr[r + 1] - b;
-    if (len <= 1 || len > kWave) continue;
-    int v = lane < len ? col[b + lane] : INT_MAX;
-#pragma unroll
-    for (int k = 2; k <= kWave; k <<= 1) {
-#pragma unroll
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        int o = __shfl_xor(v, j);
-        bool up = ((lane & k) == 0);
-        bool lower = ((lane & j) == 0);
-        // keep min on the lower lane of an ascending pair, max on the upper
-        int lo = min(v, o), hi = max(v, o);
-        v = (lower == up) ? lo : hi;
-      }
-    }
-    if (lane < len) col[b + lane] = v;
-  }
-}
-
-// Rows of length in (64, kLdsSortMax]: one workgroup per row, bitonic in LDS.
-constexpr int kLdsSortMax = 8192;
-
-__global__ void __launch_bounds__(kBlock)
-k_sort_rows_lds(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ rows,
-                int64_t nrows, int32_t *__restrict__ col) {
-  __shared__ int buf[kLdsSortMax];
-  for (int64_t ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
-    const int64_t r = rows[ri];
-    const int64_t b = row_ptr[r];
-    const int len = (int)(row_ptr[r + 1] - b);
-    int P = 1;
-    while (P < len) P <<= 1;
-    for (int i = threadIdx.x; i < P; i += blockDim.x) buf[i] = i < len ? col[b + i] : INT_MAX;
-    __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int i = threadIdx.x; i < P; i += blockDim.x) {
-          int ix = i ^ j;
-          if (ix > i) {
-            int a = buf[i], c = buf[ix];
-            bool up = ((i & k) == 0);
-            if ((a > c) == up) {
-              buf[i] = c;
-              buf[ix] = a;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (int i = threadIdx.x; i < len; i += blockDim.x) col[b + i] = buf[i];
-    __syncthreads();
-  }
-}
-
-// Rows longer than kLdsSortMax: one workgroup per row, bitonic in a global
-// scratch buffer of P (power of two >= len) ints per row slot.
-__global__ void __launch_bounds__(kBlock)
-k_sort_rows_global(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ rows,
-                   int64_t nrows, int32_t *__restrict__ col, int *__restrict__ scratch,
-                   int64_t scratch_stride) {
-  int *buf = scratch + (int64_t)blockIdx.x * scratch_stride;
-  for (int64_t ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
-    const int64_t r = rows[ri];
-    const int64_t b = row_ptr[r];
-    const int64_t len = row_ptr[r + 1] - b;
-    int64_t P = 1;
-    while (P < len) P <<= 1;
-    for (int64_t i = threadIdx.x; i < P; i += blockDim.x) buf[i] = i < len ? col[b + i] : INT_MAX;
-    __syncthreads();
-    for (int64_t k = 2; k <= P; k <<= 1) {
-      for (int64_t j = k >> 1; j > 0; j >>= 1) {
-        for (int64_t i = threadIdx.x; i < P; i += blockDim.x) {
-          int64_t ix = i ^ j;
-          if (ix > i) {
-            int a = buf[i], c = buf[ix];
-            bool up = ((i & k) == 0);
-            if ((a > c) == up) {
-              buf[i] = c;
-              buf[ix] = a;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (int64_t i = threadIdx.x; i < len; i += blockDim.x) col[b + i] = buf[i];
-    __syncthreads();
-  }
-}
-
-__global__ void k_row_class(const int64_t *__restrict__ row_ptr, int64_t n, int64_t lo,
-                            int64_t hi, uint8_t *__restrict__ flag) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+// Sorted (s << B | d) keys are the CSR in order: col = d, edge_u = s.
+__global__ void k_unpack_rows(const uint64_t *__restrict__ keys, int64_t E, int B,
+                              int32_t *__restrict__ col, int32_t *__restrict__ edge_u,
+                              int32_t *__restrict__ edge_v) {
+  const uint64_t mask = (1ull << B) - 1;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int64_t len = row_ptr[i + 1] - row_ptr[i];
-    flag[i] = (len > lo && len <= hi) ? 1 : 0;
+    const uint64_t k = keys[i];
+    const int32_t d = (int32_t)(k & mask);
+    col[i] = d;
+    if (edge_u) edge_u[i] = (int32_t)(k >> B);
+    if (edge_v) edge_v[i] = d;
   }
 }
 
@@ -266,61 +170,6 @@ __global__ void k_finish_arrays(const int64_t *__restrict__ row_ptr64, int64_t n
       if (udeg_out) udeg_out[i] = udeg[i];
     }
   }
-}
-
-int sort_rows(const int64_t *row_ptr, int64_t n, int32_t *col, hipStream_t s) {
-  if (n <= 0) return TC_OK;
-  const unsigned g = grid_for(n * kWave, kBlock, 65536);
-  hipLaunchKernelGGL(k_sort_rows_wave, dim3(g), dim3(kBlock), 0, s, row_ptr, n, col);
-  TC_HIP(hipGetLastError());
-
-  Scratch flag, rows, cnt;
-  TC_TRY(flag.alloc(n, s));
-  TC_TRY(rows.alloc(sizeof(int32_t) * n, s));
-  TC_TRY(cnt.alloc(sizeof(int64_t), s));
-  int64_t h_cnt = 0;
-
-  // LDS tier
-  hipLaunchKernelGGL(k_row_class, dim3(grid_for(n)), dim3(kBlock), 0, s, row_ptr, n,
-                     (int64_t)kWave, (int64_t)kLdsSortMax, flag.as<uint8_t>());
-  TC_HIP(hipGetLastError());
-  TC_TRY(compact_indices(flag.as<uint8_t>(), n, rows.as<int32_t>(), cnt.as<int64_t>(), s));
-  TC_HIP(hipMemcpyAsync(&h_cnt, cnt.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  TC_HIP(hipStreamSynchronize(s));
-  if (h_cnt > 0) {
-    hipLaunchKernelGGL(k_sort_rows_lds, dim3((unsigned)std::min<int64_t>(h_cnt, 65536)),
-                       dim3(kBlock), 0, s, row_ptr, (const int32_t *)rows.as<int32_t>(), h_cnt,
-                       col);
-    TC_HIP(hipGetLastError());
-  }
-
-  // global tier
-  hipLaunchKernelGGL(k_row_class, dim3(grid_for(n)), dim3(kBlock), 0, s, row_ptr, n,
-                     (int64_t)kLdsSortMax, (int64_t)INT64_MAX, flag.as<uint8_t>());
-  TC_HIP(hipGetLastError());
-  TC_TRY(compact_indices(flag.as<uint8_t>(), n, rows.as<int32_t>(), cnt.as<int64_t>(), s));
-  TC_HIP(hipMemcpyAsync(&h_cnt, cnt.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  TC_HIP(hipStreamSynchronize(s));
-  if (h_cnt > 0) {
-    int64_t maxlen = 0;
-    // longest row bounds the scratch stride: read row lengths of the class on host
-    std::vector<int32_t> ids(h_cnt);
-    TC_HIP(hipMemcpy(ids.data(), rows.ptr, sizeof(int32_t) * h_cnt, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < h_cnt; ++i) {
-      int64_t be[2];
-      TC_HIP(hipMemcpy(be, row_ptr + ids[i], 2 * sizeof(int64_t), hipMemcpyDeviceToHost));
-      maxlen = std::max(maxlen, be[1] - be[0]);
-    }
-    int64_t P = 1;
-    while (P < maxlen) P <<= 1;
-    const int64_t blocks = std::min<int64_t>(h_cnt, 64);
-    Scratch buf;
-    TC_TRY(buf.alloc(sizeof(int) * P * blocks, s));
-    hipLaunchKernelGGL(k_sort_rows_global, dim3((unsigned)blocks), dim3(kBlock), 0, s, row_ptr,
-                       (const int32_t *)rows.as<int32_t>(), h_cnt, col, buf.as<int>(), P);
-    TC_HIP(hipGetLastError());
-  }
-  return TC_OK;
 }
 
 }  // namespace tc
@@ -502,16 +351,19 @@ extern "C" int tc_csr_build_finish(tc_csr_build *b, void *row_ptr, int row_ptr_6
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = b->n, E = b->E;
   if (E > 0) {
-    Scratch cursor;
-    TC_TRY(cursor.alloc(sizeof(int) * std::max<int64_t>(n, 1), s));
-    TC_HIP(hipMemsetAsync(cursor.ptr, 0, sizeof(int) * std::max<int64_t>(n, 1), s));
-    hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(E, kBlock, 8192)), dim3(kBlock), 0, s,
-                       (const uint64_t *)b->keys, E, (const int64_t *)b->row_ptr64,
-                       cursor.as<int>(), col_idx, edge_u);
+    // rows in order and sorted inside: one radix sort of the oriented keys
+    int B = 1;
+    while ((int64_t(1) << B) < n) ++B;
+    Scratch tmp;
+    TC_TRY(tmp.alloc(sizeof(uint64_t) * E, s));
+    hipLaunchKernelGGL(k_repack_oriented, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock), 0, s,
+                       b->keys, E, B);
     TC_HIP(hipGetLastError());
-    TC_TRY(sort_rows(b->row_ptr64, n, col_idx, s));
-    if (edge_v) TC_HIP(hipMemcpyAsync(edge_v, col_idx, sizeof(int32_t) * E,
-                                      hipMemcpyDeviceToDevice, s));
+    uint64_t *sorted = nullptr;
+    TC_TRY(radix_sort_u64(b->keys, tmp.as<uint64_t>(), E, 0, 2 * B, &sorted, s));
+    hipLaunchKernelGGL(k_unpack_rows, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock), 0, s,
+                       (const uint64_t *)sorted, E, B, col_idx, edge_u, edge_v);
+    TC_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_finish_arrays, dim3(grid_for(n + 1)), dim3(kBlock), 0, s,
                      (const int64_t *)b->row_ptr64, n, (const int32_t *)b->node_list,
