@@ -230,10 +230,14 @@ k_tile_scan(const T *__restrict__ in, int64_t n, const int64_t *__restrict__ off
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanBlock - 1) out[n] = run;
 }
 
+__global__ void k_zero_i64(int64_t *p) { *p = 0; }
+
 template <class T>
 int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t * /*unused*/, hipStream_t s) {
   if (n <= 0) {
-    TC_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), s));
+    // a kernel, not a memset node, so that captured graphs replay it correctly
+    hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(1), 0, s, out);
+    TC_HIP(hipGetLastError());
     return TC_OK;
   }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
