@@ -45,28 +45,42 @@ def exact_bytes(row_ptr: np.ndarray, col: np.ndarray, eu: np.ndarray, ev: np.nda
     return int(per.sum()), int(per[heavy].sum()), int(heavy.sum())
 
 
-def cpu_baseline(rp, col, truth_hint=None, budget_s=20.0):
-    """Time the C oracle (cpu_exp exact policy, OpenMP) on a bounded sample."""
+def cpu_baseline(rp, col, budget_s=20.0):
+    """Time the C oracle (the cpu_exp exact policy, OpenMP) beside the GPU run.
+
+    All host cores: one warm-up on a 1/64 slice (which also sizes the work),
+    then the median of up to three runs over the full graph when one run fits
+    the budget (else over every k-th edge); one thread: a 1/64 slice.
+    """
     from oracle import cbind
 
     threads = len(os.sched_getaffinity(0))
     E = len(col)
     deg = np.diff(rp)
     eu_all = np.repeat(np.arange(len(deg), dtype=np.int32), deg)
-    # probe on a 1/64 slice of the edges to size the sample
     step = 64
     probe = slice(0, E, step)
+    n_probe = len(range(0, E, step))
     t0 = time.perf_counter()
     cbind.exact_count(rp, col, eu_all[probe], col[probe], policy=0, nthreads=threads)
-    t_probe = time.perf_counter() - t0
-    est_full = t_probe * step
+    est_full = (time.perf_counter() - t0) * step
+    t0 = time.perf_counter()
+    cbind.exact_count(rp, col, eu_all[probe], col[probe], policy=0, nthreads=1)
+    t1 = time.perf_counter() - t0
+    single = {"value": n_probe / t1, "unit": "oriented edges/s", "cores": 1, "seconds": t1,
+              "sample": f"every {step}th oriented edge ({n_probe} of {E})"}
+    reps = max(1, min(3, int(budget_s // max(est_full, 1e-9))))
     if est_full <= budget_s:
-        t0 = time.perf_counter()
-        c = cbind.exact_count(rp, col, None, None, policy=0, nthreads=threads)
-        dt = time.perf_counter() - t0
+        times, c = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            c = cbind.exact_count(rp, col, None, None, policy=0, nthreads=threads)
+            times.append(time.perf_counter() - t0)
+        dt = float(np.median(times))
         return {"value": E / dt, "unit": "oriented edges/s", "cores": threads,
-                "kind": "port", "seconds": dt, "count": int(c),
-                "sample": f"all {E} oriented edges (full graph)"}
+                "kind": "port", "seconds": dt, "runs": len(times), "count": int(c),
+                "sample": f"all {E} oriented edges (full graph), median of {len(times)}",
+                "single_thread": single}
     stride = int(np.ceil(est_full / budget_s))
     sel = slice(0, E, stride)
     n_s = len(range(0, E, stride))
@@ -74,8 +88,27 @@ def cpu_baseline(rp, col, truth_hint=None, budget_s=20.0):
     cbind.exact_count(rp, col, eu_all[sel], col[sel], policy=0, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": n_s / dt, "unit": "oriented edges/s", "cores": threads, "kind": "port",
-            "seconds": dt, "count": None,
-            "sample": f"every {stride}th oriented edge ({n_s} of {E}), u-major order"}
+            "seconds": dt, "runs": 1, "count": None,
+            "sample": f"every {stride}th oriented edge ({n_s} of {E}), u-major order",
+            "single_thread": single}
+
+
+def copy_bandwidth_gbs(gib: int = 4, reps: int = 5):
+    """Measured device-to-device copy bandwidth (read + write bytes / time)."""
+    import torch
+
+    src = torch.empty((gib << 30) // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    return 2 * (gib << 30) / (ms * 1e-3) / 1e9
 
 
 TRAFFIC_PROFILES = {"chunglu": "profiles/exact_chunglu117M_hbm_traffic.md",
@@ -452,6 +485,8 @@ def main():
     traffic, l2_hit = (measured_traffic(args.workload) if not args.scale_down
                        else (None, None))
 
+    copy_gbs = copy_bandwidth_gbs()
+    _log(f"device copy bandwidth {copy_gbs:.0f} GB/s")
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -515,6 +550,7 @@ def main():
             "avg_launch_ms": heavy_avg_ms,
             "step_bytes_all_edges": B_all,
             "step_frac_of_peak": B_all / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "copy_gbs_measured": copy_gbs,
         },
         "stage_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in stages.items() if v[1]},
         "csr_build_ms": t_csr * 1e3,
