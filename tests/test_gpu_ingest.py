@@ -136,3 +136,37 @@ def test_writer_rejects_negative_ids(device):
     with pytest.raises(ValueError, match="negative"):
         ingest.format_edge_list_gpu(np.array([[1, 2], [-1, 3]], dtype=np.int32))
     assert ingest.format_edge_list_gpu(np.zeros((0, 2), np.int32)).numel() == 0
+
+
+def test_self_loops_and_error_precedence(device):
+    assert _parse(b"2147483648 2147483648\n-7 -7\n-0 0\n1 2\n").tolist() == [[1, 2]]
+    with pytest.raises(ValueError, match="negative"):
+        ingest.parse_edge_list_gpu(b"1 2\n2147483648 -5\n")
+    with pytest.raises(ValueError, match="expected integer"):
+        ingest.parse_edge_list_gpu(b"1 2\n-5 x\n")
+
+
+def _oracle_first_error(lines):
+    for k, line in enumerate(lines):
+        try:
+            O.parse_edge_list(line.encode() + b"\n")
+        except (ValueError, OverflowError) as e:
+            return k + 1, type(e)
+    return None, None
+
+
+def test_random_lines_match_oracle(device):
+    # random lines over the tokens that matter to the rules (signs, '_',
+    # separators, comments, 10^18-free ids around the int32 limit)
+    alpha = ["0", "1", "7", "9", "+", "-", "_", " ", "\t", "#", "%", "x", ".", "\x0b", "\x1c",
+             "2147483647", "2147483648", "00", "-0", "12", "99999999999"]
+    rng = np.random.default_rng(17)
+    for trial in range(40):
+        lines = ["".join(rng.choice(alpha, size=rng.integers(0, 9))) for _ in range(300)]
+        text = "\n".join(lines).encode()
+        line_no, exc = _oracle_first_error(lines)
+        if exc is None:
+            assert np.array_equal(_parse(text), O.parse_edge_list(text)), trial
+        else:
+            with pytest.raises(exc, match=f"line {line_no}:"):
+                ingest.parse_edge_list_gpu(text)

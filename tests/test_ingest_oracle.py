@@ -71,3 +71,12 @@ def test_line_rules_hand_derived():
 def test_bad_lines_raise(text, exc):
     with pytest.raises(exc):
         O.parse_edge_list(b"1 2\n" + text)
+
+
+def test_self_loops_and_error_precedence():
+    # u == v drops the line before any range check (exp_gpu.py:141-142)
+    assert O.parse_edge_list(b"2147483648 2147483648\n-7 -7\n-0 0\n").shape == (0, 2)
+    with pytest.raises(ValueError, match="negative"):
+        O.parse_edge_list(b"2147483648 -5\n")  # negative before too large
+    with pytest.raises(ValueError, match="invalid literal"):
+        O.parse_edge_list(b"-5 x\n")  # int() of both fields first

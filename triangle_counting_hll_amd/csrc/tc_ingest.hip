@@ -62,12 +62,11 @@ __device__ __forceinline__ bool is_space(int c) {
   return c == ' ' || c == '\t' || c == 0x0b || c == 0x0c || (c >= 0x1c && c <= 0x1f);
 }
 
-// Byte sources for the line parser, indexed by an Index (int for the LDS
-// stage, int64 text offset for global memory).  A source returns the byte,
-// -1 past the end of the text, or -2 where it cannot see (the LDS source past
-// its staged bytes); a line that reaches -2 is parsed again from global
-// memory.  Staged bytes past the end of the text read as '\n', which ends a
-// line exactly as the end of the text does.
+// Byte sources for the line parser.  A source returns the byte at an index,
+// '\n' past the end of the text (the end of the text ends a line as a break
+// does), or -2 where it cannot see: the LDS stage past its staged bytes, after
+// which the line is parsed again from global memory.  Staged bytes past the
+// end of the text were filled with '\n'.
 struct LdsBytes {
   const uint8_t *stage;  // the __shared__ stage
   __device__ __forceinline__ int operator()(int i) const {
@@ -78,66 +77,82 @@ struct GlobalBytes {
   const uint8_t *text;
   int64_t n;
   __device__ __forceinline__ int operator()(int64_t p) const {
-    return p < n ? (int)text[p] : -1;
+    return p < n ? (int)text[p] : '\n';
   }
 };
 
 enum : int { kLineOffStage = 7 };
 
-// One field starting at p (a byte that is neither space nor break); p ends on
-// the byte after the field.
-template <class Src, class Index>
-__device__ __forceinline__ int parse_field(const Src &at, Index &p, int32_t &out) {
-  int c = at(p);
-  bool neg = false;
-  if (c == '+' || c == '-') {
-    neg = (c == '-');
-    c = at(++p);
-  }
-  int64_t v = 0;
-  bool digits = false, ok = true, big = false, under = false;
-  while (c >= 0 && !is_break(c) && !is_space(c)) {
-    if (c >= '0' && c <= '9') {
-      if (!big) {
-        v = v * 10 + (c - '0');
-        big = v > 0x7fffffffLL;
-      }
-      digits = true;
-      under = false;
-    } else if (c == '_' && digits && !under) {
-      under = true;
-    } else {
-      ok = false;
-    }
-    c = at(++p);
-  }
-  if (c == -2) return kLineOffStage;
-  if (!digits || under || !ok) return kLineBadInt;
-  if (neg && (v != 0 || big)) return kLineNegative;
-  if (big) return kLineRange;
-  out = (int32_t)v;
-  return kLinePair;
-}
+// The line rules as one predicated state machine: each byte updates the phase
+// and the current field with selects, so lanes that sit in different states
+// of their own lines do not split into branches.  Phases: 0 leading space,
+// 1 first field, 2 gap, 3 second field, 4 two fields read, 5 line skipped,
+// 6 ran past the visible bytes.  Field values are exact below 10^18 (kFHuge
+// beyond), enough to tell equal ids apart for the self-loop test.
+enum : uint32_t { kFDig = 1, kFBad = 2, kFHuge = 4, kFUnder = 8, kFNeg = 16 };
 
 template <class Src, class Index>
-__device__ __forceinline__ int parse_line(const Src &at, Index p, const CommentSet &cs,
+__device__ __forceinline__ int parse_line(const Src &at, Index i, const CommentSet &cs,
                                           int32_t &u, int32_t &v) {
-  int c = at(p);
-  while (c >= 0 && is_space(c)) c = at(++p);
-  if (c == -2) return kLineOffStage;
-  if (c < 0 || is_break(c)) return kLineSkip;
-  if (cs.has(c)) return kLineSkip;
-  const int s1 = parse_field(at, p, u);
-  if (s1 == kLineOffStage) return s1;
-  c = at(p);
-  while (c >= 0 && is_space(c)) c = at(++p);
-  if (c == -2) return kLineOffStage;
-  if (c < 0 || is_break(c)) return kLineSkip;  // one field only
-  const int s2 = parse_field(at, p, v);
-  if (s2 == kLineOffStage) return s2;
-  if (s1 != kLinePair) return s1;  // int(parts[0]) is evaluated first
-  if (s2 != kLinePair) return s2;
-  return u == v ? kLineSkip : kLinePair;
+  int phase = 0;
+  uint32_t cf = 0, f0 = 0;  // flags of the current field and of the first field
+  uint64_t cv = 0, v0 = 0;
+  while (phase < 4) {
+    const int c = at(i);
+    ++i;
+    const bool off = c < 0;
+    const bool br = (c == '\n') | (c == '\r');
+    const bool sp = (c == ' ') | (c == '\t') | (c == 0x0b) | (c == 0x0c) |
+                    ((unsigned)(c - 0x1c) <= 3u);
+    const bool tok = !off & !br & !sp;
+    const bool start = tok & ((phase == 0) | (phase == 2));
+    const bool comment = tok & (phase == 0) & cs.has(c);
+    // the current field restarts with its first byte
+    cf = start ? 0u : cf;
+    cv = start ? 0ull : cv;
+    // one byte of a field: [+-] at its start, digits, single '_' between digits
+    const bool dig = (unsigned)(c - '0') <= 9u;
+    const bool sign = start & ((c == '+') | (c == '-'));
+    const bool under_ok = (c == '_') & ((cf & (kFDig | kFUnder)) == kFDig);
+    const bool take = tok & dig & !(cf & kFHuge);
+    const bool huge = take & (cv >= 100000000000000000ull);
+    cv = (take & !huge) ? cv * 10u + (uint64_t)(c - '0') : cv;
+    uint32_t nf = cf;
+    nf |= (tok & dig) ? kFDig : 0u;
+    nf |= (tok & !dig & !sign & !under_ok) ? kFBad : 0u;
+    nf |= huge ? kFHuge : 0u;
+    nf |= (start & (c == '-')) ? kFNeg : 0u;
+    nf = (tok & dig) ? (nf & ~kFUnder) : nf;
+    nf = under_ok ? (nf | kFUnder) : nf;
+    cf = tok ? nf : cf;
+    // the first field ends at the first space or break after it
+    const bool end0 = (phase == 1) & !tok;
+    f0 = end0 ? cf : f0;
+    v0 = end0 ? cv : v0;
+    int np = phase;
+    np = (phase == 0) ? (br ? 5 : (comment ? 5 : (tok ? 1 : 0))) : np;
+    np = (phase == 1) ? (br ? 5 : (sp ? 2 : 1)) : np;
+    np = (phase == 2) ? (br ? 5 : (tok ? 3 : 2)) : np;
+    np = (phase == 3) ? (tok ? 3 : 4) : np;
+    phase = off ? 6 : np;
+  }
+  if (phase == 6) return kLineOffStage;
+  if (phase == 5) return kLineSkip;
+  // int() of both fields comes first (a field it rejects is an error whatever
+  // the other holds); then u == v drops the line; then the range checks
+  const uint32_t f1 = cf;
+  const uint64_t v1 = cv;
+  auto bad = [](uint32_t f) { return !(f & kFDig) || (f & (kFUnder | kFBad)); };
+  if (bad(f0) || bad(f1)) return kLineBadInt;
+  const bool z0 = v0 == 0 && !(f0 & kFHuge), z1 = v1 == 0 && !(f1 & kFHuge);
+  const bool n0 = (f0 & kFNeg) && !z0, n1 = (f1 & kFNeg) && !z1;  // "-0" is 0
+  if (!((f0 | f1) & kFHuge) && v0 == v1 && n0 == n1) return kLineSkip;  // self-loop
+  if (n0 || n1) return kLineNegative;
+  if ((f0 & kFHuge) || (f1 & kFHuge) || v0 > 0x7fffffffull || v1 > 0x7fffffffull)
+    return kLineRange;
+  u = (int32_t)v0;
+  v = (int32_t)v1;
+  return kLinePair;
 }
 
 // Lines longer than the staged halo (rare): parsed from global memory.
@@ -226,7 +241,6 @@ k_parse_tiles(const uint8_t *__restrict__ text, int64_t n, int aligned, CommentS
   }
   __syncthreads();
 
-  const LdsBytes lds{stage};
   int64_t out = kWrite ? tile_off[blockIdx.x] : 0;
   int cnt = 0;
   for (int r0 = 0; r0 < n_lines; r0 += kBlock) {  // uniform trip count
@@ -237,7 +251,7 @@ k_parse_tiles(const uint8_t *__restrict__ text, int64_t n, int aligned, CommentS
     if (k < n_lines) {
       const int i = kPre + starts[k];
       p = t0 + (i - kPre);
-      st = parse_line(lds, i, cs, u, v);
+      st = parse_line(LdsBytes{stage}, i, cs, u, v);
       if (st == kLineOffStage) {
         const LineResult r = parse_line_global(text, n, p, cs);
         st = r.st;
