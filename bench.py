@@ -191,6 +191,32 @@ def run_hybrid_extra(csr, truth, reps=3):
     return out
 
 
+def run_prototype_extra(csr, truth):
+    """The prototype's timing table (experiment1_gpu/gpu_experiment_fixed.py:337-701)
+    on the same graph: p = 8, gammas 32..256, its own hash and estimator."""
+    import tempfile
+
+    from triangle_counting_hll_amd import gpu_hybrid, prototype
+
+    with tempfile.TemporaryDirectory() as d:
+        res = prototype.run_gpu_experiment(csr.n_nodes, csr.n_oriented, csr.row_ptr, csr.col_idx,
+                                           csr.degrees, csr.edge_u, csr.edge_v,
+                                           [32, 64, 128, 256], p=8, warmup=2, repeat=5,
+                                           out_dir=d, verbose=False)
+    if res["exact_count"] != truth:
+        raise RuntimeError("prototype exact count differs from the exact pipeline")
+    out = {"all_exact_input_ms": res["exact_time"], "all_exact_sorted_ms": res["sorted_time"],
+           "all_hll_ms": res["hll_time"],
+           "all_hll_rel_err": gpu_hybrid.relative_error(res["hll_estimate"], truth),
+           "sketch_build_s": res["sketch_build_time_s"], "decision": res["decision"]}
+    for g, h in res["hybrid_results"].items():
+        out[f"hybrid_g{g}"] = {"ms": h["total_time"], "exact_ms": h["light_time"],
+                               "hll_ms": h["heavy_time"], "cls_ms": h["cls_time"],
+                               "heavy_pct": h["heavy_pct"],
+                               "rel_err": gpu_hybrid.relative_error(h["estimate"], truth)}
+    return out
+
+
 def workload_pairs(workload: str, scale_down: int = 0):
     """Raw pairs of a named workload on the device."""
     import torch
@@ -406,7 +432,7 @@ def main():
     ap.add_argument("--unroll", type=int, default=8)
     ap.add_argument("--extras", default="all",
                     help="comma-separated extra measurements to run (hybrid_fixed_threshold, "
-                         "ingest_text, batch4096, learned_router) or 'all'")
+                         "ingest_text, batch4096, learned_router, prototype_table) or 'all'")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the hybrid and many-graph measurements added to the line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -502,7 +528,8 @@ def main():
         for name, fn in (("hybrid_fixed_threshold", lambda: run_hybrid_extra(csr, gpu_count)),
                          ("ingest_text", lambda: run_ingest_extra(args.workload, csr)),
                          ("batch4096", lambda: run_batch_extra(args)),
-                         ("learned_router", lambda: run_router_extra())):
+                         ("learned_router", lambda: run_router_extra()),
+                         ("prototype_table", lambda: run_prototype_extra(csr, gpu_count))):
             if args.extras != "all" and name not in args.extras.split(","):
                 continue
             try:

@@ -183,9 +183,17 @@ int tc_hash64(const uint64_t *x, uint64_t *out, int64_t n, void *stream);
  *                   index = low p bits, rank from the high 64 - p bits;
  *                   estimator returning LinearCounting or the 2^32 large-range
  *                   correction of the raw value (cpu_exp/triangle_hll_experiment.py:
- *                   347-398, 615-635).  `seed` is used by this mode only. */
+ *                   347-398, 615-635).  `seed` is used by this mode only.
+ *   TC_HLL_PROTO32  h = 32-bit mixer of (uint32)id ^ 0x9747B28C, index = top p
+ *                   bits, rank from the low 32 - p bits of (h << p) | 1;
+ *                   estimator with LinearCounting and no large-range step;
+ *                   in tc_hybrid_count the HLL support is est(u) + est(v) -
+ *                   est(u | v), raised to 0 when negative (counted in
+ *                   stats[2]) and not capped above (experiment1_gpu/
+ *                   gpu_experiment_fixed.py:131-166, 209-254). */
 #define TC_HLL_GPU 0
 #define TC_HLL_CPU_EXP 1
+#define TC_HLL_PROTO32 2
 
 /* Builds HLL registers of N+(u) for every u with deg+(u) >= theta and sets
  * has_sketch[u] (uint8, every u).  Row of u: regs + slot[u] * 2^p when `slot`

@@ -326,6 +326,183 @@ def hybrid_count_cpu_exp(row_ptr, col_idx, edge_u, edge_v, out_degree, regs, thr
 
 
 # ---------------------------------------------------------------------------
+# The prototype GPU experiment's HLL variant  (experiment1_gpu/
+# gpu_experiment_fixed.py:131-166, 209-254, 257-271, 397-417, 437-526;
+# experiment1_gpu/gpu_experiment.py:137-175 builds the same registers on the CPU)
+# ---------------------------------------------------------------------------
+
+MASK32 = (1 << 32) - 1
+PROTO_SEED32 = 0x9747B28C
+PROTO_MULT32 = 0x45D9F3B
+
+
+def proto_hash32_int(v: int) -> int:
+    """The prototype's 32-bit mixer of an id (gpu_experiment_fixed.py:143-149)."""
+    h = PROTO_SEED32 ^ (v & MASK32)
+    h ^= h >> 16
+    h = (h * PROTO_MULT32) & MASK32
+    h ^= h >> 16
+    h = (h * PROTO_MULT32) & MASK32
+    return h ^ (h >> 16)
+
+
+def proto_rank_loop(h: int, p: int) -> int:
+    """Rank by the prototype's bit loop (gpu_experiment_fixed.py:151-158): one
+    plus the zeros above the first set bit of ``(h << p) | 1`` scanned from
+    bit 31 - p down to bit 0."""
+    remaining = ((h << p) | 1) & MASK32
+    rho = 1
+    for bit in range(31 - p, -1, -1):
+        if remaining & (1 << bit):
+            break
+        rho += 1
+    return rho
+
+
+def proto_hash32(v: np.ndarray) -> np.ndarray:
+    """``proto_hash32_int`` over a uint32 array."""
+    h = np.asarray(v).astype(np.uint32) ^ np.uint32(PROTO_SEED32)
+    with np.errstate(over="ignore"):
+        h = h ^ (h >> np.uint32(16))
+        h = h * np.uint32(PROTO_MULT32)
+        h = h ^ (h >> np.uint32(16))
+        h = h * np.uint32(PROTO_MULT32)
+    return h ^ (h >> np.uint32(16))
+
+
+def proto_index_rank(h: np.ndarray, p: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Bucket (top p bits) and rank (closed form of ``proto_rank_loop``: the
+    scanned window is the low 32 - p bits of ``(h << p) | 1``, never zero)."""
+    h = np.asarray(h).astype(np.uint64)
+    j = (h >> np.uint64(32 - p)).astype(np.int64)
+    window = ((h << np.uint64(p)) | np.uint64(1)) & np.uint64((1 << (32 - p)) - 1)
+    bl = np.zeros(window.shape, dtype=np.int64)
+    for b in range(32 - p):
+        bl += (window >= np.uint64(1 << b)).astype(np.int64)
+    return j, (32 - p) - bl + 1
+
+
+def build_registers_proto_py(row_ptr, col_idx, p: int) -> np.ndarray:
+    """``build_sketches`` (gpu_experiment_fixed.py:131-166) one element at a
+    time: uint8 [n, 2^p], every vertex sketched (small graphs only)."""
+    n = len(row_ptr) - 1
+    m = 1 << p
+    regs = np.zeros((n, m), dtype=np.uint8)
+    for u in range(n):
+        for k in range(int(row_ptr[u]), int(row_ptr[u + 1])):
+            h = proto_hash32_int(int(col_idx[k]))
+            j = (h >> (32 - p)) & (m - 1)
+            r = proto_rank_loop(h, p)
+            if r > regs[u, j]:
+                regs[u, j] = r
+    return regs
+
+
+def build_registers_proto(row_ptr, col_idx, p: int) -> np.ndarray:
+    """Vectorised ``build_registers_proto_py``: uint8 [n, 2^p]."""
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    n = len(row_ptr) - 1
+    m = 1 << p
+    j, rho = proto_index_rank(proto_hash32(np.asarray(col_idx)), p)
+    u = np.repeat(np.arange(n, dtype=np.int64), np.diff(row_ptr))
+    regs = np.zeros(n * m, dtype=np.uint8)
+    np.maximum.at(regs, u * m + j, rho.astype(np.uint8))
+    return regs.reshape(n, m)
+
+
+def estimate_proto_rows(regs: np.ndarray) -> np.ndarray:
+    """The prototype's estimator per row (gpu_experiment_fixed.py:236-249) in
+    FP64: raw ``alpha m^2 / Z``, LinearCounting when raw <= 2.5 m and a
+    register is zero, no large-range correction."""
+    regs = np.asarray(regs)
+    k, m = regs.shape
+    Z = np.sum(np.ldexp(1.0, -regs.astype(np.int64)), axis=1)
+    E = alpha(m) * m * m / Z
+    V = np.sum(regs == 0, axis=1)
+    small = (E <= 2.5 * m) & (V > 0)
+    with np.errstate(divide="ignore"):
+        return np.where(small, m * np.log(m / np.maximum(V, 1)), E)
+
+
+def support_proto_rows(ra: np.ndarray, rb: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """``hll_union`` (gpu_experiment_fixed.py:209-254) per row pair in FP64:
+    ``est(a) + est(b) - est(max(a, b))`` raised to 0 when negative.
+    Returns (supports, raised-to-zero flags)."""
+    s = estimate_proto_rows(ra) + estimate_proto_rows(rb) - estimate_proto_rows(np.maximum(ra, rb))
+    neg = s < 0.0
+    return np.where(neg, 0.0, s), neg
+
+
+def support_proto_fp32(ra: np.ndarray, rb: np.ndarray) -> float:
+    """The reference kernel's own FP32 arithmetic for one edge
+    (gpu_experiment_fixed.py:224-253): float sums in register order, float
+    alpha, logf.  Used to bound the FP64 restatement's distance from it."""
+    m = len(ra)
+    f = np.float32
+    a = f(0.7213 / (1.0 + 1.079 / m)) if m >= 128 else f({16: 0.673, 32: 0.697, 64: 0.709}[m])
+    su = sv = sw = f(0.0)
+    zu = zv = zw = 0
+    for k in range(m):
+        x, y = int(ra[k]), int(rb[k])
+        w = max(x, y)
+        su = f(su + np.exp2(f(-x)))
+        sv = f(sv + np.exp2(f(-y)))
+        sw = f(sw + np.exp2(f(-w)))
+        zu += x == 0
+        zv += y == 0
+        zw += w == 0
+    fm = f(m)
+    est = []
+    for s_, z in ((su, zu), (sv, zv), (sw, zw)):
+        e = f(a * fm * fm / s_)
+        if e <= f(2.5) * fm and z > 0:
+            e = f(fm * np.log(f(fm / f(z))))
+        est.append(e)
+    s = f(est[0] + est[1] - est[2])
+    return float(s) if s > 0 else 0.0
+
+
+def hybrid_count_proto(row_ptr, col_idx, edge_u, edge_v, regs: np.ndarray, gamma: int) -> dict:
+    """The prototype's hybrid (gpu_experiment_fixed.py:257-271, 437-526):
+    light iff ``min(deg+u, deg+v) <= gamma`` (exact merge count), heavy edges
+    by ``support_proto_rows``.  The prototype times the phases and keeps the
+    per-edge values on the device; this returns them with the totals."""
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    eu = np.asarray(edge_u, dtype=np.int64)
+    ev = np.asarray(edge_v, dtype=np.int64)
+    deg = np.diff(row_ptr)
+    light = np.minimum(deg[eu], deg[ev]) <= gamma
+    sup = np.zeros(len(eu))
+    sup[light] = edge_supports(row_ptr, col_idx, eu[light], ev[light])
+    heavy = np.nonzero(~light)[0]
+    raw = np.zeros(len(eu))
+    scale = np.zeros(len(eu))
+    n_neg = 0
+    if len(heavy):
+        est = estimate_proto_rows(regs)  # per vertex, reused by every edge
+        ra, rb = regs[eu[heavy]], regs[ev[heavy]]
+        ea, eb = est[eu[heavy]], est[ev[heavy]]
+        raw[heavy] = ea + eb - estimate_proto_rows(np.maximum(ra, rb))
+        neg = raw[heavy] < 0.0
+        sup[heavy] = np.where(neg, 0.0, raw[heavy])
+        n_neg = int(neg.sum())
+        scale[heavy] = np.maximum(ea, eb)
+    exact_part = int(np.sum(sup[light]))
+    hll_part = math.fsum(sup[heavy].tolist())
+    # raw supports that are 0 in exact arithmetic (e.g. LinearCounting with
+    # m * V(u|v) == V(u) * V(v)) come out as +-1e-13: their raise-to-0 flag
+    # is a rounding tie, bounded by n_raised_min .. n_raised_max.  An exact 0
+    # (an empty row: est 0 and est(u | v) == est(u) bit for bit) is no tie.
+    tie = (np.abs(raw) <= 1e-9 * scale) & (raw != 0)
+    return {"supports": sup, "light": light, "n_light": int(light.sum()),
+            "n_heavy": int(len(heavy)), "n_raised": n_neg, "s_hat": raw,
+            "n_raised_min": int(((raw < 0) & ~tie & ~light).sum()),
+            "n_raised_max": int(((raw < 0) | tie)[~light].sum()),
+            "exact_part": exact_part, "hll_part_fsum": hll_part,
+            "total_fsum": exact_part + hll_part}
+
+
+# ---------------------------------------------------------------------------
 # Edge-list text  (experiment/exp_gpu.py:93, 131-143, 291-313)
 # ---------------------------------------------------------------------------
 

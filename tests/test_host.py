@@ -98,3 +98,19 @@ def test_bench_rmat_parameters_match_named_workloads():
     for name, (scale, ef, seed) in bench.RMAT_WORKLOADS.items():
         sd = scale - 9
         assert np.array_equal(graphgen.config_pairs(name, sd), graphgen.rmat(9, ef, seed=seed))
+
+
+def test_header_constants_match_python_tables():
+    import os
+    import re
+
+    from triangle_counting_hll_amd import _lib
+
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "tchll.h")).read()
+    defs = {k: int(v) for k, v in re.findall(r"#define (TC_\w+) (-?\d+)", hdr)}
+    assert {"gpu": defs["TC_HLL_GPU"], "cpu_exp": defs["TC_HLL_CPU_EXP"],
+            "proto32": defs["TC_HLL_PROTO32"]} == _lib.HASH_MODES
+    assert (defs["TC_PART_BOTH"], defs["TC_PART_EXACT"], defs["TC_PART_HLL"]) == (
+        _lib.TC_PART_BOTH, _lib.TC_PART_EXACT, _lib.TC_PART_HLL)
+    stages = sorted((v, k) for k, v in defs.items() if k.startswith("TC_STAGE_"))
+    assert [k[len("TC_STAGE_"):].lower() for _, k in stages] == list(_lib.STAGES)

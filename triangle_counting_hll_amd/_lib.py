@@ -89,7 +89,7 @@ class ExactParams(ctypes.Structure):
 PLANS = {"sorted": 0, "atomic": 1}
 
 # hash / estimator variants of the HLL entry points (TC_HLL_* in include/tchll.h)
-HASH_MODES = {"gpu": 0, "cpu_exp": 1}
+HASH_MODES = {"gpu": 0, "cpu_exp": 1, "proto32": 2}
 
 
 def hash_mode(name) -> int:

@@ -12,6 +12,13 @@
 // (id + seed + 0x9E3779B97F4A7C15, then the finaliser); j = low p bits;
 // rho from the high 64 - p bits; estimator with early returns and the 2^32
 // large-range constant.
+//
+// TC_HLL_PROTO32 selects the prototype GPU experiment's variant
+// (experiment1_gpu/gpu_experiment_fixed.py:131-166, 209-254): a 32-bit mixer
+// of id ^ 0x9747B28C; j = top p bits; rho = 1 + the zeros above the first set
+// bit of the low 32 - p bits of (h << p) | 1; estimator with LinearCounting
+// and no large-range correction; HLL support est(u) + est(v) - est(u | v),
+// raised to 0 when negative (no upper clamp).
 #include <algorithm>
 
 #include "tc_common.hpp"
@@ -30,10 +37,26 @@ __device__ __forceinline__ uint32_t hll_rank(uint64_t h, int p) {
   return w == 0 ? (uint32_t)(65 - p) : (uint32_t)(__clzll((long long)w) - p + 1);
 }
 
-// Register index and rank of one id under either hash mode.
+__device__ __forceinline__ uint32_t mix32_proto(uint32_t v) {
+  uint32_t h = 0x9747B28Cu ^ v;
+  h = (h ^ (h >> 16)) * 0x45D9F3Bu;
+  h = (h ^ (h >> 16)) * 0x45D9F3Bu;
+  return h ^ (h >> 16);
+}
+
+__host__ __device__ constexpr bool valid_hash_mode(int mode) {
+  return mode == TC_HLL_GPU || mode == TC_HLL_CPU_EXP || mode == TC_HLL_PROTO32;
+}
+
+// Register index and rank of one id under each hash mode.
 __device__ __forceinline__ void hll_slot(int64_t id, int p, int mode, uint64_t seed, uint32_t &j,
                                          uint32_t &rho) {
-  if (mode == TC_HLL_CPU_EXP) {
+  if (mode == TC_HLL_PROTO32) {
+    const uint32_t h = mix32_proto((uint32_t)id);
+    j = h >> (32 - p);
+    const uint32_t window = ((h << p) | 1u) & ((1u << (32 - p)) - 1u);  // never zero
+    rho = (uint32_t)__clz((int)window) - (uint32_t)p + 1u;
+  } else if (mode == TC_HLL_CPU_EXP) {
     const uint64_t h = mix64((uint64_t)id + seed + 0x9E3779B97F4A7C15ull);
     j = (uint32_t)(h & ((1ull << p) - 1));
     const uint64_t w = h >> p;  // the high 64 - p bits
@@ -107,10 +130,16 @@ __device__ __forceinline__ double hll_alpha(int m) {
 // Z = sum_j 2^-r_j (zero registers included), zeros = #{j : r_j = 0}.
 // TC_HLL_CPU_EXP (cpu_exp/triangle_hll_experiment.py:356-365): LinearCounting
 // or the 2^32 large-range correction of the raw estimate, each returned as is.
+// TC_HLL_PROTO32 (experiment1_gpu/gpu_experiment_fixed.py:236-249): LinearCounting
+// only.
 __device__ __forceinline__ double hll_estimate(int m, uint64_t zeros, double Z,
                                               int mode = TC_HLL_GPU) {
   const double md = (double)m;
   double E = hll_alpha(m) * md * md / Z;
+  if (mode == TC_HLL_PROTO32) {
+    if (E <= 2.5 * md && zeros > 0) E = md * log(md / (double)zeros);
+    return E;
+  }
   if (mode == TC_HLL_CPU_EXP) {
     if (E <= 2.5 * md && zeros > 0) return md * log(md / (double)zeros);
     const double two32 = 4294967296.0;
@@ -133,65 +162,111 @@ __device__ __forceinline__ void acc_reg(uint32_t r, uint32_t &zeros, double &z) 
   z += pow2_neg(r);
 }
 
-// Wave-cooperative register merge: every lane sees the totals on return.
-// Each lane sums its registers in a fixed order, then a fixed butterfly adds
-// the lanes, so the result is deterministic; the FP64 sum of at most 2^16
-// powers of two is within ~1e-14 relative of the exact value.
-__device__ __forceinline__ void wave_union_stats(const uint8_t *__restrict__ ra,
-                                                 const uint8_t *__restrict__ rb, int m,
-                                                 uint64_t &zeros_out, double &z_out) {
-  const int lane = lane_id();
-  uint32_t zeros = 0;
-  double z = 0.0;
-  if (m >= 16 * kWave) {
-#pragma unroll 4
-    for (int j = lane * 16; j < m; j += 16 * kWave) {
-      const uint4 a = *(const uint4 *)(ra + j);
-      const uint4 b = *(const uint4 *)(rb + j);
-      const uint32_t aw[4] = {a.x, a.y, a.z, a.w}, bw[4] = {b.x, b.y, b.z, b.w};
+// ---------------------------------------------------------------------------
+// Batched per-edge register statistics.
+//
+// A wavefront takes 64 edges at a time ("batch slots"), one slot per lane.  An
+// edge's two register rows are read by a group of G = min(64, m / 16) lanes,
+// lane s of the group reading the 16-byte chunks s, s + G, s + 2G, ... of both
+// rows, so a wave step covers 64 / G edges and small sketches (m < 1024) keep
+// every lane loading.  The group's sums are reduced with log2(G) butterfly
+// steps and handed to the lane that owns the slot; after G steps every lane
+// holds the statistics of its own edge and the 64 estimates (FP64 divide and
+// log) run side by side instead of on one lane per edge.  Each lane sums its
+// chunks in a fixed order and the butterfly is fixed, so results are
+// deterministic.
+// ---------------------------------------------------------------------------
+struct RegStats {
+  uint32_t zeros[3];  // [row a, row b, merged]; a and b only when kTriple
+  double z[3];
+};
+
+template <int G, bool kTriple>
+__device__ __forceinline__ void group_partials(const uint8_t *__restrict__ ra,
+                                               const uint8_t *__restrict__ rb, int m, int sub,
+                                               RegStats &st) {
+#pragma unroll 2
+  for (int j = sub * 16; j < m; j += 16 * G) {
+    const uint4 a = *(const uint4 *)(ra + j);
+    const uint4 b = *(const uint4 *)(rb + j);
+    const uint32_t aw[4] = {a.x, a.y, a.z, a.w}, bw[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int s = 0; s < 32; s += 8)
-          acc_reg(max((aw[q] >> s) & 0xffu, (bw[q] >> s) & 0xffu), zeros, z);
-    }
-  } else if (m >= 4 * kWave) {
-    for (int j = lane * 4; j < m; j += 4 * kWave) {
-      const uint32_t a = *(const uint32_t *)(ra + j), b = *(const uint32_t *)(rb + j);
-#pragma unroll
-      for (int s = 0; s < 32; s += 8) acc_reg(max((a >> s) & 0xffu, (b >> s) & 0xffu), zeros, z);
-    }
-  } else {
-    for (int j = lane; j < m; j += kWave) acc_reg(max((uint32_t)ra[j], (uint32_t)rb[j]), zeros, z);
+      for (int sh = 0; sh < 32; sh += 8) {
+        const uint32_t x = (aw[q] >> sh) & 0xffu, y = (bw[q] >> sh) & 0xffu;
+        if (kTriple) {
+          acc_reg(x, st.zeros[0], st.z[0]);
+          acc_reg(y, st.zeros[1], st.z[1]);
+        }
+        acc_reg(max(x, y), st.zeros[2], st.z[2]);
+      }
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    zeros += __shfl_xor(zeros, off);
-    z += __shfl_xor(z, off);
-  }
-  zeros_out = zeros;
-  z_out = z;
 }
 
+// Statistics of the edge in this lane's batch slot over the first `count`
+// registers of its two rows (rows `stride` bytes apart).  row_a / row_b: this
+// lane's slot's register rows (any value for unused slots).  n_valid: slots in
+// use.
+template <int G, bool kTriple>
+__device__ __forceinline__ RegStats batch_stats(const uint8_t *__restrict__ regs, int stride,
+                                                int count, int32_t row_a, int32_t row_b,
+                                                int n_valid) {
+  constexpr int kEpw = kWave / G;  // edges per wave step
+  const int lane = lane_id();
+  const int sub = lane % G, grp = lane / G;
+  RegStats mine = {};
+  for (int t = 0; t < G; ++t) {
+    const int src = t * kEpw + grp;  // slot handled by this lane's group
+    const int32_t a = __shfl(row_a, src, kWave), b = __shfl(row_b, src, kWave);
+    RegStats st = {};
+    if (src < n_valid)
+      group_partials<G, kTriple>(regs + (int64_t)a * stride, regs + (int64_t)b * stride, count,
+                                 sub, st);
+#pragma unroll
+    for (int q = kTriple ? 0 : 2; q < 3; ++q) {
+#pragma unroll
+      for (int off = G / 2; off > 0; off >>= 1) {
+        st.zeros[q] += __shfl_xor(st.zeros[q], off, kWave);
+        st.z[q] += __shfl_xor(st.z[q], off, kWave);
+      }
+      // slot `lane` is handled at step lane / kEpw by group lane % kEpw
+      const int from = (lane % kEpw) * G;
+      const uint32_t zz = __shfl(st.zeros[q], from, kWave);
+      const double zv = __shfl(st.z[q], from, kWave);
+      if (lane / kEpw == t) {
+        mine.zeros[q] = zz;
+        mine.z[q] = zv;
+      }
+    }
+  }
+  return mine;
+}
+
+template <int G>
 __global__ void __launch_bounds__(kBlock)
 k_union_estimate(const uint8_t *__restrict__ regs, const int32_t *__restrict__ slot, int p,
                  const int32_t *__restrict__ us, const int32_t *__restrict__ vs, int64_t k,
                  double *__restrict__ out, int mode) {
   const int m = 1 << p;
+  const int lane = lane_id();
   const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
-  for (int64_t i = wid; i < k; i += nw) {
-    const int64_t a = slot ? slot[us[i]] : us[i];
-    const int64_t b = slot ? slot[vs[i]] : vs[i];
-    uint64_t zeros;
-    double z;
-    wave_union_stats(regs + a * m, regs + b * m, m, zeros, z);
-    if (lane_id() == 0) out[i] = hll_estimate(m, zeros, z, mode);
+  for (int64_t base = wid * kWave; base < k; base += nw * kWave) {
+    const int64_t i = base + lane;
+    const int n_valid = (int)min<int64_t>(kWave, k - base);
+    int32_t a = 0, b = 0;
+    if (i < k) {
+      a = slot ? slot[us[i]] : us[i];
+      b = slot ? slot[vs[i]] : vs[i];
+    }
+    const RegStats st = batch_stats<G, false>(regs, m, m, a, b, n_valid);
+    if (i < k) out[i] = hll_estimate(m, st.zeros[2], st.z[2], mode);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Router features: one wavefront per candidate edge.
+// Router features, candidates in batches of 64 per wavefront (batch_stats):
 //   f0 = log2(1 + deg u), f1 = log2(1 + deg v), f2 = f0 - f1,
 //   f3 = log2(1 + card u), f4 = log2(1 + card v)  (per-vertex estimates),
 //   f5 = log2(1 + sampled union): the estimator on the first 2^s registers of
@@ -200,6 +275,7 @@ k_union_estimate(const uint8_t *__restrict__ regs, const int32_t *__restrict__ s
 //        union -- scaled by 2^(p-s).  It reads 2 * 2^s bytes per edge instead
 //        of 2 * 2^p.
 // ---------------------------------------------------------------------------
+template <int G>
 __global__ void __launch_bounds__(kBlock)
 k_router_features(const int32_t *__restrict__ degrees, const int32_t *__restrict__ eu,
                   const int32_t *__restrict__ ev, const int32_t *__restrict__ cand, int64_t n_cand,
@@ -208,28 +284,33 @@ k_router_features(const int32_t *__restrict__ degrees, const int32_t *__restrict
   const int m = 1 << p;
   const int ms = 1 << sample_bits;
   const double scale = (double)(1 << (p - sample_bits));
+  const int lane = lane_id();
   const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
-  for (int64_t i = wid; i < n_cand; i += nw) {
-    const int32_t e = cand[i];
-    const int32_t u = eu[e], v = ev[e];
-    const int64_t a = slot ? slot[u] : u;
-    const int64_t b = slot ? slot[v] : v;
-    uint64_t zeros;
-    double z;
-    wave_union_stats(regs + a * m, regs + b * m, ms, zeros, z);
-    if (lane_id() == 0) {
-      const double us = hll_estimate(ms, zeros, z) * scale;
-      const float f0 = log2f(1.0f + (float)degrees[u]);
-      const float f1 = log2f(1.0f + (float)degrees[v]);
-      float *f = feats + i * 6;
-      f[0] = f0;
-      f[1] = f1;
-      f[2] = f0 - f1;
-      f[3] = log2f(1.0f + card[u]);
-      f[4] = log2f(1.0f + card[v]);
-      f[5] = (float)log2(1.0 + us);
+  for (int64_t base = wid * kWave; base < n_cand; base += nw * kWave) {
+    const int64_t i = base + lane;
+    const bool valid = i < n_cand;
+    const int n_valid = (int)min<int64_t>(kWave, n_cand - base);
+    int32_t u = 0, v = 0, a = 0, b = 0;
+    if (valid) {
+      const int32_t e = cand[i];
+      u = eu[e];
+      v = ev[e];
+      a = slot ? slot[u] : u;
+      b = slot ? slot[v] : v;
     }
+    const RegStats st = batch_stats<G, false>(regs, m, ms, a, b, n_valid);
+    if (!valid) continue;
+    const double us = hll_estimate(ms, st.zeros[2], st.z[2]) * scale;
+    const float f0 = log2f(1.0f + (float)degrees[u]);
+    const float f1 = log2f(1.0f + (float)degrees[v]);
+    float *f = feats + i * 6;
+    f[0] = f0;
+    f[1] = f1;
+    f[2] = f0 - f1;
+    f[3] = log2f(1.0f + card[u]);
+    f[4] = log2f(1.0f + card[v]);
+    f[5] = (float)log2(1.0 + us);
   }
 }
 
@@ -278,8 +359,10 @@ __global__ void k_route(Degrees<I> D, const int32_t *__restrict__ eu, const int3
   }
 }
 
-// One wavefront per HLL-routed edge: support estimate, clamp, clamp count.
-template <class I>
+// HLL-routed edges in batches of 64 per wavefront (batch_stats): support
+// estimate, clamp, clamp count.  kTriple: the prototype's support from three
+// estimates (TC_HLL_PROTO32).
+template <class I, int G, bool kTriple>
 __global__ void __launch_bounds__(kBlock)
 k_hll_edges(Degrees<I> D, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
             const int32_t *__restrict__ list, const int64_t *__restrict__ n_list_p,
@@ -287,22 +370,37 @@ k_hll_edges(Degrees<I> D, const int32_t *__restrict__ eu, const int32_t *__restr
             double *__restrict__ s_out, double *__restrict__ per_edge,
             unsigned long long *__restrict__ n_clamped, int mode) {
   const int m = 1 << p;
+  const int lane = lane_id();
   const int64_t n_list = *n_list_p;
   const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
   uint64_t clamped = 0;
-  for (int64_t i = wid; i < n_list; i += nw) {
-    const int32_t e = list[i];
-    const int32_t u = eu[e], v = ev[e];
-    const int64_t a = slot ? slot[u] : u;
-    const int64_t b = slot ? slot[v] : v;
-    uint64_t zeros;
-    double z;
-    wave_union_stats(regs + a * m, regs + b * m, m, zeros, z);
-    if (lane_id() == 0) {
-      const double E = hll_estimate(m, zeros, z, mode);
+  for (int64_t base = wid * kWave; base < n_list; base += nw * kWave) {
+    const int64_t i = base + lane;
+    const bool valid = i < n_list;
+    const int n_valid = (int)min<int64_t>(kWave, n_list - base);
+    int32_t e = 0, u = 0, v = 0, a = 0, b = 0;
+    if (valid) {
+      e = list[i];
+      u = eu[e];
+      v = ev[e];
+      a = slot ? slot[u] : u;
+      b = slot ? slot[v] : v;
+    }
+    const RegStats st = batch_stats<G, kTriple>(regs, m, m, a, b, n_valid);
+    if (!valid) continue;
+    double s;
+    if (kTriple) {
+      s = hll_estimate(m, st.zeros[0], st.z[0], mode) + hll_estimate(m, st.zeros[1], st.z[1], mode) -
+          hll_estimate(m, st.zeros[2], st.z[2], mode);
+      if (s < 0.0) {
+        s = 0.0;
+        ++clamped;
+      }
+    } else {
+      const double E = hll_estimate(m, st.zeros[2], st.z[2], mode);
       const int64_t du = D(u), dv = D(v);
-      double s = (double)du + (double)dv - E;
+      s = (double)du + (double)dv - E;
       const double upper = (double)min(du, dv);
       if (s < 0.0) {
         s = 0.0;
@@ -311,12 +409,12 @@ k_hll_edges(Degrees<I> D, const int32_t *__restrict__ eu, const int32_t *__restr
         s = upper;
         ++clamped;
       }
-      s_out[i] = s;
-      if (per_edge) per_edge[e] = s;
     }
+    s_out[i] = s;
+    if (per_edge) per_edge[e] = s;
   }
   clamped = wave_sum(clamped);
-  if (lane_id() == 0 && clamped) atomicAdd(n_clamped, (unsigned long long)clamped);
+  if (lane == 0 && clamped) atomicAdd(n_clamped, (unsigned long long)clamped);
 }
 
 template <class I>
@@ -420,6 +518,20 @@ k_hybrid_finish(const double *__restrict__ partial, const int64_t *__restrict__ 
   }
 }
 
+// Lanes per edge for m = 2^p registers: 16 bytes per lane per step, at most a wave.
+inline int group_lanes(int p) { return p >= 10 ? 64 : 1 << (p - 4); }
+
+#define TC_DISPATCH_GROUP(p, G, ...)                                 \
+  switch (group_lanes(p)) {                                          \
+    case 1: { constexpr int G = 1; __VA_ARGS__; } break;             \
+    case 2: { constexpr int G = 2; __VA_ARGS__; } break;             \
+    case 4: { constexpr int G = 4; __VA_ARGS__; } break;             \
+    case 8: { constexpr int G = 8; __VA_ARGS__; } break;             \
+    case 16: { constexpr int G = 16; __VA_ARGS__; } break;           \
+    case 32: { constexpr int G = 32; __VA_ARGS__; } break;           \
+    default: { constexpr int G = 64; __VA_ARGS__; } break;           \
+  }
+
 }  // namespace tc
 
 // ---------------------------------------------------------------------------
@@ -443,8 +555,7 @@ extern "C" int tc_hll_build(const void *row_ptr, int row_ptr_64, const int32_t *
   using namespace tc;
   Frame _frame((hipStream_t)stream);
   TC_CHECK(p >= 4 && p <= 16, TC_ERR_INVALID, "p must be in [4, 16], got %d", p);
-  TC_CHECK(hash_mode == TC_HLL_GPU || hash_mode == TC_HLL_CPU_EXP, TC_ERR_INVALID,
-           "unknown hash mode %d", hash_mode);
+  TC_CHECK(valid_hash_mode(hash_mode), TC_ERR_INVALID, "unknown hash mode %d", hash_mode);
   TC_CHECK(row_ptr && has_sketch && n_nodes >= 0, TC_ERR_INVALID,
            "row_ptr and has_sketch are required");
   TC_CHECK(n_nodes == 0 || regs, TC_ERR_INVALID, "regs is required");
@@ -479,15 +590,17 @@ extern "C" int tc_union_estimate(const uint8_t *regs, const int32_t *slot, int p
   using namespace tc;
   Frame _frame((hipStream_t)stream);
   TC_CHECK(p >= 4 && p <= 16, TC_ERR_INVALID, "p must be in [4, 16], got %d", p);
-  TC_CHECK(hash_mode == TC_HLL_GPU || hash_mode == TC_HLL_CPU_EXP, TC_ERR_INVALID,
-           "unknown hash mode %d", hash_mode);
+  TC_CHECK(valid_hash_mode(hash_mode), TC_ERR_INVALID, "unknown hash mode %d", hash_mode);
   TC_CHECK(k >= 0 && (k == 0 || (regs && us && vs && out)), TC_ERR_INVALID,
            "invalid arguments to tc_union_estimate");
   TC_CHECK(((uintptr_t)regs & 15) == 0, TC_ERR_INVALID, "regs must be 16-byte aligned");
   TC_TRY(ensure_init());
   if (k == 0) return TC_OK;
-  hipLaunchKernelGGL(k_union_estimate, dim3(grid_for(k * kWave, kBlock, 65536)), dim3(kBlock), 0,
-                     (hipStream_t)stream, regs, slot, p, us, vs, k, out, hash_mode);
+  const unsigned g = (unsigned)std::min<int64_t>((k + kBlock - 1) / kBlock, num_cus() * 16);
+  TC_DISPATCH_GROUP(p, G, {
+    hipLaunchKernelGGL(k_union_estimate<G>, dim3(g), dim3(kBlock), 0, (hipStream_t)stream, regs,
+                       slot, p, us, vs, k, out, hash_mode);
+  });
   TC_HIP(hipGetLastError());
   return TC_OK;
 }
@@ -506,9 +619,13 @@ extern "C" int tc_router_features(const int32_t *degrees, const int32_t *edge_u,
   TC_CHECK(((uintptr_t)regs & 15) == 0, TC_ERR_INVALID, "regs must be 16-byte aligned");
   TC_TRY(ensure_init());
   if (n_cand == 0) return TC_OK;
-  hipLaunchKernelGGL(k_router_features, dim3(grid_for(n_cand * kWave, kBlock, 65536)),
-                     dim3(kBlock), 0, (hipStream_t)stream, degrees, edge_u, edge_v, cand, n_cand,
-                     regs, slot, p, sample_bits, card, feats);
+  const unsigned g =
+      (unsigned)std::min<int64_t>((n_cand + kBlock - 1) / kBlock, num_cus() * 16);
+  TC_DISPATCH_GROUP(sample_bits, G, {
+    hipLaunchKernelGGL(k_router_features<G>, dim3(g), dim3(kBlock), 0, (hipStream_t)stream,
+                       degrees, edge_u, edge_v, cand, n_cand, regs, slot, p, sample_bits, card,
+                       feats);
+  });
   TC_HIP(hipGetLastError());
   return TC_OK;
 }
@@ -522,8 +639,7 @@ extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_
                                void *stream) {
   using namespace tc;
   Frame _frame((hipStream_t)stream);
-  TC_CHECK(hash_mode == TC_HLL_GPU || hash_mode == TC_HLL_CPU_EXP, TC_ERR_INVALID,
-           "unknown hash mode %d", hash_mode);
+  TC_CHECK(valid_hash_mode(hash_mode), TC_ERR_INVALID, "unknown hash mode %d", hash_mode);
   TC_CHECK(part == TC_PART_BOTH || part == TC_PART_EXACT || part == TC_PART_HLL, TC_ERR_INVALID,
            "unknown part %d", part);
   TC_CHECK(part == TC_PART_EXACT || (p >= 4 && p <= 16), TC_ERR_INVALID,
@@ -571,13 +687,26 @@ extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_
       }
       if (part != TC_PART_EXACT) {
         TC_TRY(compact_indices(use.as<uint8_t>(), E, list.as<int32_t>(), nlist.as<int64_t>(), s));
-        const unsigned gw = (unsigned)std::max(1, num_cus() * 8);
+        // the HLL edge count is on the device: size the grid for all E edges
+        const unsigned gw = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>((E + kBlock - 1) / kBlock, num_cus() * 16));
         {
           StageTimer _timer(TC_STAGE_HLL_EDGES, s);
-          hipLaunchKernelGGL(k_hll_edges<I>, dim3(gw), dim3(kBlock), 0, s, D, edge_u, edge_v,
-                             (const int32_t *)list.as<int32_t>(), (const int64_t *)nlist.as<int64_t>(),
-                             regs, slot, p, svals.as<double>(), per_edge, counters + 2,
-                             hash_mode);
+          const int32_t *lp = list.as<int32_t>();
+          const int64_t *np_ = nlist.as<int64_t>();
+          if (hash_mode == TC_HLL_PROTO32) {
+            TC_DISPATCH_GROUP(p, G, {
+              hipLaunchKernelGGL((k_hll_edges<I, G, true>), dim3(gw), dim3(kBlock), 0, s, D, edge_u,
+                                 edge_v, lp, np_, regs, slot, p, svals.as<double>(), per_edge,
+                                 counters + 2, hash_mode);
+            });
+          } else {
+            TC_DISPATCH_GROUP(p, G, {
+              hipLaunchKernelGGL((k_hll_edges<I, G, false>), dim3(gw), dim3(kBlock), 0, s, D,
+                                 edge_u, edge_v, lp, np_, regs, slot, p, svals.as<double>(),
+                                 per_edge, counters + 2, hash_mode);
+            });
+          }
         }
         hipLaunchKernelGGL(k_tile_sums_f64, dim3((unsigned)std::min<int64_t>(max_tiles, 65536)),
                            dim3(kBlock), 0, s, (const double *)svals.as<double>(),
