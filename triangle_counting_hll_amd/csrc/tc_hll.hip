@@ -157,11 +157,6 @@ __device__ __forceinline__ double pow2_neg(uint32_t r) {
   return __hiloint2double((int)((1023u - r) << 20), 0);
 }
 
-__device__ __forceinline__ void acc_reg(uint32_t r, uint32_t &zeros, double &z) {
-  zeros += (r == 0);
-  z += pow2_neg(r);
-}
-
 // ---------------------------------------------------------------------------
 // Batched per-edge register statistics.
 //
@@ -181,26 +176,50 @@ struct RegStats {
   double z[3];
 };
 
+// Number of zero bytes in a 32-bit word: a byte's top bit in t is set iff the
+// byte is non-zero (adding 0x7F carries into bit 7 for any non-zero low 7 bits).
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t w) {
+  const uint32_t t = ((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w;
+  return 4u - (uint32_t)__popc(t & 0x80808080u);
+}
+
+// Adds this lane's chunks of two register rows: Z terms per byte position in
+// four accumulators (independent FP64 add chains), combined in a fixed order.
 template <int G, bool kTriple>
 __device__ __forceinline__ void group_partials(const uint8_t *__restrict__ ra,
                                                const uint8_t *__restrict__ rb, int m, int sub,
                                                RegStats &st) {
+  constexpr int kQ = kTriple ? 3 : 1;  // statistics kept: [merged] or [a, b, merged]
+  double z4[kQ][4] = {};
+  uint32_t zeros[kQ] = {};
 #pragma unroll 2
   for (int j = sub * 16; j < m; j += 16 * G) {
     const uint4 a = *(const uint4 *)(ra + j);
     const uint4 b = *(const uint4 *)(rb + j);
     const uint32_t aw[4] = {a.x, a.y, a.z, a.w}, bw[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int sh = 0; sh < 32; sh += 8) {
-        const uint32_t x = (aw[q] >> sh) & 0xffu, y = (bw[q] >> sh) & 0xffu;
-        if (kTriple) {
-          acc_reg(x, st.zeros[0], st.z[0]);
-          acc_reg(y, st.zeros[1], st.z[1]);
-        }
-        acc_reg(max(x, y), st.zeros[2], st.z[2]);
+    for (int q = 0; q < 4; ++q) {
+      zeros[kQ - 1] += zero_bytes(aw[q] | bw[q]);  // max is 0 iff both are 0
+      if (kTriple) {
+        zeros[0] += zero_bytes(aw[q]);
+        zeros[1] += zero_bytes(bw[q]);
       }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t x = (aw[q] >> (8 * k)) & 0xffu, y = (bw[q] >> (8 * k)) & 0xffu;
+        z4[kQ - 1][k] += pow2_neg(max(x, y));
+        if (kTriple) {
+          z4[0][k] += pow2_neg(x);
+          z4[1][k] += pow2_neg(y);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) {
+    const int o = kTriple ? q : 2;
+    st.zeros[o] += zeros[q];
+    st.z[o] += (z4[q][0] + z4[q][1]) + (z4[q][2] + z4[q][3]);
   }
 }
 
