@@ -98,6 +98,22 @@ def test_block_phase_bounds_on_cliques(device, blk):
         _count(csr, params={"block_min_bits": 7})
 
 
+def test_center_beyond_workgroup_table_takes_global_search(device):
+    # K_16385, the smallest graph with a forward row above 16383 ids (the
+    # most the 128 KiB workgroup table holds): vertex 0 has d+ = 16384, so
+    # its work items take the global binary search inside k_heavy_big
+    import torch
+
+    n = 16385
+    iu = torch.triu_indices(n, n, 1, device="cuda")
+    pairs = iu.t().to(torch.int32).contiguous()
+    del iu
+    csr = gpu_graph.build_csr_gpu(pairs)
+    del pairs
+    assert int(csr.degrees.max().item()) == n - 1
+    assert _count(csr) == math.comb(n, 3)
+
+
 def test_rgg_small(device):
     pairs = random_geometric(1 << 15, 2.8, seed=1)
     csr = gpu_graph.build_csr_gpu(pairs)
