@@ -154,7 +154,11 @@ typedef struct tc_exact_params {
 /* count[0] += sum over the given oriented edges (edge_u[e] -> edge_v[e]) of
  * |N+(u) ∩ N+(v)|, with N+(x) = col_idx[row_ptr[x] : row_ptr[x+1]] sorted.
  * `count` is a device uint64.  edge_u/edge_v may both be NULL: the edge list
- * is then the CSR's own COO.  params may be NULL.
+ * is then the CSR's own COO.  params may be NULL.  Stream-ordered, except that
+ * for at most 2^24 edges outside stream capture the call synchronises the
+ * stream once, after the merge pass, and returns early when no edge needs the
+ * hash-probe path (small inputs are bound by launch latency); the same holds
+ * for the exact part of tc_hybrid_count.
  * Replaces src/gpu_hybrid.py:96-130 (exact_count_kernel) and :332-354
  * (gpu_exact_triangle_count). */
 int tc_exact_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,

@@ -28,6 +28,7 @@ constexpr int kTblMaxDeg = 1023;  // longer center rows: one table per workgroup
 constexpr int kBlkMinBits = 10;   // default: d+ in [512, 4095] take the workgroup's four wave tables
 constexpr int kBlkTblInts = 4 * kTblInts;  // joined into one (2048 buckets, 32 KiB)
 constexpr int kBigMinBits = 13;   // d+ >= 4096: the 128 KiB workgroup kernel
+constexpr int64_t kEarlyExitMaxEdges = int64_t(1) << 24;  // see exact_sorted_plan
 constexpr int kBigTblInts = 32768;  // block-shared table: 8192 buckets x 4 slots (128 KiB)
 constexpr int kBigMaxDeg = kBigTblInts / 2 - 1;  // longer rows: global binary search
 constexpr int kDefaultUnroll = 8; // streamed ids in flight per lane
@@ -765,6 +766,20 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                          count, n_heavy);
     }
     TC_HIP(hipGetLastError());
+    // Small inputs are launch-bound: when classify found no heavy edge, the
+    // sort, planner and heavy kernels (about 30 launches) have nothing to do.
+    // Outside stream capture and graph building, read the heavy count back
+    // once and stop; large inputs never wait on the host.
+    if (E <= kEarlyExitMaxEdges && current_arena() == nullptr) {
+      hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+      TC_HIP(hipStreamIsCapturing(s, &cap));
+      if (cap == hipStreamCaptureStatusNone) {
+        unsigned long long h_heavy = 0;
+        TC_HIP(hipMemcpyAsync(&h_heavy, n_heavy, sizeof(h_heavy), hipMemcpyDeviceToHost, s));
+        TC_HIP(hipStreamSynchronize(s));
+        if (h_heavy == 0) return TC_OK;
+      }
+    }
     uint64_t *sorted = nullptr;
     TC_TRY(radix_sort_u64(packed.as<uint64_t>(), tmp.as<uint64_t>(), E, 32, 32 + key_bits,
                           &sorted, s));
