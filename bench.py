@@ -429,7 +429,7 @@ def main():
     ap.add_argument("--light-max", type=int, default=32)
     ap.add_argument("--chunk", type=int, default=256)
     ap.add_argument("--waves-per-cu", type=int, default=16)
-    ap.add_argument("--unroll", type=int, default=8)
+    ap.add_argument("--unroll", type=int, default=16)
     ap.add_argument("--extras", default="all",
                     help="comma-separated extra measurements to run (hybrid_fixed_threshold, "
                          "ingest_text, batch4096, learned_router, prototype_table) or 'all'")

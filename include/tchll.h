@@ -138,7 +138,7 @@ typedef struct tc_exact_params {
   int light_max;     /* edges with deg+u + deg+v <= light_max use a lane merge */
   int chunk;         /* others per (center, chunk) work item                   */
   int waves_per_cu;  /* persistent waves per CU for the heavy kernel           */
-  int unroll;        /* streamed ids in flight per lane: 1, 2, 4 or 8          */
+  int unroll;        /* streamed ids in flight per lane: 1, 2, 4, 8 or 16     */
   int plan;          /* TC_PLAN_SORTED (default) or TC_PLAN_ATOMIC             */
   int block_min_bits; /* centers with bit_length(deg+) in [block_min_bits, 12]
                          are taken by whole workgroups over one joined LDS
@@ -171,6 +171,14 @@ int tc_exact_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
 int tc_edge_supports(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
                      const int32_t *edge_u, const int32_t *edge_v, int64_t n_edges,
                      int32_t *out, void *stream);
+
+/* Stream compaction: idx receives, in increasing order, every i < n with
+ * flags[i] != 0, and count (device int64) their number.  The device form of
+ * the prototype's edge partition cp.where(light_mask)[0]
+ * (experiment1_gpu/gpu_experiment_fixed.py:470); the library's own planners,
+ * CSR build and sketch lists use the same kernels.  n < 2^31. */
+int tc_compact_flags(const uint8_t *flags, int64_t n, int32_t *idx, int64_t *count,
+                     void *stream);
 
 /* ---- (c) HLL sketch build ----------------------------------------------- */
 

@@ -76,6 +76,7 @@ def test_rmat14_config(device):
 @pytest.mark.parametrize("light_max,chunk,wpc,unroll,blk", [(0, 1, 4, 1, 8), (2, 7, 8, 2, 13),
                                                             (16, 256, 16, 4, 0),
                                                             (64, 32, 32, 8, 10),
+                                                            (32, 256, 16, 16, 10),
                                                             (10**6, 256, 16, 4, 9)])
 def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc, unroll, blk, plan):
     pairs = rmat(12, 16, seed=7)
@@ -96,6 +97,8 @@ def test_block_phase_bounds_on_cliques(device, blk):
     assert _count(csr, params={"block_min_bits": blk}) == math.comb(n, 3)
     with pytest.raises(ValueError, match="block_min_bits"):
         _count(csr, params={"block_min_bits": 7})
+    with pytest.raises(ValueError, match="unroll"):
+        _count(csr, params={"unroll": 3})
 
 
 def test_center_beyond_workgroup_table_takes_global_search(device):
@@ -178,3 +181,28 @@ def test_repeat_is_deterministic(device):
     csr = gpu_graph.build_csr_gpu(pairs)
     a = [_count(csr) for _ in range(3)]
     assert len(set(a)) == 1 and a[0] == _truth(pairs)
+
+
+@pytest.mark.parametrize("n", [0, 1, 15, 16, 17, 1023, 1024, 1025, 70_001])
+@pytest.mark.parametrize("shift", [0, 1, 13])
+def test_flag_indices_match_flatnonzero(device, n, shift):
+    # chunk (1024 flags per wavefront) and 16-byte lane boundaries, 16-byte
+    # aligned arrays (vector loads) and unaligned slices (byte loads), flag
+    # values other than 1 (any nonzero byte counts)
+    import torch
+
+    rng = np.random.default_rng(n + shift)
+    host = (rng.random(n + shift) < 0.3).astype(np.uint8) * rng.integers(1, 256, n + shift,
+                                                                          dtype=np.uint8)
+    dev = torch.from_numpy(host).cuda()[shift:]
+    got = gpu_hybrid.flag_indices(dev).cpu().numpy()
+    assert np.array_equal(got, np.flatnonzero(host[shift:]))
+
+
+def test_flag_indices_all_set_and_none_set(device):
+    import torch
+
+    for fill in (0, 1):
+        f = torch.full((5000,), fill, dtype=torch.uint8, device="cuda")
+        got = gpu_hybrid.flag_indices(f).cpu().numpy()
+        assert np.array_equal(got, np.flatnonzero(np.full(5000, fill)))

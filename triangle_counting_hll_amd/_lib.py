@@ -48,6 +48,7 @@ _SIGNATURES = {
     "tc_exact_count": (I32, [P, I32, P, I64, P, P, I64, P, P, P]),
     "tc_edge_supports": (I32, [P, I32, P, P, P, I64, P, P]),
     "tc_hash64": (I32, [P, P, I64, P]),
+    "tc_compact_flags": (I32, [P, I64, P, P, P]),
     "tc_hll_build": (I32, [P, I32, P, I64, I64, I32, P, P, P, I32, I32, ctypes.c_uint64, P]),
     "tc_union_estimate": (I32, [P, P, I32, P, P, I64, P, I32, P]),
     "tc_hybrid_count": (I32, [P, I32, P, I64, P, P, I64, P, P, P, P, I64, I32, P, I32, P, P, P, P,

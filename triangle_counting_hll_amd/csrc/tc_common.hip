@@ -265,13 +265,6 @@ template int exclusive_scan<int64_t>(const int64_t *, int64_t *, int64_t, int64_
 template int exclusive_scan<uint8_t>(const uint8_t *, int64_t *, int64_t, int64_t *, hipStream_t);
 template int exclusive_scan<uint32_t>(const uint32_t *, int64_t *, int64_t, int64_t *, hipStream_t);
 
-__global__ void k_scatter_flagged(const uint8_t *__restrict__ flag, const int64_t *__restrict__ pos,
-                                  int64_t n, int32_t *__restrict__ idx) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    if (flag[i]) idx[pos[i]] = (int32_t)i;
-}
-
 __global__ void k_copy_i64(const int64_t *src, int64_t *dst) { *dst = *src; }
 
 // ---------------------------------------------------------------------------
@@ -340,18 +333,102 @@ int bucket_sort_u8(const uint8_t *key, int64_t n, int nb, int32_t *perm, int64_t
   return TC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Stream compaction by flag.  A wavefront owns a chunk of 1024 flags, 16
+// consecutive flags per lane read as one 16-byte load when the array is
+// 16-byte aligned.  Pass 1 counts the nonzero flags of every chunk; an
+// exclusive scan of the chunk counts gives each chunk its first output slot;
+// pass 2 re-reads the flags and writes the indices in increasing order (lane
+// offsets from a wave scan).  Traffic: two reads of the flags plus the index
+// writes, and a scan over n / 1024 counts.
+// ---------------------------------------------------------------------------
+constexpr int kCompactPerLane = 16;
+constexpr int kCompactChunk = kWave * kCompactPerLane;
+
+// 16 flags of lane `lane` of chunk `chunk` as four little-endian words
+// (bytes past n read as 0)
+template <bool kVec>
+__device__ __forceinline__ uint4 load_flags16(const uint8_t *__restrict__ flag, int64_t n,
+                                              int64_t first) {
+  if (kVec && first + kCompactPerLane <= n)
+    return *reinterpret_cast<const uint4 *>(flag + first);
+  uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int k = 0; k < kCompactPerLane; ++k)
+    if (first + k < n) w[k >> 2] |= (uint32_t)flag[first + k] << (8 * (k & 3));
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// 0x80 in each byte of the result whose byte in w is nonzero
+__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t w) {
+  return (((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u;
+}
+
+__device__ __forceinline__ int count_flags16(uint4 f) {
+  return __popc(nonzero_bytes(f.x)) + __popc(nonzero_bytes(f.y)) + __popc(nonzero_bytes(f.z)) +
+         __popc(nonzero_bytes(f.w));
+}
+
+template <bool kVec>
+__global__ void __launch_bounds__(kBlock)
+k_compact_counts(const uint8_t *__restrict__ flag, int64_t n, int64_t nchunks,
+                 int32_t *__restrict__ cnt) {
+  const int64_t chunk = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (chunk >= nchunks) return;  // whole wavefronts leave together
+  const int lane = lane_id();
+  const int c = count_flags16(
+      load_flags16<kVec>(flag, n, chunk * kCompactChunk + (int64_t)lane * kCompactPerLane));
+  const int tot = wave_inclusive_scan(c);
+  if (lane == kWave - 1) cnt[chunk] = tot;
+}
+
+template <bool kVec>
+__global__ void __launch_bounds__(kBlock)
+k_compact_write(const uint8_t *__restrict__ flag, int64_t n, int64_t nchunks,
+                const int64_t *__restrict__ offs, int32_t *__restrict__ idx) {
+  const int64_t chunk = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (chunk >= nchunks) return;
+  const int lane = lane_id();
+  const int64_t first = chunk * kCompactChunk + (int64_t)lane * kCompactPerLane;
+  const uint4 f = load_flags16<kVec>(flag, n, first);
+  const int c = count_flags16(f);
+  int64_t o = offs[chunk] + (wave_inclusive_scan(c) - c);
+  const uint32_t w[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+  for (int k = 0; k < kCompactPerLane; ++k)
+    if ((w[k >> 2] >> (8 * (k & 3))) & 0xFFu) idx[o++] = (int32_t)(first + k);
+}
+
 int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d_count,
                     hipStream_t s) {
-  Scratch pos;
-  TC_TRY(pos.alloc(sizeof(int64_t) * (n + 1), s));
-  TC_TRY(exclusive_scan<uint8_t>(flag, pos.as<int64_t>(), n, nullptr, s));
-  if (n > 0) {
-    hipLaunchKernelGGL(k_scatter_flagged, dim3(grid_for(n)), dim3(kBlock), 0, s, flag,
-                       pos.as<int64_t>(), n, idx_out);
+  const int64_t nchunks = (n + kCompactChunk - 1) / kCompactChunk;
+  Scratch cnt, offs;
+  TC_TRY(cnt.alloc(sizeof(int32_t) * std::max<int64_t>(nchunks, 1), s));
+  TC_TRY(offs.alloc(sizeof(int64_t) * (nchunks + 1), s));
+  const bool vec = ((uintptr_t)flag & 15) == 0;
+  const unsigned g = (unsigned)((nchunks * kWave + kBlock - 1) / kBlock);
+  if (nchunks > 0) {
+    if (vec)
+      hipLaunchKernelGGL(k_compact_counts<true>, dim3(g), dim3(kBlock), 0, s, flag, n, nchunks,
+                         cnt.as<int32_t>());
+    else
+      hipLaunchKernelGGL(k_compact_counts<false>, dim3(g), dim3(kBlock), 0, s, flag, n, nchunks,
+                         cnt.as<int32_t>());
+    TC_HIP(hipGetLastError());
+  }
+  // offs[nchunks] = number of flagged indices (0 when n == 0)
+  TC_TRY(exclusive_scan<int32_t>(cnt.as<int32_t>(), offs.as<int64_t>(), nchunks, nullptr, s));
+  if (nchunks > 0) {
+    if (vec)
+      hipLaunchKernelGGL(k_compact_write<true>, dim3(g), dim3(kBlock), 0, s, flag, n, nchunks,
+                         (const int64_t *)offs.as<int64_t>(), idx_out);
+    else
+      hipLaunchKernelGGL(k_compact_write<false>, dim3(g), dim3(kBlock), 0, s, flag, n, nchunks,
+                         (const int64_t *)offs.as<int64_t>(), idx_out);
     TC_HIP(hipGetLastError());
   }
   if (d_count) {
-    hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(1), 0, s, pos.as<int64_t>() + n, d_count);
+    hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(1), 0, s, offs.as<int64_t>() + nchunks, d_count);
     TC_HIP(hipGetLastError());
   }
   return TC_OK;
@@ -415,4 +492,14 @@ extern "C" int tc_device_info(int *n_cu, int *wave_size, int64_t *lds_per_block,
     snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
   }
   return TC_OK;
+}
+
+extern "C" int tc_compact_flags(const uint8_t *flags, int64_t n, int32_t *idx, int64_t *count,
+                                void *stream) {
+  using namespace tc;
+  Frame _frame((hipStream_t)stream);
+  TC_CHECK(n >= 0 && n < INT32_MAX && count && (n == 0 || (flags && idx)), TC_ERR_INVALID,
+           "invalid arguments to tc_compact_flags");
+  TC_TRY(ensure_init());
+  return compact_indices(flags, n, idx, count, (hipStream_t)stream);
 }

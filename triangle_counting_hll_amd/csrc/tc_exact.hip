@@ -31,7 +31,7 @@ constexpr int kBigMinBits = 13;   // d+ >= 4096: the 128 KiB workgroup kernel
 constexpr int64_t kEarlyExitMaxEdges = int64_t(1) << 24;  // see exact_sorted_plan
 constexpr int kBigTblInts = 32768;  // block-shared table: 8192 buckets x 4 slots (128 KiB)
 constexpr int kBigMaxDeg = kBigTblInts / 2 - 1;  // longer rows: global binary search
-constexpr int kDefaultUnroll = 8; // streamed ids in flight per lane
+constexpr int kDefaultUnroll = 16;  // streamed ids in flight per lane
 constexpr int kWavesPerBlock = kBlock / kWave;
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -252,9 +252,29 @@ struct OthersPacked {
   }
 };
 
+// Position of the id at flattened offset `off + t` of col[].  With kNarrow
+// (col[] smaller than 4 GiB) the offset is a 32-bit byte offset added to the
+// array base, one VGPR and one 32-bit add per id instead of 64-bit address
+// arithmetic; `off` may be "negative" (it wraps), only the sum is in range.
+template <bool kNarrow>
+struct IdOffset {
+  using T = int64_t;
+  __device__ __forceinline__ static int32_t load(const int32_t *__restrict__ col, T off, int t) {
+    return col[off + t];
+  }
+};
+template <>
+struct IdOffset<true> {
+  using T = uint32_t;
+  __device__ __forceinline__ static int32_t load(const int32_t *__restrict__ col, T off, int t) {
+    const uint32_t b = (off + (uint32_t)t) << 2;
+    return *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(col) + b);
+  }
+};
+
 // Streams the rows of others[ob:oe] as one flattened sequence, U ids
 // per lane in flight, and counts the ids found in the center's set.
-template <class I, bool kLds, int U, class O>
+template <class I, bool kLds, int U, class O, bool kNarrow>
 __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__restrict__ col,
                                                   O others, int64_t ob,
                                                   int64_t oe, int *__restrict__ seg_end,
@@ -280,9 +300,10 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
     // current row of this lane: its end in the flattened sequence and the
     // offset that maps a flattened position to col[]; LDS is read only when
     // the lane crosses into the next row
+    using Off = IdOffset<kNarrow>;
     int seg = 0;
     int seg_hi = seg_end[0];
-    int64_t seg_off = seg_base[0];
+    typename Off::T seg_off = (typename Off::T)seg_base[0];
     for (int w0 = 0; w0 < total; w0 += kWave * U) {
       const int t0 = w0 + lane;
       int xs[U];
@@ -293,9 +314,9 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
           while (seg_hi <= t) {
             ++seg;
             seg_hi = seg_end[seg];
-            seg_off = seg_base[seg];
+            seg_off = (typename Off::T)seg_base[seg];
           }
-          xs[u] = col[seg_off + t];
+          xs[u] = Off::load(col, seg_off, t);
         }
       } else {
 #pragma unroll
@@ -306,9 +327,9 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
             while (seg_hi <= t) {
               ++seg;
               seg_hi = seg_end[seg];
-              seg_off = seg_base[seg];
+              seg_off = (typename Off::T)seg_base[seg];
             }
-            xs[u] = col[seg_off + t];
+            xs[u] = Off::load(col, seg_off, t);
           }
         }
       }
@@ -318,15 +339,13 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
         // id continue along the probe sequence
         // groups of up to four reads keep the register footprint small
         constexpr int G = U < 4 ? U : 4;
-        uint32_t bk[U];
         uint32_t more = 0;  // bit u: the home bucket of xs[u] is full and lacks it
 #pragma unroll
         for (int g = 0; g < U; g += G) {
           int4 vv[G];
 #pragma unroll
           for (int j = 0; j < G; ++j) {
-            bk[g + j] = bucket_of((uint32_t)xs[g + j], shift);
-            vv[j] = tbl[bk[g + j]];
+            vv[j] = tbl[bucket_of((uint32_t)xs[g + j], shift)];
           }
 #pragma unroll
           for (int j = 0; j < G; ++j) {
@@ -338,7 +357,9 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
         if (more) {
 #pragma unroll
           for (int u = 0; u < U; ++u)
-            if (more & (1u << u)) acc += table_probe_from(tbl, xs[u], (bk[u] + 1) & mask, mask);
+            if (more & (1u << u))
+              acc += table_probe_from(tbl, xs[u], (bucket_of((uint32_t)xs[u], shift) + 1) & mask,
+                                      mask);
         }
       } else {
 #pragma unroll
@@ -374,7 +395,7 @@ struct GraphAcc {
   }
 };
 
-template <class I, int U, class O>
+template <class I, int U, class O, bool kNarrow>
 __global__ void __launch_bounds__(kBlock)
 k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
         O others, const int2 *__restrict__ items,
@@ -426,7 +447,7 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
       int shift;
       uint32_t mask;
       table_build<kBlkTblInts, true>(tbl_blk, col + cb, dc, shift, mask);
-      uint64_t acc = stream_others<I, true, U>(R, col, others, wb, we, seg_end, seg_base,
+      uint64_t acc = stream_others<I, true, U, O, kNarrow>(R, col, others, wb, we, seg_end, seg_base,
                                                &tbl_all[0][0], shift, mask, cb, dc);
       if (graph_of)
         gacc.add(graph_of[c], acc, count);
@@ -462,10 +483,10 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
       int shift;
       uint32_t mask;
       table_build<kTblInts, false>(reinterpret_cast<int *>(tbl), col + cb, dc, shift, mask);
-      acc = stream_others<I, true, U>(R, col, others, ob, oe, seg_end, seg_base, tbl, shift, mask,
+      acc = stream_others<I, true, U, O, kNarrow>(R, col, others, ob, oe, seg_end, seg_base, tbl, shift, mask,
                                       cb, dc);
     } else {
-      acc = stream_others<I, false, U>(R, col, others, ob, oe, seg_end, seg_base, tbl, 0, 0, cb,
+      acc = stream_others<I, false, U, O, kNarrow>(R, col, others, ob, oe, seg_end, seg_base, tbl, 0, 0, cb,
                                        dc);
     }
     if (graph_of)
@@ -513,7 +534,7 @@ __global__ void k_edge_supports(Rows<I> R, const int32_t *__restrict__ col,
 // four wavefronts streaming disjoint quarters of the item's others.  Items of
 // these centers come first in the heavy-first order; this kernel takes items
 // [0, n_first) and k_heavy the rest.
-template <class I, int U, class O>
+template <class I, int U, class O, bool kNarrow>
 __global__ void __launch_bounds__(kBlock)
 k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
             O others, const int2 *__restrict__ items, const int64_t *__restrict__ iptr,
@@ -549,11 +570,11 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restric
       int shift;
       uint32_t mask;
       table_build<kBigTblInts, true>(tbl_i, col + cb, dc, shift, mask);
-      acc = stream_others<I, true, U>(R, col, others, wb, we, seg_end_all[w], seg_base_all[w],
+      acc = stream_others<I, true, U, O, kNarrow>(R, col, others, wb, we, seg_end_all[w], seg_base_all[w],
                                       tbl, shift, mask, cb, dc);
       __syncthreads();  // the table is rebuilt for the next item
     } else {
-      acc = stream_others<I, false, U>(R, col, others, wb, we, seg_end_all[w], seg_base_all[w],
+      acc = stream_others<I, false, U, O, kNarrow>(R, col, others, wb, we, seg_end_all[w], seg_base_all[w],
                                        tbl, 0, 0, cb, dc);
     }
     if (graph_of)
@@ -569,24 +590,53 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restric
   }
 }
 
-template <class I, class O>
-static auto heavy_big_kernel(int unroll) -> decltype(&k_heavy_big<I, 4, O>) {
+// kNarrow = every id position fits a 32-bit byte offset (E <= 2^30)
+template <class I, class O, bool kNarrow>
+static auto heavy_big_kernel_t(int unroll) -> decltype(&k_heavy_big<I, 4, O, kNarrow>) {
   switch (unroll) {
-    case 1: return &k_heavy_big<I, 1, O>;
-    case 2: return &k_heavy_big<I, 2, O>;
-    case 8: return &k_heavy_big<I, 8, O>;
-    default: return &k_heavy_big<I, 4, O>;
+    case 1: return &k_heavy_big<I, 1, O, kNarrow>;
+    case 2: return &k_heavy_big<I, 2, O, kNarrow>;
+    case 4: return &k_heavy_big<I, 4, O, kNarrow>;
+    case 8: return &k_heavy_big<I, 8, O, kNarrow>;
+    default: return &k_heavy_big<I, 16, O, kNarrow>;
   }
 }
 
-template <class I, class O>
-static auto heavy_kernel(int unroll) -> decltype(&k_heavy<I, 4, O>) {
+template <class I, class O, bool kNarrow>
+static auto heavy_kernel_t(int unroll) -> decltype(&k_heavy<I, 4, O, kNarrow>) {
   switch (unroll) {
-    case 1: return &k_heavy<I, 1, O>;
-    case 2: return &k_heavy<I, 2, O>;
-    case 8: return &k_heavy<I, 8, O>;
-    default: return &k_heavy<I, 4, O>;
+    case 1: return &k_heavy<I, 1, O, kNarrow>;
+    case 2: return &k_heavy<I, 2, O, kNarrow>;
+    case 4: return &k_heavy<I, 4, O, kNarrow>;
+    case 8: return &k_heavy<I, 8, O, kNarrow>;
+    default: return &k_heavy<I, 16, O, kNarrow>;
   }
+}
+
+// The entry points do not receive the length of col[], so the bound comes
+// from the allocation holding it: every in-bounds id lies between col and the
+// allocation's end, and the narrow form is taken when that span is at most
+// 4 GiB.  A failed query (memory the runtime does not track) keeps the
+// 64-bit form.
+static bool ids_fit_narrow(const int32_t *col) {
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)col) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  const uintptr_t b = (uintptr_t)base, p = (uintptr_t)col;
+  return p >= b && b + size - p <= (uintptr_t(1) << 32);
+}
+
+template <class I, class O>
+static auto heavy_big_kernel(int unroll, bool narrow) -> decltype(&k_heavy_big<I, 4, O, true>) {
+  return narrow ? heavy_big_kernel_t<I, O, true>(unroll) : heavy_big_kernel_t<I, O, false>(unroll);
+}
+
+template <class I, class O>
+static auto heavy_kernel(int unroll, bool narrow) -> decltype(&k_heavy<I, 4, O, true>) {
+  return narrow ? heavy_kernel_t<I, O, true>(unroll) : heavy_kernel_t<I, O, false>(unroll);
 }
 
 // ---------------------------------------------------------------------------
@@ -808,17 +858,18 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                        (const int64_t *)iptr.as<int64_t>(), n_sorted_runs, items.as<int2>());
     TC_HIP(hipGetLastError());
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
+    const bool narrow = ids_fit_narrow(col);
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
       const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
       const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - blk_bits + 1);
-      const auto kbig = heavy_big_kernel<I, OthersPacked>(unroll);
+      const auto kbig = heavy_big_kernel<I, OthersPacked>(unroll, narrow);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)run_ptr.as<int64_t>(),
                          OthersPacked{(const uint64_t *)sorted}, (const int2 *)items.as<int2>(),
                          (const int64_t *)iptr.as<int64_t>(), n_huge, chunk, next_big, graph_of,
                          count);
-      const auto kfn = heavy_kernel<I, OthersPacked>(unroll);
+      const auto kfn = heavy_kernel<I, OthersPacked>(unroll, narrow);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R,
                          col, (const int64_t *)run_ptr.as<int64_t>(),
                          OthersPacked{(const uint64_t *)sorted}, (const int2 *)items.as<int2>(),
@@ -840,6 +891,8 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
   const int chunk = (params && params->chunk > 0) ? std::min(params->chunk, 65536) : 256;
   const int wpc = (params && params->waves_per_cu > 0) ? params->waves_per_cu : 16;
   const int unroll = (params && params->unroll > 0) ? params->unroll : kDefaultUnroll;
+  TC_CHECK(unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8 || unroll == 16,
+           TC_ERR_INVALID, "unroll must be 1, 2, 4, 8 or 16, got %d", unroll);
   const int plan = params ? params->plan : TC_PLAN_SORTED;
   const int blk_bits = (params && params->block_min_bits > 0) ? params->block_min_bits
                                                               : kBlkMinBits;
@@ -911,17 +964,18 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                        (const int64_t *)iptr.as<int64_t>(), n_cent, items.as<int2>());
     TC_HIP(hipGetLastError());
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
+    const bool narrow = ids_fit_narrow(col);
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
       const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
       const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - blk_bits + 1);
-      const auto kbig = heavy_big_kernel<I, OthersIds>(unroll);
+      const auto kbig = heavy_big_kernel<I, OthersIds>(unroll, narrow);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)cptr.as<int64_t>(),
                          OthersIds{(const int32_t *)others.as<int32_t>()},
                          (const int2 *)items.as<int2>(), (const int64_t *)iptr.as<int64_t>(),
                          n_huge, chunk, ctr.as<unsigned long long>() + 1, graph_of, count);
-      const auto kfn = heavy_kernel<I, OthersIds>(unroll);
+      const auto kfn = heavy_kernel<I, OthersIds>(unroll, narrow);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)cptr.as<int64_t>(),
                          OthersIds{(const int32_t *)others.as<int32_t>()},

@@ -53,6 +53,24 @@ def _warn_if_float64_atomic_slow(cuda_module=None) -> None:
     _FLOAT64_ATOMIC_WARNED = True
 
 
+def flag_indices(flags, stream=None):
+    """Indices of the nonzero entries of a uint8/bool device array, ascending
+    (int32 device tensor): the device form of the prototype's edge partition
+    ``cp.where(light_mask)[0]`` (experiment1_gpu/gpu_experiment_fixed.py:470)."""
+    t = _lib.require_device()
+    f = _lib.as_device(flags, name="flags")
+    if f.dtype == t.bool:
+        f = f.view(t.uint8)
+    if f.dtype != t.uint8:
+        raise ValueError(f"flags must be uint8 or bool, got {f.dtype}")
+    n = f.numel()
+    idx = t.empty(max(n, 1), dtype=t.int32, device=f.device)
+    cnt = t.empty(1, dtype=t.int64, device=f.device)
+    _lib.check(_lib.load().tc_compact_flags(_lib.ptr(f), n, _lib.ptr(idx), _lib.ptr(cnt),
+                                            _lib.stream_ptr(stream)))
+    return idx[:int(cnt.item())]
+
+
 def _exact_params(params: Optional[dict]):
     if not params:
         return None
