@@ -306,7 +306,7 @@ def run_batch_extra(args, reps=10):
     import torch
 
     from oracle import cbind
-    from triangle_counting_hll_amd import autotune, batch
+    from triangle_counting_hll_amd import autotune, batch, gpu_hybrid
 
     spec = batch.many_graphs_spec()
     t0 = time.perf_counter()
@@ -320,6 +320,7 @@ def run_batch_extra(args, reps=10):
     _, ms = _timed(g.launch, reps)
     counts = g.counts.cpu().numpy()
     plain, plain_s = batch.plain_per_graph_counts(b, params)
+    hyb, hyb_s, hyb_build_s = batch.plain_per_graph_hybrid(b, gamma=1024, p=12)
     # CPU oracle per graph on the same packed arrays
     rp = b.csr.row_ptr.cpu().numpy().astype(np.int64)
     col = b.csr.col_idx.cpu().numpy()
@@ -338,6 +339,18 @@ def run_batch_extra(args, reps=10):
         "plain_ms_total": plain_s * 1e3, "plain_ms_per_graph": plain_s * 1e3 / b.n_graphs,
         "speedup_vs_plain": plain_s * 1e3 / ms,
         "cpu_oracle_s": cpu_s, "cpu_threads": len(os.sched_getaffinity(0)),
+        "plain_hybrid": {
+            "rule": "per-graph gpu_hybrid_triangle_count, p = 12, gamma = theta = 1024, "
+                    "compact sketches built once over the batch",
+            "ms_total": hyb_s * 1e3, "sketch_build_ms": hyb_build_s * 1e3,
+            "rel_err_total": gpu_hybrid.relative_error(float(hyb.sum()), int(cpu_counts.sum())),
+            "max_rel_err_per_graph": float(max(
+                (gpu_hybrid.relative_error(float(e), int(c))
+                 for e, c in zip(hyb, cpu_counts) if c > 0), default=0.0)),
+            "graphs_without_triangles_estimated_nonzero": int(np.sum((cpu_counts == 0)
+                                                                     & (hyb != 0))),
+            "speedup_batched_vs_plain_hybrid": hyb_s * 1e3 / ms,
+        },
         "parity_batched_vs_cpu": bool(np.array_equal(counts, cpu_counts)),
         "parity_plain_vs_cpu": bool(np.array_equal(plain, cpu_counts)),
         "tuned_params": params, "autotune_evals": len(table),
@@ -397,7 +410,7 @@ def run_router_extra(reps=3, p=12, theta=128, budget=0.005):
         return {"ms": ms, "rel_err": gpu_hybrid.relative_error(est, truth),
                 "coverage": int(s[0].item()) / csr.n_oriented}
 
-    out["fixed_threshold"] = {f"gamma{g}": fixed(g) for g in (64, 128, 256)}
+    out["fixed_threshold"] = {f"gamma{g}": fixed(g) for g in (64, 128, 256, 512, 1024)}
 
     def learned(gate):
         m = router.RouterModel(params, norm, thr, p, 8, theta, min_cost_degree=gate)
@@ -414,6 +427,9 @@ def run_router_extra(reps=3, p=12, theta=128, budget=0.005):
     ok_learned = [v["ms"] for v in out["learned"].values() if v["rel_err"] <= 0.01]
     if ok_fixed and ok_learned:
         out["speedup_learned_vs_fixed_at_1pct"] = min(ok_fixed) / min(ok_learned)
+    if ok_learned:
+        out["speedup_learned_vs_fastest_fixed"] = (
+            min(v["ms"] for v in out["fixed_threshold"].values()) / min(ok_learned))
     return out
 
 

@@ -235,3 +235,40 @@ def plain_per_graph_counts(b: BatchedCSR, params: Optional[dict] = None):
             out[g], _ = gpu_hybrid.gpu_exact_triangle_count(b.csr.row_ptr, b.csr.col_idx,
                                                             eu[lo:hi], ev[lo:hi], params=params)
     return out, time.perf_counter() - t0
+
+
+def plain_per_graph_hybrid(b: BatchedCSR, gamma: int = 1024, p: int = 12):
+    """The plain fixed-threshold hybrid, one graph at a time.
+
+    One sketch build over the batch (vertices with forward degree >= gamma,
+    compact register slots; the reference experiment passes theta = gamma,
+    experiment/exp_gpu.py:556-557), then one ``gpu_hybrid_triangle_count`` call
+    per graph (src/gpu_hybrid.py:357-431: split exact and HLL kernels,
+    synchronised).  Returns ``(estimates float64[G], seconds, build_seconds)``.
+    """
+    from . import gpu_hll
+
+    t = _lib.require_device()
+    deg = b.csr.degrees
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    sk = deg >= gamma
+    n_sk = int(sk.sum().item())
+    slot = t.full_like(deg, -1)
+    slot[sk] = t.arange(n_sk, dtype=t.int32, device=deg.device)
+    regs = t.zeros(max(n_sk, 1) << p, dtype=t.uint8, device=deg.device)
+    has = t.zeros_like(deg, dtype=t.uint8)
+    gpu_hll.build_sketches(b.csr.row_ptr, b.csr.col_idx, b.csr.n_nodes, gamma, p, regs, has,
+                           slot=slot, accumulate=False)
+    t.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    eu, ev = b.csr.edge_u, b.csr.edge_v
+    out = np.zeros(b.n_graphs, dtype=np.float64)
+    t0 = time.perf_counter()
+    for g in range(b.n_graphs):
+        lo, hi = int(b.edge_off[g]), int(b.edge_off[g + 1])
+        if hi > lo:
+            out[g], _, _ = gpu_hybrid.gpu_hybrid_triangle_count(
+                b.csr.row_ptr, b.csr.col_idx, eu[lo:hi], ev[lo:hi], deg, regs, has, gamma, p,
+                slot=slot)
+    return out, build_s + time.perf_counter() - t0, build_s

@@ -1,8 +1,10 @@
 // Stable LSD radix sort of 64-bit words (8-bit digits), reduce-then-scan form:
 // per pass one histogram kernel, one exclusive scan of the digit-major
-// (digit, tile) count table, and one scatter kernel.  Inside a tile the four
-// wavefronts own consecutive quarters, and a wavefront ranks its keys round
-// by round with ballot "match" masks, so the order of equal digits is kept.
+// (digit, tile) count table, and one scatter kernel.  A tile is 8192 keys
+// for a 512-thread workgroup: its eight wavefronts own consecutive eighths,
+// and a wavefront ranks its keys round by round with ballot "match" masks, so
+// the order of equal digits is kept.  With uniform digits a tile writes runs
+// of 32 keys (256 B) per digit.
 #include <algorithm>
 
 #include "tc_common.hpp"
@@ -12,9 +14,11 @@ namespace tc {
 constexpr int kRadixBits = 8;
 constexpr int kRadixDigits = 1 << kRadixBits;
 constexpr int kRadixRounds = 16;                              // keys per lane
-constexpr int kRadixWaveKeys = kWave * kRadixRounds;         // 1024
-constexpr int kRadixTile = kRadixWaveKeys * (kBlock / kWave);  // 4096
-static_assert(kBlock == kRadixDigits, "the scatter assigns one digit per thread");
+constexpr int kRadixBlock = 512;                              // threads per tile
+constexpr int kRadixWaves = kRadixBlock / kWave;              // 8
+constexpr int kRadixWaveKeys = kWave * kRadixRounds;          // 1024
+constexpr int kRadixTile = kRadixWaveKeys * kRadixWaves;      // 8192
+static_assert(kRadixBlock >= kRadixDigits, "the scatter assigns one digit per thread");
 
 // Exclusive scan of one int per thread over the workgroup (*total = sum).
 __device__ __forceinline__ int block_exclusive_scan_int(int x, int *wave_tot, int *total) {
@@ -24,7 +28,7 @@ __device__ __forceinline__ int block_exclusive_scan_int(int x, int *wave_tot, in
   __syncthreads();
   int before = 0, sum = 0;
 #pragma unroll
-  for (int i = 0; i < kBlock / kWave; ++i) {
+  for (int i = 0; i < kRadixWaves; ++i) {
     const int t = wave_tot[i];
     before += (i < w) ? t : 0;
     sum += t;
@@ -33,11 +37,11 @@ __device__ __forceinline__ int block_exclusive_scan_int(int x, int *wave_tot, in
   return before + inc - x;
 }
 
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kRadixBlock)
 k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift, int dmask, int64_t ntiles,
              int32_t *__restrict__ counts) {
   __shared__ int hist[kRadixDigits];
-  for (int i = threadIdx.x; i < kRadixDigits; i += kBlock) hist[i] = 0;
+  for (int i = threadIdx.x; i < kRadixDigits; i += kRadixBlock) hist[i] = 0;
   __syncthreads();
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
@@ -48,25 +52,25 @@ k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift, int dmask,
     if (i < n) atomicAdd(&hist[(int)(keys[i] >> shift) & dmask], 1);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < kRadixDigits; d += kBlock)
+  for (int d = threadIdx.x; d < kRadixDigits; d += kRadixBlock)
     counts[(int64_t)d * ntiles + blockIdx.x] = hist[d];
 }
 
 // Scatter of one tile: keys are ranked (stable) inside the tile, staged in
 // LDS in digit order, then written out digit run by digit run, so that
 // consecutive threads write consecutive addresses of the output.
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kRadixBlock)
 k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int64_t n, int shift,
                 int dmask, int64_t ntiles, const int64_t *__restrict__ offs) {
-  __shared__ int wpos[kBlock / kWave][kRadixDigits];  // counts, then running positions
+  __shared__ int wpos[kRadixWaves][kRadixDigits];  // counts, then running positions
   __shared__ int tstart[kRadixDigits];                // first tile slot of each digit
   __shared__ int64_t gstart[kRadixDigits];            // first output slot of each digit
   __shared__ uint64_t stage[kRadixTile];
-  __shared__ int wave_tot[kBlock / kWave];
+  __shared__ int wave_tot[kRadixWaves];
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int i = threadIdx.x; i < (kBlock / kWave) * kRadixDigits; i += kBlock)
+  for (int i = threadIdx.x; i < kRadixWaves * kRadixDigits; i += kRadixBlock)
     (&wpos[0][0])[i] = 0;
   const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
   const int tile_n = (int)min<int64_t>(kRadixTile, n - t0);
@@ -96,26 +100,29 @@ k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int
   }
   __syncthreads();
 
-  // thread d: the tile's digit-d slots start after all smaller digits; the
-  // wavefronts' runs follow each other inside a digit (wavefront order is
+  // thread d < 256: the tile's digit-d slots start after all smaller digits;
+  // the wavefronts' runs follow each other inside a digit (wavefront order is
   // input order)
   {
-    const int d = threadIdx.x;  // kBlock == kRadixDigits
-    int c[kBlock / kWave], tot = 0;
+    const int d = threadIdx.x;
+    const bool has_digit = d < kRadixDigits;
+    int c[kRadixWaves], tot = 0;
 #pragma unroll
-    for (int ww = 0; ww < kBlock / kWave; ++ww) {
-      c[ww] = wpos[ww][d];
+    for (int ww = 0; ww < kRadixWaves; ++ww) {
+      c[ww] = has_digit ? wpos[ww][d] : 0;
       tot += c[ww];
     }
     int dummy;
     const int st = block_exclusive_scan_int(tot, wave_tot, &dummy);
-    tstart[d] = st;
-    gstart[d] = offs[(int64_t)d * ntiles + blockIdx.x];
-    int p = st;
+    if (has_digit) {
+      tstart[d] = st;
+      gstart[d] = offs[(int64_t)d * ntiles + blockIdx.x];
+      int p = st;
 #pragma unroll
-    for (int ww = 0; ww < kBlock / kWave; ++ww) {
-      wpos[ww][d] = p;
-      p += c[ww];
+      for (int ww = 0; ww < kRadixWaves; ++ww) {
+        wpos[ww][d] = p;
+        p += c[ww];
+      }
     }
   }
   __syncthreads();
@@ -131,7 +138,7 @@ k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int
   }
   __syncthreads();
 
-  for (int i = threadIdx.x; i < tile_n; i += kBlock) {
+  for (int i = threadIdx.x; i < tile_n; i += kRadixBlock) {
     const uint64_t key = stage[i];
     const int d = (int)(key >> shift) & dmask;
     out[gstart[d] + (i - tstart[d])] = key;
@@ -154,12 +161,12 @@ int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
   for (int shift = bit_lo; shift < bit_hi; shift += kRadixBits) {
     // digits never reach past bit_hi, so higher bits do not perturb the order
     const int dmask = (1 << std::min(kRadixBits, bit_hi - shift)) - 1;
-    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kRadixBlock), 0, s,
                        (const uint64_t *)src, n, shift, dmask, ntiles, counts.as<int32_t>());
     TC_HIP(hipGetLastError());
     TC_TRY(exclusive_scan<int32_t>(counts.as<int32_t>(), offs.as<int64_t>(),
                                    ntiles * kRadixDigits, nullptr, s));
-    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kRadixBlock), 0, s,
                        (const uint64_t *)src, dst, n, shift, dmask, ntiles,
                        (const int64_t *)offs.as<int64_t>());
     TC_HIP(hipGetLastError());
