@@ -2,18 +2,24 @@
 //
 // T = sum over oriented edges (u -> v) of |N+(u) ∩ N+(v)|, both rows sorted.
 //
-// Pipeline (no host synchronisation; every buffer is stream-ordered):
+// Pipeline (stream-ordered; small inputs read one counter back, see
+// exact_sorted_plan):
 //   1. classify: an edge whose two rows together hold <= light_max ids is
 //      merged by one lane right away.  Every other edge is assigned to its
 //      "center" = the endpoint with the longer row (ties: v); the other
-//      endpoint's row will be streamed against the center's row.
-//   2. counting sort of the heavy edges by center (atomic histogram, scan,
-//      scatter) -> for each center the list of its "others".
-//   3. work items = (center, chunk of <= `chunk` others).
-//   4. persistent heavy kernel: a wavefront pulls an item, builds an LDS
-//      open-addressing hash set of N+(center) (load <= 1/2), then streams the
-//      concatenation of the others' rows with all 64 lanes (flattened across
-//      rows, so short rows do not idle lanes) and probes each id.
+//      endpoint's row will be streamed against the center's row.  Each heavy
+//      edge becomes one packed word (center << 32 | other).
+//   2. LSD radix sort of the packed words on the center bits, then the runs
+//      of equal centers (run starts flagged and compacted) -> for each center
+//      the list of its "others", in edge order.  (The atomic plan, kept as an
+//      option, groups them with a counting sort instead.)
+//   3. work items = (run, chunk of <= `chunk` others), longest center row
+//      first (bucket sort on the row length's bit length).
+//   4. persistent heavy kernels: a wavefront (or a workgroup, for long center
+//      rows) pulls an item, builds an LDS open-addressing hash set of
+//      N+(center), then streams the concatenation of the others' rows with
+//      all 64 lanes (flattened across rows, so short rows do not idle lanes),
+//      16 ids in flight per lane, and probes each id.
 // Streaming the shorter row into a hash of the longer one costs
 // sum_e min(d+u, d+v) probes, against sum_e (d+u + d+v) for a merge.
 #include <algorithm>
