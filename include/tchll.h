@@ -143,7 +143,11 @@ typedef struct tc_exact_params {
   int block_min_bits; /* centers with bit_length(deg+) in [block_min_bits, 12]
                          are taken by whole workgroups over one joined LDS
                          table: 8..13 (13 = none), 0 = 10 (deg+ >= 512)       */
-  int reserved[2];
+  int pack_ids;      /* sorted plan: 1 = packed words carry the other's id;
+                         0 = they carry its row start and length when the
+                         fields fit (no row-offset reads in the intersection
+                         kernel), else its id                                 */
+  int reserved[1];
 } tc_exact_params;
 
 /* How heavy edges are grouped by center before the intersection kernel:

@@ -46,9 +46,9 @@ def test_clique_beyond_lds_table(device):
     assert _count(csr) == math.comb(n, 3)
 
 
-@pytest.mark.parametrize("plan", ["sorted", "atomic"])
+@pytest.mark.parametrize("plan,pack_ids", [("sorted", False), ("sorted", True), ("atomic", False)])
 @pytest.mark.parametrize("n", [2100, 4400])
-def test_big_centers_mixed_with_small(device, plan, n):
+def test_big_centers_mixed_with_small(device, plan, pack_ids, n):
     # a K_n block plus an R-MAT graph on disjoint ids: forward rows of
     # 1024..4095 ids take the joined workgroup table, rows of 4096 ids and up
     # (n = 4400) the 128 KiB workgroup kernel, the R-MAT rows one wavefront
@@ -57,7 +57,8 @@ def test_big_centers_mixed_with_small(device, plan, n):
     other = rmat(12, 16, seed=8).astype(np.int64) + n
     pairs = np.concatenate([clique, other]).astype(np.int32)
     csr = gpu_graph.build_csr_gpu(pairs)
-    assert _count(csr, params={"plan": plan}) == math.comb(n, 3) + _truth(other)
+    assert (_count(csr, params={"plan": plan, "pack_ids": pack_ids})
+            == math.comb(n, 3) + _truth(other))
 
 
 def test_rmat14_config(device):
@@ -72,18 +73,21 @@ def test_rmat14_config(device):
     assert got == truth and secs > 0
 
 
-@pytest.mark.parametrize("plan", ["sorted", "atomic"])
+@pytest.mark.parametrize("plan,pack_ids", [("sorted", False), ("sorted", True), ("atomic", False)])
 @pytest.mark.parametrize("light_max,chunk,wpc,unroll,blk", [(0, 1, 4, 1, 8), (2, 7, 8, 2, 13),
                                                             (16, 256, 16, 4, 0),
                                                             (64, 32, 32, 8, 10),
                                                             (32, 256, 16, 16, 10),
                                                             (10**6, 256, 16, 4, 9)])
-def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc, unroll, blk, plan):
+def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc, unroll, blk, plan,
+                                          pack_ids):
+    # pack_ids: the sorted plan's packed words carry the other's id instead of
+    # its row start and length (the layout large graphs fall back to)
     pairs = rmat(12, 16, seed=7)
     truth = _truth(pairs)
     csr = gpu_graph.build_csr_gpu(pairs)
     params = {"light_max": light_max, "chunk": chunk, "waves_per_cu": wpc, "unroll": unroll,
-              "plan": plan, "block_min_bits": blk}
+              "plan": plan, "block_min_bits": blk, "pack_ids": pack_ids}
     assert _count(csr, params=params) == truth
 
 

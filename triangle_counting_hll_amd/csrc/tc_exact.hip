@@ -8,7 +8,9 @@
 //      merged by one lane right away.  Every other edge is assigned to its
 //      "center" = the endpoint with the longer row (ties: v); the other
 //      endpoint's row will be streamed against the center's row.  Each heavy
-//      edge becomes one packed word (center << 32 | other).
+//      edge becomes one packed word: the center in the top bits, below it the
+//      other's row (start, length), or the other's id when the row fields do
+//      not fit (PackLayout).
 //   2. LSD radix sort of the packed words on the center bits, then the runs
 //      of equal centers (run starts flagged and compacted) -> for each center
 //      the list of its "others", in edge order.  (The atomic plan, kept as an
@@ -240,21 +242,59 @@ __device__ __forceinline__ uint32_t global_probe(const int32_t *__restrict__ col
   return (lo < cb + dc && col[lo] == x) ? 1u : 0u;
 }
 
-// The "others" of a work item, in either planning layout: a plain id array
+// Layout of the sorted plan's packed words.  The center id fills the top
+// 64 - shift bits.  Below it, either
+//   row layout: the other's row start in bits [lbits, shift) and its length
+//               in bits [0, lbits), so the intersection kernel reads no row
+//               offsets for the streamed rows, or
+//   id layout:  the other's id in the low 32 bits.
+// The row layout needs lbits > 0 (host: the start field holds every offset
+// into col[]) and every forward degree below 2^lbits; the degree bound is
+// measured on the device (k_max_degree) and read by each kernel, so the
+// choice stays stream-ordered.
+struct PackLayout {
+  int shift;
+  int lbits;                               // 0: id layout only
+  const unsigned long long *max_deg;       // device max forward degree
+  __device__ __forceinline__ bool rows() const {
+    return lbits > 0 && *max_deg < (1ull << lbits);
+  }
+};
+
+// The "others" of a work item, in either planning form: a plain id array
 // indexed through per-center offsets (item.x = center), or the sorted packed
-// (center << 32 | other) words indexed through run offsets (item.x = run).
+// words indexed through run offsets (item.x = run).  row() gives the row of
+// the k-th other; resolve() is called once per kernel before any row().
 struct OthersIds {
   const int32_t *p;
-  __device__ __forceinline__ int32_t operator()(int64_t k) const { return p[k]; }
+  __device__ __forceinline__ void resolve() {}
+  template <class I>
+  __device__ __forceinline__ void row(Rows<I> R, int64_t k, int64_t &st, int &len) const {
+    const int32_t o = p[k];
+    st = R.begin(o);
+    len = (int)(R.end(o) - st);
+  }
   __device__ __forceinline__ int32_t center(int item_x, int64_t) const { return item_x; }
 };
 struct OthersPacked {
   const uint64_t *p;
-  __device__ __forceinline__ int32_t operator()(int64_t k) const {
-    return (int32_t)(uint32_t)p[k];
+  PackLayout L;
+  bool rows_layout = false;
+  __device__ __forceinline__ void resolve() { rows_layout = L.rows(); }
+  template <class I>
+  __device__ __forceinline__ void row(Rows<I> R, int64_t k, int64_t &st, int &len) const {
+    const uint64_t w = p[k];
+    if (rows_layout) {
+      st = (int64_t)((w >> L.lbits) & ((1ull << (L.shift - L.lbits)) - 1));
+      len = (int)(w & ((1ull << L.lbits) - 1));
+    } else {
+      const int32_t o = (int32_t)(uint32_t)w;
+      st = R.begin(o);
+      len = (int)(R.end(o) - st);
+    }
   }
   __device__ __forceinline__ int32_t center(int, int64_t ob) const {
-    return (int32_t)(p[ob] >> 32);
+    return (int32_t)(p[ob] >> L.shift);
   }
 };
 
@@ -293,11 +333,7 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
     const int64_t k = base + lane;
     int len = 0;
     int64_t st = 0;
-    if (k < oe) {
-      const int32_t o = others(k);
-      st = R.begin(o);
-      len = (int)(R.end(o) - st);
-    }
+    if (k < oe) others.row(R, k, st, len);
     const int inc = wave_inclusive_scan(len);
     const int total = __shfl(inc, kWave - 1);
     seg_end[lane] = inc;
@@ -416,6 +452,7 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
   __shared__ int64_t s_ticket;
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
+  others.resolve();
   int4 *tbl = tbl_all[w];
   int *seg_end = seg_end_all[w];
   int64_t *seg_base = seg_base_all[w];
@@ -553,6 +590,7 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restric
   __shared__ int64_t s_item;
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
+  others.resolve();
   const int64_t n_first = iptr[*n_first_runs_p];
   int *tbl_i = reinterpret_cast<int *>(tbl);
   uint64_t acc_total = 0;
@@ -619,20 +657,38 @@ static auto heavy_kernel_t(int unroll) -> decltype(&k_heavy<I, 4, O, kNarrow>) {
   }
 }
 
-// The entry points do not receive the length of col[], so the bound comes
-// from the allocation holding it: every in-bounds id lies between col and the
-// allocation's end, and the narrow form is taken when that span is at most
-// 4 GiB.  A failed query (memory the runtime does not track) keeps the
-// 64-bit form.
-static bool ids_fit_narrow(const int32_t *col) {
+// The entry points do not receive the length of col[], so bounds on its
+// offsets come from the allocation holding it: every in-bounds id lies
+// between col and the allocation's end.  Returns the number of int32 slots
+// in that span, or 0 when the runtime does not track the memory (then the
+// 64-bit id addressing and the id layout of the packed words are used).
+static int64_t col_span_ids(const int32_t *col) {
   hipDeviceptr_t base = nullptr;
   size_t size = 0;
   if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)col) != hipSuccess) {
     (void)hipGetLastError();
-    return false;
+    return 0;
   }
   const uintptr_t b = (uintptr_t)base, p = (uintptr_t)col;
-  return p >= b && b + size - p <= (uintptr_t(1) << 32);
+  if (p < b || p >= b + size) return 0;
+  return (int64_t)((b + size - p) / sizeof(int32_t));
+}
+
+// narrow id addressing: every byte offset into col[] fits 32 bits
+static bool narrow_ids(int64_t span) { return span > 0 && span <= (int64_t(1) << 30); }
+
+__global__ void __launch_bounds__(kBlock)
+k_max_degree(const int64_t *__restrict__ rp64, const int32_t *__restrict__ rp32, int64_t n,
+             unsigned long long *__restrict__ max_deg) {
+  unsigned long long m = 0;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t d = rp64 ? rp64[u + 1] - rp64[u] : (int64_t)rp32[u + 1] - rp32[u];
+    m = max(m, (unsigned long long)d);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned long long)__shfl_xor(m, off, kWave));
+  if (lane_id() == 0 && m) atomicMax(max_deg, m);
 }
 
 template <class I, class O>
@@ -655,11 +711,13 @@ template <class I>
 __global__ void __launch_bounds__(kBlock)
 k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict__ eu,
                 const int32_t *__restrict__ ev, int64_t E, const uint8_t *__restrict__ skip,
-                int light_max, uint64_t sentinel, uint64_t *__restrict__ packed,
+                int light_max, PackLayout L, uint64_t *__restrict__ packed,
                 const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count,
-                unsigned long long *__restrict__ n_heavy) {
+                unsigned long long *__restrict__ n_heavy, int64_t n) {
   uint64_t acc = 0, nh = 0;
   const int lane = lane_id();
+  const bool rows_layout = L.rows();
+  const uint64_t sentinel = (uint64_t)n << L.shift;
   // the loop bound is uniform per wavefront, so every lane stays active for
   // the cross-lane aggregation below
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < E;
@@ -680,7 +738,11 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
           } else {
             const bool cv = dv >= du;
             const uint32_t c = (uint32_t)(cv ? v : u), o = (uint32_t)(cv ? u : v);
-            word = ((uint64_t)c << 32) | o;
+            word = (uint64_t)c << L.shift;
+            if (rows_layout)
+              word |= ((uint64_t)(cv ? ub : vb) << L.lbits) | (uint64_t)(cv ? du : dv);
+            else
+              word |= o;
             ++nh;
           }
         }
@@ -726,11 +788,11 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
 }
 
 __global__ void k_run_flags(const uint64_t *__restrict__ s, const unsigned long long *n_heavy_p,
-                            int64_t E, uint8_t *__restrict__ flag) {
+                            int64_t E, int shift, uint8_t *__restrict__ flag) {
   const int64_t H = (int64_t)*n_heavy_p;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
        i += (int64_t)gridDim.x * blockDim.x)
-    flag[i] = (i < H && (i == 0 || (s[i] >> 32) != (s[i - 1] >> 32))) ? 1 : 0;
+    flag[i] = (i < H && (i == 0 || (s[i] >> shift) != (s[i - 1] >> shift))) ? 1 : 0;
 }
 
 // run offsets (int64, R + 1 entries) and the bucket key of each run's center
@@ -738,14 +800,14 @@ template <class I>
 __global__ void k_run_offsets(Rows<I> R, const uint64_t *__restrict__ s,
                               const int32_t *__restrict__ run_idx, const int64_t *__restrict__ R_p,
                               const unsigned long long *__restrict__ n_heavy_p, int64_t max_runs,
-                              int64_t *__restrict__ run_ptr, uint8_t *__restrict__ key) {
+                              int shift, int64_t *__restrict__ run_ptr, uint8_t *__restrict__ key) {
   const int64_t nr = *R_p;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= max_runs;
        r += (int64_t)gridDim.x * blockDim.x) {
     if (r < nr) {
       const int64_t b = run_idx[r];
       run_ptr[r] = b;
-      const uint64_t d = (uint64_t)R.deg((int64_t)(s[b] >> 32));
+      const uint64_t d = (uint64_t)R.deg((int64_t)(s[b] >> shift));
       key[r] = (uint8_t)(kCenterBuckets - (64 - __clzll((long long)d)));
     } else {
       if (r == nr) run_ptr[r] = (int64_t)*n_heavy_p;
@@ -774,7 +836,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              const int32_t *eu, const int32_t *ev, int64_t E,
                              const uint8_t *skip, const int32_t *graph_of,
                              unsigned long long *count, int light_max, int chunk, int wpc,
-                             int unroll, int blk_bits, hipStream_t s) {
+                             int unroll, int blk_bits, bool pack_ids, hipStream_t s) {
   Scratch eu_buf;
   if (!eu) {
     TC_TRY(eu_buf.alloc(sizeof(int32_t) * E, s));
@@ -788,6 +850,13 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   }
   int key_bits = 1;
   while ((int64_t(1) << key_bits) <= n) ++key_bits;  // center ids and the sentinel n
+  // packed-word layout: the center in the top key_bits bits; the row start
+  // field must hold every offset into col[] (at most `span`)
+  const int64_t span = col_span_ids(col);
+  int start_bits = 1;
+  while (start_bits < 63 && (int64_t(1) << start_bits) <= span) ++start_bits;
+  const int shift = 64 - key_bits;
+  const int lbits = (!pack_ids && span > 0 && shift - start_bits >= 1) ? shift - start_bits : 0;
   const int64_t max_runs = n;                         // at most one run per center
   Scratch packed, tmp, flags, run_idx, n_runs, run_ptr, rkey, perm, starts, nitems, iptr,
       items, ctr;
@@ -804,22 +873,30 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   TC_TRY(iptr.alloc(sizeof(int64_t) * (max_runs + 1), s));
   const int64_t max_items = max_runs + E / chunk + 1;
   TC_TRY(items.alloc(sizeof(int2) * max_items, s));
-  TC_TRY(ctr.alloc(4 * sizeof(unsigned long long), s));
+  TC_TRY(ctr.alloc(5 * sizeof(unsigned long long), s));
   unsigned long long *next_item = ctr.as<unsigned long long>();
   unsigned long long *n_heavy = next_item + 1;
+  unsigned long long *max_deg = next_item + 4;
+  const PackLayout layout{shift, lbits, max_deg};
 
   const unsigned gr = grid_for(max_runs + 1, kBlock, 16384);
   const unsigned ge = grid_for(E, kBlock, 16384);
   unsigned long long *next_big = next_item + 2;
   unsigned long long *next_blk = next_item + 3;
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, next_item, 4, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, next_item, 5, 0ull);
+  if (lbits > 0) {
+    hipLaunchKernelGGL(k_max_degree, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, s,
+                       row_ptr_64 ? (const int64_t *)row_ptr : nullptr,
+                       row_ptr_64 ? nullptr : (const int32_t *)row_ptr, n, max_deg);
+    TC_HIP(hipGetLastError());
+  }
   TC_DISPATCH_ROWS(row_ptr_64, I, {
     Rows<I> R{(const I *)row_ptr};
     {
       StageTimer _timer(TC_STAGE_CLASSIFY, s);
       hipLaunchKernelGGL(k_classify_pack<I>, dim3(ge), dim3(kBlock), 0, s, R, col, eu, ev, E,
-                         skip, light_max, (uint64_t)n << 32, packed.as<uint64_t>(), graph_of,
-                         count, n_heavy);
+                         skip, light_max, layout, packed.as<uint64_t>(), graph_of, count, n_heavy,
+                         n);
     }
     TC_HIP(hipGetLastError());
     // Small inputs are launch-bound: when classify found no heavy edge, the
@@ -837,17 +914,16 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
       }
     }
     uint64_t *sorted = nullptr;
-    TC_TRY(radix_sort_u64(packed.as<uint64_t>(), tmp.as<uint64_t>(), E, 32, 32 + key_bits,
-                          &sorted, s));
+    TC_TRY(radix_sort_u64(packed.as<uint64_t>(), tmp.as<uint64_t>(), E, shift, 64, &sorted, s));
     hipLaunchKernelGGL(k_run_flags, dim3(ge), dim3(kBlock), 0, s, (const uint64_t *)sorted,
-                       (const unsigned long long *)n_heavy, E, flags.as<uint8_t>());
+                       (const unsigned long long *)n_heavy, E, shift, flags.as<uint8_t>());
     TC_HIP(hipGetLastError());
     TC_TRY(compact_indices(flags.as<uint8_t>(), E, run_idx.as<int32_t>(), n_runs.as<int64_t>(),
                            s));
     hipLaunchKernelGGL(k_run_offsets<I>, dim3(gr), dim3(kBlock), 0, s, R,
                        (const uint64_t *)sorted, (const int32_t *)run_idx.as<int32_t>(),
                        (const int64_t *)n_runs.as<int64_t>(), (const unsigned long long *)n_heavy,
-                       max_runs, run_ptr.as<int64_t>(), rkey.as<uint8_t>());
+                       max_runs, shift, run_ptr.as<int64_t>(), rkey.as<uint8_t>());
     TC_HIP(hipGetLastError());
     // runs ordered longest center row first
     TC_TRY(bucket_sort_u8(rkey.as<uint8_t>(), max_runs, kCenterBuckets, perm.as<int32_t>(),
@@ -864,7 +940,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                        (const int64_t *)iptr.as<int64_t>(), n_sorted_runs, items.as<int2>());
     TC_HIP(hipGetLastError());
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
-    const bool narrow = ids_fit_narrow(col);
+    const bool narrow = narrow_ids(span);
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
       const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
@@ -872,13 +948,15 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
       const auto kbig = heavy_big_kernel<I, OthersPacked>(unroll, narrow);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
                          (const int64_t *)run_ptr.as<int64_t>(),
-                         OthersPacked{(const uint64_t *)sorted}, (const int2 *)items.as<int2>(),
+                         OthersPacked{(const uint64_t *)sorted, layout},
+                         (const int2 *)items.as<int2>(),
                          (const int64_t *)iptr.as<int64_t>(), n_huge, chunk, next_big, graph_of,
                          count);
       const auto kfn = heavy_kernel<I, OthersPacked>(unroll, narrow);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R,
                          col, (const int64_t *)run_ptr.as<int64_t>(),
-                         OthersPacked{(const uint64_t *)sorted}, (const int2 *)items.as<int2>(),
+                         OthersPacked{(const uint64_t *)sorted, layout},
+                         (const int2 *)items.as<int2>(),
                          (const int64_t *)(iptr.as<int64_t>() + max_runs),
                          (const int64_t *)iptr.as<int64_t>(), n_huge, n_first, n_big_runs, chunk,
                          next_item, next_blk, graph_of, count);
@@ -907,7 +985,8 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
   if (E <= 0 || n <= 0) return TC_OK;
   if (plan == TC_PLAN_SORTED)
     return exact_sorted_plan(row_ptr, row_ptr_64, col, n, eu, ev, E, skip, graph_of, count,
-                             light_max, chunk, wpc, unroll, blk_bits, s);
+                             light_max, chunk, wpc, unroll, blk_bits,
+                             params && params->pack_ids != 0, s);
 
   Scratch eu_buf;
   if (!eu) {
@@ -970,7 +1049,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                        (const int64_t *)iptr.as<int64_t>(), n_cent, items.as<int2>());
     TC_HIP(hipGetLastError());
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
-    const bool narrow = ids_fit_narrow(col);
+    const bool narrow = narrow_ids(col_span_ids(col));
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
       const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);

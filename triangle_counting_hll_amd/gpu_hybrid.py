@@ -78,7 +78,8 @@ def _exact_params(params: Optional[dict]):
     return _lib.ExactParams(int(params.get("light_max", 0)), int(params.get("chunk", 0)),
                             int(params.get("waves_per_cu", 0)), int(params.get("unroll", 0)),
                             _lib.PLANS[plan] if isinstance(plan, str) else int(plan),
-                            int(params.get("block_min_bits", 0)))
+                            int(params.get("block_min_bits", 0)),
+                            int(bool(params.get("pack_ids", False))))
 
 
 def exact_count(row_ptr, col_idx, edge_u=None, edge_v=None, count=None, params=None,
