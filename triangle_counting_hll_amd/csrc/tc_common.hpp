@@ -231,9 +231,13 @@ int bucket_sort_u8(const uint8_t *key, int64_t n, int nbuckets, int32_t *perm, i
 
 // Stable LSD radix sort of 64-bit words by their bits [bit_lo, bit_hi), 8 bits
 // per pass.  `a` holds the input; `b` is a buffer of the same size.  On return
-// *result points at whichever of the two holds the sorted words.
+// *result points at whichever of the two holds the sorted words.  With `drop`
+// (host pointer to a key value), the first pass leaves out every word equal to
+// *drop and the sorted output holds only the others; then n_dev (device) must
+// hold their number, which the later passes read (n sizes the launches).
 int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
-                   uint64_t **result, hipStream_t s);
+                   uint64_t **result, hipStream_t s, const uint64_t *drop = nullptr,
+                   const unsigned long long *n_dev = nullptr);
 
 // Stream compaction by flag: idx_out receives the indices i with flag[i] != 0 in
 // increasing order; *d_count (device) receives their number.

@@ -702,10 +702,10 @@ static auto heavy_kernel(int unroll, bool narrow) -> decltype(&k_heavy<I, 4, O, 
 }
 
 // ---------------------------------------------------------------------------
-// Sorted plan: one packed word per edge -- (center << 32 | other) for a heavy
-// edge, (n << 32) for every other edge -- then a radix sort on the center
-// bits groups the others of each center (in edge order) and moves the rest
-// to the end.  No per-edge global atomics.
+// Sorted plan: one packed word per edge -- the center and the other's row (or
+// id) for a heavy edge, the sentinel (n << shift) for every other edge -- then
+// a radix sort on the center bits whose first pass drops the sentinels groups
+// the others of each center (in edge order).  No per-edge global atomics.
 // ---------------------------------------------------------------------------
 template <class I>
 __global__ void __launch_bounds__(kBlock)
@@ -914,7 +914,11 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
       }
     }
     uint64_t *sorted = nullptr;
-    TC_TRY(radix_sort_u64(packed.as<uint64_t>(), tmp.as<uint64_t>(), E, shift, 64, &sorted, s));
+    // the first pass leaves out the light edges' sentinel words; the later
+    // passes sort the n_heavy heavy words only
+    const uint64_t sentinel = (uint64_t)n << shift;
+    TC_TRY(radix_sort_u64(packed.as<uint64_t>(), tmp.as<uint64_t>(), E, shift, 64, &sorted, s,
+                          &sentinel, n_heavy));
     hipLaunchKernelGGL(k_run_flags, dim3(ge), dim3(kBlock), 0, s, (const uint64_t *)sorted,
                        (const unsigned long long *)n_heavy, E, shift, flags.as<uint8_t>());
     TC_HIP(hipGetLastError());

@@ -38,8 +38,10 @@ __device__ __forceinline__ int block_exclusive_scan_int(int x, int *wave_tot, in
 }
 
 __global__ void __launch_bounds__(kRadixBlock)
-k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift, int dmask, int64_t ntiles,
-             int32_t *__restrict__ counts) {
+k_radix_hist(const uint64_t *__restrict__ keys, int64_t n_max,
+             const unsigned long long *__restrict__ n_dev, int drop, uint64_t drop_key, int shift,
+             int dmask, int64_t ntiles, int32_t *__restrict__ counts) {
+  const int64_t n = n_dev ? min(n_max, (int64_t)*n_dev) : n_max;
   __shared__ int hist[kRadixDigits];
   for (int i = threadIdx.x; i < kRadixDigits; i += kRadixBlock) hist[i] = 0;
   __syncthreads();
@@ -49,7 +51,10 @@ k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift, int dmask,
 #pragma unroll 4
   for (int r = 0; r < kRadixRounds; ++r) {
     const int64_t i = base + (int64_t)r * kWave;
-    if (i < n) atomicAdd(&hist[(int)(keys[i] >> shift) & dmask], 1);
+    if (i < n) {
+      const uint64_t key = keys[i];
+      if (!(drop && key == drop_key)) atomicAdd(&hist[(int)(key >> shift) & dmask], 1);
+    }
   }
   __syncthreads();
   for (int d = threadIdx.x; d < kRadixDigits; d += kRadixBlock)
@@ -60,20 +65,23 @@ k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift, int dmask,
 // LDS in digit order, then written out digit run by digit run, so that
 // consecutive threads write consecutive addresses of the output.
 __global__ void __launch_bounds__(kRadixBlock)
-k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int64_t n, int shift,
-                int dmask, int64_t ntiles, const int64_t *__restrict__ offs) {
+k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int64_t n_max,
+                const unsigned long long *__restrict__ n_dev, int drop, uint64_t drop_key,
+                int shift, int dmask, int64_t ntiles, const int64_t *__restrict__ offs) {
+  const int64_t n = n_dev ? min(n_max, (int64_t)*n_dev) : n_max;
+  if ((int64_t)blockIdx.x * kRadixTile >= n) return;  // tile past the device count
   __shared__ int wpos[kRadixWaves][kRadixDigits];  // counts, then running positions
   __shared__ int tstart[kRadixDigits];                // first tile slot of each digit
   __shared__ int64_t gstart[kRadixDigits];            // first output slot of each digit
   __shared__ uint64_t stage[kRadixTile];
   __shared__ int wave_tot[kRadixWaves];
+  __shared__ int s_kept;                              // keys of the tile that are sorted
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int i = threadIdx.x; i < kRadixWaves * kRadixDigits; i += kRadixBlock)
     (&wpos[0][0])[i] = 0;
   const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
-  const int tile_n = (int)min<int64_t>(kRadixTile, n - t0);
   const int64_t base = t0 + (int64_t)w * kRadixWaveKeys + lane;
   uint64_t k[kRadixRounds];
 #pragma unroll
@@ -87,7 +95,7 @@ k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int
   uint64_t peer[kRadixRounds];
 #pragma unroll
   for (int r = 0; r < kRadixRounds; ++r) {
-    const bool valid = base + (int64_t)r * kWave < n;
+    const bool valid = base + (int64_t)r * kWave < n && !(drop && k[r] == drop_key);
     const int d = (int)(k[r] >> shift) & dmask;
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -112,8 +120,7 @@ k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int
       c[ww] = has_digit ? wpos[ww][d] : 0;
       tot += c[ww];
     }
-    int dummy;
-    const int st = block_exclusive_scan_int(tot, wave_tot, &dummy);
+    const int st = block_exclusive_scan_int(tot, wave_tot, &s_kept);
     if (has_digit) {
       tstart[d] = st;
       gstart[d] = offs[(int64_t)d * ntiles + blockIdx.x];
@@ -138,7 +145,8 @@ k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int
   }
   __syncthreads();
 
-  for (int i = threadIdx.x; i < tile_n; i += kRadixBlock) {
+  const int tile_kept = s_kept;
+  for (int i = threadIdx.x; i < tile_kept; i += kRadixBlock) {
     const uint64_t key = stage[i];
     const int d = (int)(key >> shift) & dmask;
     out[gstart[d] + (i - tstart[d])] = key;
@@ -146,7 +154,8 @@ k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int
 }
 
 int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
-                   uint64_t **result, hipStream_t s) {
+                   uint64_t **result, hipStream_t s, const uint64_t *drop,
+                   const unsigned long long *n_dev) {
   *result = a;
   if (n <= 1 || bit_hi <= bit_lo) return TC_OK;
   const int64_t ntiles = (n + kRadixTile - 1) / kRadixTile;
@@ -161,14 +170,20 @@ int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
   for (int shift = bit_lo; shift < bit_hi; shift += kRadixBits) {
     // digits never reach past bit_hi, so higher bits do not perturb the order
     const int dmask = (1 << std::min(kRadixBits, bit_hi - shift)) - 1;
+    // the first pass reads all n words (dropping *drop); later ones the kept count
+    const bool first = shift == bit_lo;
+    const int drop_now = (first && drop) ? 1 : 0;
+    const uint64_t drop_key = drop ? *drop : 0;
+    const unsigned long long *nd = first ? nullptr : n_dev;
     hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kRadixBlock), 0, s,
-                       (const uint64_t *)src, n, shift, dmask, ntiles, counts.as<int32_t>());
+                       (const uint64_t *)src, n, nd, drop_now, drop_key, shift, dmask, ntiles,
+                       counts.as<int32_t>());
     TC_HIP(hipGetLastError());
     TC_TRY(exclusive_scan<int32_t>(counts.as<int32_t>(), offs.as<int64_t>(),
                                    ntiles * kRadixDigits, nullptr, s));
     hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kRadixBlock), 0, s,
-                       (const uint64_t *)src, dst, n, shift, dmask, ntiles,
-                       (const int64_t *)offs.as<int64_t>());
+                       (const uint64_t *)src, dst, n, nd, drop_now, drop_key, shift, dmask,
+                       ntiles, (const int64_t *)offs.as<int64_t>());
     TC_HIP(hipGetLastError());
     std::swap(src, dst);
   }
