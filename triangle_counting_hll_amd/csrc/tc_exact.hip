@@ -242,22 +242,40 @@ __device__ __forceinline__ uint32_t global_probe(const int32_t *__restrict__ col
   return (lo < cb + dc && col[lo] == x) ? 1u : 0u;
 }
 
+// Row statistics measured on the device before a count (k_row_stats):
+// stats[0] = the largest forward degree, stats[1] = row_ptr[n], the number of
+// ids in col[] that the rows reach.  Every choice below is made from them
+// inside the kernels, so it needs no host synchronisation and holds for any
+// memory the arrays live in.
+__device__ __forceinline__ int bit_length_u64(unsigned long long x) {
+  return x ? 64 - __clzll((long long)x) : 0;
+}
+
+// Narrow id addressing: every id position of the rows fits a 32-bit byte
+// offset.
+__device__ __forceinline__ bool rows_narrow(const unsigned long long *stats) {
+  return stats[1] <= (1ull << 30);
+}
+
 // Layout of the sorted plan's packed words.  The center id fills the top
 // 64 - shift bits.  Below it, either
 //   row layout: the other's row start in bits [lbits, shift) and its length
 //               in bits [0, lbits), so the intersection kernel reads no row
 //               offsets for the streamed rows, or
 //   id layout:  the other's id in the low 32 bits.
-// The row layout needs lbits > 0 (host: the start field holds every offset
-// into col[]) and every forward degree below 2^lbits; the degree bound is
-// measured on the device (k_max_degree) and read by each kernel, so the
-// choice stays stream-ordered.
+// The row layout is used when the start field (lbits = shift - bit_length of
+// row_ptr[n]) leaves at least one bit for lengths and every forward degree is
+// below 2^lbits, unless force_ids.
 struct PackLayout {
   int shift;
-  int lbits;                               // 0: id layout only
-  const unsigned long long *max_deg;       // device max forward degree
+  int force_ids;
+  const unsigned long long *stats;
+  __device__ __forceinline__ int lbits() const {
+    return shift - max(1, bit_length_u64(stats[1]));
+  }
   __device__ __forceinline__ bool rows() const {
-    return lbits > 0 && *max_deg < (1ull << lbits);
+    const int lb = lbits();
+    return !force_ids && lb > 0 && stats[0] < (1ull << lb);
   }
 };
 
@@ -280,13 +298,17 @@ struct OthersPacked {
   const uint64_t *p;
   PackLayout L;
   bool rows_layout = false;
-  __device__ __forceinline__ void resolve() { rows_layout = L.rows(); }
+  int lb = 0;
+  __device__ __forceinline__ void resolve() {
+    rows_layout = L.rows();
+    lb = L.lbits();
+  }
   template <class I>
   __device__ __forceinline__ void row(Rows<I> R, int64_t k, int64_t &st, int &len) const {
     const uint64_t w = p[k];
     if (rows_layout) {
-      st = (int64_t)((w >> L.lbits) & ((1ull << (L.shift - L.lbits)) - 1));
-      len = (int)(w & ((1ull << L.lbits) - 1));
+      st = (int64_t)((w >> lb) & ((1ull << (L.shift - lb)) - 1));
+      len = (int)(w & ((1ull << lb) - 1));
     } else {
       const int32_t o = (int32_t)(uint32_t)w;
       st = R.begin(o);
@@ -414,6 +436,22 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
   return acc;
 }
 
+// stream_others with the id addressing chosen at run time (uniform per
+// kernel): 32-bit offsets when the rows lie within 2^30 ids, else 64-bit.
+template <class I, bool kLds, int U, class O>
+__device__ __forceinline__ uint64_t stream_others_rt(bool narrow, Rows<I> R,
+                                                     const int32_t *__restrict__ col, O others,
+                                                     int64_t ob, int64_t oe,
+                                                     int *__restrict__ seg_end,
+                                                     int64_t *__restrict__ seg_base,
+                                                     const int4 *__restrict__ tbl, int shift,
+                                                     uint32_t mask, int64_t cb, int dc) {
+  return narrow ? stream_others<I, kLds, U, O, true>(R, col, others, ob, oe, seg_end, seg_base,
+                                                     tbl, shift, mask, cb, dc)
+                : stream_others<I, kLds, U, O, false>(R, col, others, ob, oe, seg_end, seg_base,
+                                                      tbl, shift, mask, cb, dc);
+}
+
 // Per-graph counts of a batch: a wavefront keeps per-lane partial sums for
 // the graph of its current work item and adds them to that graph's counter
 // (one wave reduction, one atomic) only when the graph changes and at the end.
@@ -437,10 +475,10 @@ struct GraphAcc {
   }
 };
 
-template <class I, int U, class O, bool kNarrow>
+template <class I, int U, class O>
 __global__ void __launch_bounds__(kBlock)
-k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
-        O others, const int2 *__restrict__ items,
+k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__restrict__ row_stats,
+        const int64_t *__restrict__ cptr, O others, const int2 *__restrict__ items,
         const int64_t *__restrict__ n_items_p, const int64_t *__restrict__ iptr,
         const int64_t *__restrict__ n_huge_runs_p, const int64_t *__restrict__ n_first_runs_p,
         const int64_t *__restrict__ n_big_centers_p, int chunk,
@@ -453,6 +491,7 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   others.resolve();
+  const bool narrow = rows_narrow(row_stats);
   int4 *tbl = tbl_all[w];
   int *seg_end = seg_end_all[w];
   int64_t *seg_base = seg_base_all[w];
@@ -490,8 +529,8 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
       int shift;
       uint32_t mask;
       table_build<kBlkTblInts, true>(tbl_blk, col + cb, dc, shift, mask);
-      uint64_t acc = stream_others<I, true, U, O, kNarrow>(R, col, others, wb, we, seg_end, seg_base,
-                                               &tbl_all[0][0], shift, mask, cb, dc);
+      uint64_t acc = stream_others_rt<I, true, U, O>(narrow, R, col, others, wb, we, seg_end,
+                                                     seg_base, &tbl_all[0][0], shift, mask, cb, dc);
       if (graph_of)
         gacc.add(graph_of[c], acc, count);
       else
@@ -526,11 +565,11 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ 
       int shift;
       uint32_t mask;
       table_build<kTblInts, false>(reinterpret_cast<int *>(tbl), col + cb, dc, shift, mask);
-      acc = stream_others<I, true, U, O, kNarrow>(R, col, others, ob, oe, seg_end, seg_base, tbl, shift, mask,
-                                      cb, dc);
+      acc = stream_others_rt<I, true, U, O>(narrow, R, col, others, ob, oe, seg_end, seg_base,
+                                            tbl, shift, mask, cb, dc);
     } else {
-      acc = stream_others<I, false, U, O, kNarrow>(R, col, others, ob, oe, seg_end, seg_base, tbl, 0, 0, cb,
-                                       dc);
+      acc = stream_others_rt<I, false, U, O>(narrow, R, col, others, ob, oe, seg_end, seg_base,
+                                             tbl, 0, 0, cb, dc);
     }
     if (graph_of)
       gacc.add(graph_of[c], acc, count);
@@ -577,9 +616,10 @@ __global__ void k_edge_supports(Rows<I> R, const int32_t *__restrict__ col,
 // four wavefronts streaming disjoint quarters of the item's others.  Items of
 // these centers come first in the heavy-first order; this kernel takes items
 // [0, n_first) and k_heavy the rest.
-template <class I, int U, class O, bool kNarrow>
+template <class I, int U, class O>
 __global__ void __launch_bounds__(kBlock)
-k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restrict__ cptr,
+k_heavy_big(Rows<I> R, const int32_t *__restrict__ col,
+            const unsigned long long *__restrict__ row_stats, const int64_t *__restrict__ cptr,
             O others, const int2 *__restrict__ items, const int64_t *__restrict__ iptr,
             const int64_t *__restrict__ n_first_runs_p, int chunk,
             unsigned long long *__restrict__ next_item, const int32_t *__restrict__ graph_of,
@@ -591,6 +631,7 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restric
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   others.resolve();
+  const bool narrow = rows_narrow(row_stats);
   const int64_t n_first = iptr[*n_first_runs_p];
   int *tbl_i = reinterpret_cast<int *>(tbl);
   uint64_t acc_total = 0;
@@ -614,12 +655,12 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restric
       int shift;
       uint32_t mask;
       table_build<kBigTblInts, true>(tbl_i, col + cb, dc, shift, mask);
-      acc = stream_others<I, true, U, O, kNarrow>(R, col, others, wb, we, seg_end_all[w], seg_base_all[w],
-                                      tbl, shift, mask, cb, dc);
+      acc = stream_others_rt<I, true, U, O>(narrow, R, col, others, wb, we, seg_end_all[w],
+                                            seg_base_all[w], tbl, shift, mask, cb, dc);
       __syncthreads();  // the table is rebuilt for the next item
     } else {
-      acc = stream_others<I, false, U, O, kNarrow>(R, col, others, wb, we, seg_end_all[w], seg_base_all[w],
-                                       tbl, 0, 0, cb, dc);
+      acc = stream_others_rt<I, false, U, O>(narrow, R, col, others, wb, we, seg_end_all[w],
+                                             seg_base_all[w], tbl, 0, 0, cb, dc);
     }
     if (graph_of)
       gacc.add(graph_of[c], acc, count);
@@ -634,52 +675,31 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col, const int64_t *__restric
   }
 }
 
-// kNarrow = every id position fits a 32-bit byte offset (E <= 2^30)
-template <class I, class O, bool kNarrow>
-static auto heavy_big_kernel_t(int unroll) -> decltype(&k_heavy_big<I, 4, O, kNarrow>) {
+template <class I, class O>
+static auto heavy_big_kernel(int unroll) -> decltype(&k_heavy_big<I, 4, O>) {
   switch (unroll) {
-    case 1: return &k_heavy_big<I, 1, O, kNarrow>;
-    case 2: return &k_heavy_big<I, 2, O, kNarrow>;
-    case 4: return &k_heavy_big<I, 4, O, kNarrow>;
-    case 8: return &k_heavy_big<I, 8, O, kNarrow>;
-    default: return &k_heavy_big<I, 16, O, kNarrow>;
+    case 1: return &k_heavy_big<I, 1, O>;
+    case 2: return &k_heavy_big<I, 2, O>;
+    case 4: return &k_heavy_big<I, 4, O>;
+    case 8: return &k_heavy_big<I, 8, O>;
+    default: return &k_heavy_big<I, 16, O>;
   }
 }
 
-template <class I, class O, bool kNarrow>
-static auto heavy_kernel_t(int unroll) -> decltype(&k_heavy<I, 4, O, kNarrow>) {
+template <class I, class O>
+static auto heavy_kernel(int unroll) -> decltype(&k_heavy<I, 4, O>) {
   switch (unroll) {
-    case 1: return &k_heavy<I, 1, O, kNarrow>;
-    case 2: return &k_heavy<I, 2, O, kNarrow>;
-    case 4: return &k_heavy<I, 4, O, kNarrow>;
-    case 8: return &k_heavy<I, 8, O, kNarrow>;
-    default: return &k_heavy<I, 16, O, kNarrow>;
+    case 1: return &k_heavy<I, 1, O>;
+    case 2: return &k_heavy<I, 2, O>;
+    case 4: return &k_heavy<I, 4, O>;
+    case 8: return &k_heavy<I, 8, O>;
+    default: return &k_heavy<I, 16, O>;
   }
 }
-
-// The entry points do not receive the length of col[], so bounds on its
-// offsets come from the allocation holding it: every in-bounds id lies
-// between col and the allocation's end.  Returns the number of int32 slots
-// in that span, or 0 when the runtime does not track the memory (then the
-// 64-bit id addressing and the id layout of the packed words are used).
-static int64_t col_span_ids(const int32_t *col) {
-  hipDeviceptr_t base = nullptr;
-  size_t size = 0;
-  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)col) != hipSuccess) {
-    (void)hipGetLastError();
-    return 0;
-  }
-  const uintptr_t b = (uintptr_t)base, p = (uintptr_t)col;
-  if (p < b || p >= b + size) return 0;
-  return (int64_t)((b + size - p) / sizeof(int32_t));
-}
-
-// narrow id addressing: every byte offset into col[] fits 32 bits
-static bool narrow_ids(int64_t span) { return span > 0 && span <= (int64_t(1) << 30); }
 
 __global__ void __launch_bounds__(kBlock)
-k_max_degree(const int64_t *__restrict__ rp64, const int32_t *__restrict__ rp32, int64_t n,
-             unsigned long long *__restrict__ max_deg) {
+k_row_stats(const int64_t *__restrict__ rp64, const int32_t *__restrict__ rp32, int64_t n,
+            unsigned long long *__restrict__ stats) {
   unsigned long long m = 0;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
        u += (int64_t)gridDim.x * blockDim.x) {
@@ -688,17 +708,19 @@ k_max_degree(const int64_t *__restrict__ rp64, const int32_t *__restrict__ rp32,
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned long long)__shfl_xor(m, off, kWave));
-  if (lane_id() == 0 && m) atomicMax(max_deg, m);
+  if (lane_id() == 0 && m) atomicMax(&stats[0], m);
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    stats[1] = (unsigned long long)(rp64 ? rp64[n] : (int64_t)rp32[n]);
 }
 
-template <class I, class O>
-static auto heavy_big_kernel(int unroll, bool narrow) -> decltype(&k_heavy_big<I, 4, O, true>) {
-  return narrow ? heavy_big_kernel_t<I, O, true>(unroll) : heavy_big_kernel_t<I, O, false>(unroll);
-}
-
-template <class I, class O>
-static auto heavy_kernel(int unroll, bool narrow) -> decltype(&k_heavy<I, 4, O, true>) {
-  return narrow ? heavy_kernel_t<I, O, true>(unroll) : heavy_kernel_t<I, O, false>(unroll);
+// stats[0..1] = (max forward degree, row_ptr[n]); stats[0] must be zero
+static int launch_row_stats(const void *row_ptr, int row_ptr_64, int64_t n,
+                            unsigned long long *stats, hipStream_t s) {
+  hipLaunchKernelGGL(k_row_stats, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, s,
+                     row_ptr_64 ? (const int64_t *)row_ptr : nullptr,
+                     row_ptr_64 ? nullptr : (const int32_t *)row_ptr, n, stats);
+  TC_HIP(hipGetLastError());
+  return TC_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -717,6 +739,7 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
   uint64_t acc = 0, nh = 0;
   const int lane = lane_id();
   const bool rows_layout = L.rows();
+  const int lbits = L.lbits();
   const uint64_t sentinel = (uint64_t)n << L.shift;
   // the loop bound is uniform per wavefront, so every lane stays active for
   // the cross-lane aggregation below
@@ -740,7 +763,7 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
             const uint32_t c = (uint32_t)(cv ? v : u), o = (uint32_t)(cv ? u : v);
             word = (uint64_t)c << L.shift;
             if (rows_layout)
-              word |= ((uint64_t)(cv ? ub : vb) << L.lbits) | (uint64_t)(cv ? du : dv);
+              word |= ((uint64_t)(cv ? ub : vb) << lbits) | (uint64_t)(cv ? du : dv);
             else
               word |= o;
             ++nh;
@@ -850,13 +873,8 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   }
   int key_bits = 1;
   while ((int64_t(1) << key_bits) <= n) ++key_bits;  // center ids and the sentinel n
-  // packed-word layout: the center in the top key_bits bits; the row start
-  // field must hold every offset into col[] (at most `span`)
-  const int64_t span = col_span_ids(col);
-  int start_bits = 1;
-  while (start_bits < 63 && (int64_t(1) << start_bits) <= span) ++start_bits;
+  // packed words: the center in the top key_bits bits (PackLayout)
   const int shift = 64 - key_bits;
-  const int lbits = (!pack_ids && span > 0 && shift - start_bits >= 1) ? shift - start_bits : 0;
   const int64_t max_runs = n;                         // at most one run per center
   Scratch packed, tmp, flags, run_idx, n_runs, run_ptr, rkey, perm, starts, nitems, iptr,
       items, ctr;
@@ -873,23 +891,18 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   TC_TRY(iptr.alloc(sizeof(int64_t) * (max_runs + 1), s));
   const int64_t max_items = max_runs + E / chunk + 1;
   TC_TRY(items.alloc(sizeof(int2) * max_items, s));
-  TC_TRY(ctr.alloc(5 * sizeof(unsigned long long), s));
+  TC_TRY(ctr.alloc(6 * sizeof(unsigned long long), s));
   unsigned long long *next_item = ctr.as<unsigned long long>();
   unsigned long long *n_heavy = next_item + 1;
-  unsigned long long *max_deg = next_item + 4;
-  const PackLayout layout{shift, lbits, max_deg};
+  unsigned long long *row_stats = next_item + 4;  // 2 entries
+  const PackLayout layout{shift, pack_ids ? 1 : 0, row_stats};
 
   const unsigned gr = grid_for(max_runs + 1, kBlock, 16384);
   const unsigned ge = grid_for(E, kBlock, 16384);
   unsigned long long *next_big = next_item + 2;
   unsigned long long *next_blk = next_item + 3;
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, next_item, 5, 0ull);
-  if (lbits > 0) {
-    hipLaunchKernelGGL(k_max_degree, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, s,
-                       row_ptr_64 ? (const int64_t *)row_ptr : nullptr,
-                       row_ptr_64 ? nullptr : (const int32_t *)row_ptr, n, max_deg);
-    TC_HIP(hipGetLastError());
-  }
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, next_item, 6, 0ull);
+  TC_TRY(launch_row_stats(row_ptr, row_ptr_64, n, row_stats, s));
   TC_DISPATCH_ROWS(row_ptr_64, I, {
     Rows<I> R{(const I *)row_ptr};
     {
@@ -944,21 +957,22 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                        (const int64_t *)iptr.as<int64_t>(), n_sorted_runs, items.as<int2>());
     TC_HIP(hipGetLastError());
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
-    const bool narrow = narrow_ids(span);
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
       const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
       const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - blk_bits + 1);
-      const auto kbig = heavy_big_kernel<I, OthersPacked>(unroll, narrow);
+      const auto kbig = heavy_big_kernel<I, OthersPacked>(unroll);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
+                         (const unsigned long long *)row_stats,
                          (const int64_t *)run_ptr.as<int64_t>(),
                          OthersPacked{(const uint64_t *)sorted, layout},
                          (const int2 *)items.as<int2>(),
                          (const int64_t *)iptr.as<int64_t>(), n_huge, chunk, next_big, graph_of,
                          count);
-      const auto kfn = heavy_kernel<I, OthersPacked>(unroll, narrow);
+      const auto kfn = heavy_kernel<I, OthersPacked>(unroll);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R,
-                         col, (const int64_t *)run_ptr.as<int64_t>(),
+                         col, (const unsigned long long *)row_stats,
+                         (const int64_t *)run_ptr.as<int64_t>(),
                          OthersPacked{(const uint64_t *)sorted, layout},
                          (const int2 *)items.as<int2>(),
                          (const int64_t *)(iptr.as<int64_t>() + max_runs),
@@ -1015,12 +1029,14 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
   TC_TRY(iptr.alloc(sizeof(int64_t) * (n + 1), s));
   const int64_t max_items = n + E / chunk + 1;
   TC_TRY(items.alloc(sizeof(int2) * max_items, s));
-  TC_TRY(ctr.alloc(3 * sizeof(unsigned long long), s));
+  TC_TRY(ctr.alloc(5 * sizeof(unsigned long long), s));
+  unsigned long long *row_stats = ctr.as<unsigned long long>() + 3;  // 2 entries
 
   const unsigned gn = grid_for(n, kBlock, 16384);
   const unsigned ge = grid_for(E, kBlock, 16384);
   hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(kBlock), 0, s, cnt.as<int32_t>(), n, 0);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr.as<unsigned long long>(), 3, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr.as<unsigned long long>(), 5, 0ull);
+  TC_TRY(launch_row_stats(row_ptr, row_ptr_64, n, row_stats, s));
   TC_DISPATCH_ROWS(row_ptr_64, I, {
     Rows<I> R{(const I *)row_ptr};
     {
@@ -1053,19 +1069,20 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                        (const int64_t *)iptr.as<int64_t>(), n_cent, items.as<int2>());
     TC_HIP(hipGetLastError());
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
-    const bool narrow = narrow_ids(col_span_ids(col));
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
       const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
       const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - blk_bits + 1);
-      const auto kbig = heavy_big_kernel<I, OthersIds>(unroll, narrow);
+      const auto kbig = heavy_big_kernel<I, OthersIds>(unroll);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
+                         (const unsigned long long *)row_stats,
                          (const int64_t *)cptr.as<int64_t>(),
                          OthersIds{(const int32_t *)others.as<int32_t>()},
                          (const int2 *)items.as<int2>(), (const int64_t *)iptr.as<int64_t>(),
                          n_huge, chunk, ctr.as<unsigned long long>() + 1, graph_of, count);
-      const auto kfn = heavy_kernel<I, OthersIds>(unroll, narrow);
+      const auto kfn = heavy_kernel<I, OthersIds>(unroll);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R, col,
+                         (const unsigned long long *)row_stats,
                          (const int64_t *)cptr.as<int64_t>(),
                          OthersIds{(const int32_t *)others.as<int32_t>()},
                          (const int2 *)items.as<int2>(), (const int64_t *)(iptr.as<int64_t>() + n),
