@@ -103,3 +103,30 @@ def test_empty_graphs_in_batch(device):
     got = batch.batch_exact_count(b).cpu().numpy()
     assert np.array_equal(got, truth)
     assert got[3] == 0 and got[7] == 0
+
+
+def test_graph_replays_with_only_light_edges(device):
+    # every edge is merged in the classify pass (rows of at most 3 ids), and a
+    # captured pipeline never takes the eager early exit: the radix sort drops
+    # every word and the intersection kernels get no work item
+    import math
+
+    import torch
+
+    parts = [np.stack(np.triu_indices(4, 1), axis=1), np.array([[0, 1], [1, 2], [2, 0]]),
+             np.array([[0, 1], [1, 2], [2, 3], [3, 4]]), np.stack(np.triu_indices(3, 1), axis=1)]
+    n_ids = np.array([4, 3, 5, 3], dtype=np.int64)
+    spec = batch.BatchSpec(kind=np.zeros(4, np.uint8), param=np.zeros(4, np.int32), n_ids=n_ids,
+                           n_pairs=np.array([len(p) for p in parts], dtype=np.int64),
+                           keys=np.zeros(4, np.uint64), seed=0)
+    pairs = np.concatenate([p.astype(np.int64) + off
+                            for p, off in zip(parts, spec.id_off[:-1])]).astype(np.int32)
+    truth = np.array([math.comb(4, 3), 1, 0, 1], dtype=np.int64)
+    b = batch.build_batch(spec, pairs=torch.from_numpy(pairs).cuda())
+    assert np.array_equal(batch.batch_exact_count(b).cpu().numpy(), truth)
+    g = batch.BatchCountGraph(b)
+    for i in range(3):
+        got = g.launch().clone()
+        torch.cuda.synchronize()
+        assert np.array_equal(got.cpu().numpy(), truth), f"replay {i}"
+    g.close()
