@@ -210,3 +210,29 @@ def test_flag_indices_all_set_and_none_set(device):
         f = torch.full((5000,), fill, dtype=torch.uint8, device="cuda")
         got = gpu_hybrid.flag_indices(f).cpu().numpy()
         assert np.array_equal(got, np.flatnonzero(np.full(5000, fill)))
+
+
+def test_chunglu_full_size_matches_cpu_oracle(device):
+    # the configured 117M-edge power-law graph (n = 3,072,441, beta 2.1,
+    # w_max 33,000, seed 2) with int64 offsets: the count of both packed-word
+    # layouts and of the atomic plan equals the C oracle on the same CSR
+    csr = gpu_graph.build_csr_gpu(config_pairs("chunglu"), row_ptr_dtype="int64")
+    rp = csr.row_ptr.cpu().numpy()
+    col = csr.col_idx.cpu().numpy()
+    truth = cbind.exact_count(rp, col)
+    assert truth == 1_836_795_280
+    for params in ({}, {"pack_ids": True}, {"plan": "atomic"}):
+        assert _count(csr, params=params) == truth, params
+
+
+def test_rmat23_full_size_planners_agree(device):
+    # the 129M-edge R-MAT scale 23 graph (seed 4) built and counted on the
+    # device: the planners and packed-word layouts agree with each other and
+    # with the count a 16-core run of the C oracle gave for this graph
+    from triangle_counting_hll_amd import batch
+
+    csr = gpu_graph.build_csr_gpu(batch.rmat_gpu(23, 16, seed=4), row_ptr_dtype="int64")
+    assert csr.n_oriented == 129_335_302
+    counts = {str(p): _count(csr, params=p)
+              for p in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"unroll": 4, "chunk": 64})}
+    assert set(counts.values()) == {4_672_556_999}, counts
