@@ -85,6 +85,45 @@ __device__ __forceinline__ uint32_t merge_count(const int32_t *__restrict__ col,
   return c;
 }
 
+// Light edges of one wavefront round, each with rows of at most 32 ids.  A
+// lane's serial merge is a chain of dependent loads, and the whole wavefront
+// waits for its slowest lane; when a round holds few light edges, the
+// wavefront intersects them one after the other instead: one load puts row a
+// in lanes [0, la) and row b in lanes [32, 32 + lb), then one ballot per id of
+// a counts its match in b.  With many light edges in the round, each lane
+// merges its own.  Returns |a ∩ b| in the edge's lane, 0 elsewhere.
+constexpr int kCoopMaxPerRound = 16;
+
+__device__ __forceinline__ uint32_t light_intersections(const int32_t *__restrict__ col, bool coop,
+                                                        int64_t a0, int la, int64_t b0, int lb) {
+  uint64_t lm = __ballot(coop);
+  if (!lm) return 0;
+  if (__popcll(lm) > kCoopMaxPerRound)
+    return coop ? merge_count(col, a0, a0 + la, b0, b0 + lb) : 0u;
+  const int lane = lane_id();
+  constexpr int kHalf = kWave / 2;
+  uint32_t mine = 0;
+  while (lm) {
+    const int src = __ffsll((long long)lm) - 1;
+    lm &= lm - 1;
+    const int64_t ea = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(a0 >> 32), src) << 32) |
+                                 (uint32_t)__builtin_amdgcn_readlane((int)a0, src));
+    const int64_t eb = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(b0 >> 32), src) << 32) |
+                                 (uint32_t)__builtin_amdgcn_readlane((int)b0, src));
+    const int na = __builtin_amdgcn_readlane(la, src), nb = __builtin_amdgcn_readlane(lb, src);
+    int x = -2;  // matches no id
+    if (lane < na) x = col[ea + lane];
+    else if (lane >= kHalf && lane - kHalf < nb) x = col[eb + (lane - kHalf)];
+    uint32_t c = 0;
+    for (int i = 0; i < na; ++i) {
+      const int ai = __builtin_amdgcn_readlane(x, i);
+      c += (uint32_t)__popcll(__ballot(lane >= kHalf && x == ai));
+    }
+    if (lane == src) mine = c;
+  }
+  return mine;
+}
+
 // Edge filter shared by the exact and hybrid pipelines: mask == nullptr means
 // every edge is counted; otherwise edges with mask[e] != 0 are skipped.
 template <class I>
@@ -748,6 +787,9 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
     const int64_t e = base + threadIdx.x;
     uint32_t tri = 0;
     int32_t u = 0;
+    bool coop = false;           // light edge whose rows fit 32 lanes each
+    int64_t la0 = 0, lb0 = 0;    // its rows
+    int la = 0, lb = 0;
     if (e < E) {
       uint64_t word = sentinel;
       if (!(skip && skip[e])) {
@@ -757,7 +799,15 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
         const int64_t du = ue - ub, dv = ve - vb;
         if (du > 0 && dv > 0) {
           if (du + dv <= light_max) {
-            tri = merge_count(col, ub, ue, vb, ve);
+            if (du <= kWave / 2 && dv <= kWave / 2) {
+              coop = true;
+              la0 = ub;
+              lb0 = vb;
+              la = (int)du;
+              lb = (int)dv;
+            } else {
+              tri = merge_count(col, ub, ue, vb, ve);
+            }
           } else {
             const bool cv = dv >= du;
             const uint32_t c = (uint32_t)(cv ? v : u), o = (uint32_t)(cv ? u : v);
@@ -772,6 +822,7 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
       }
       packed[e] = word;
     }
+    tri += light_intersections(col, coop, la0, la, lb0, lb);
     if (graph_of) {
       // one atomic per wavefront when all its triangles belong to one graph
       const int g = tri ? graph_of[u] : -1;
