@@ -94,6 +94,12 @@ __device__ __forceinline__ uint32_t merge_count(const int32_t *__restrict__ col,
 // merges its own.  Returns |a ∩ b| in the edge's lane, 0 elsewhere.
 constexpr int kCoopMaxPerRound = 16;
 
+__device__ __forceinline__ int64_t readlane_i64(int64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ uint32_t light_intersections(const int32_t *__restrict__ col, bool coop,
                                                         int64_t a0, int la, int64_t b0, int lb) {
   uint64_t lm = __ballot(coop);
@@ -106,10 +112,7 @@ __device__ __forceinline__ uint32_t light_intersections(const int32_t *__restric
   while (lm) {
     const int src = __ffsll((long long)lm) - 1;
     lm &= lm - 1;
-    const int64_t ea = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(a0 >> 32), src) << 32) |
-                                 (uint32_t)__builtin_amdgcn_readlane((int)a0, src));
-    const int64_t eb = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(b0 >> 32), src) << 32) |
-                                 (uint32_t)__builtin_amdgcn_readlane((int)b0, src));
+    const int64_t ea = readlane_i64(a0, src), eb = readlane_i64(b0, src);
     const int na = __builtin_amdgcn_readlane(la, src), nb = __builtin_amdgcn_readlane(lb, src);
     int x = -2;  // matches no id
     if (lane < na) x = col[ea + lane];
@@ -360,9 +363,10 @@ struct OthersPacked {
 };
 
 // Position of the id at flattened offset `off + t` of col[].  With kNarrow
-// (col[] smaller than 4 GiB) the offset is a 32-bit byte offset added to the
-// array base, one VGPR and one 32-bit add per id instead of 64-bit address
-// arithmetic; `off` may be "negative" (it wraps), only the sum is in range.
+// (every row within the first 2^30 ids, see rows_narrow) the offset is a
+// 32-bit byte offset added to the array base, one VGPR and one 32-bit add per
+// id instead of 64-bit address arithmetic; `off` may be "negative" (it wraps),
+// only the sum is in range.
 template <bool kNarrow>
 struct IdOffset {
   using T = int64_t;
