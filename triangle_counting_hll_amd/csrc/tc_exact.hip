@@ -109,20 +109,39 @@ __device__ __forceinline__ uint32_t light_intersections(const int32_t *__restric
   const int lane = lane_id();
   constexpr int kHalf = kWave / 2;
   uint32_t mine = 0;
-  while (lm) {
-    const int src = __ffsll((long long)lm) - 1;
-    lm &= lm - 1;
+  // rows of the light edge of lane `src`: a in lanes [0, na), b in [32, 32 + nb)
+  auto fetch = [&](int src, int &na) -> int {
     const int64_t ea = readlane_i64(a0, src), eb = readlane_i64(b0, src);
-    const int na = __builtin_amdgcn_readlane(la, src), nb = __builtin_amdgcn_readlane(lb, src);
+    na = __builtin_amdgcn_readlane(la, src);
+    const int nb = __builtin_amdgcn_readlane(lb, src);
     int x = -2;  // matches no id
     if (lane < na) x = col[ea + lane];
     else if (lane >= kHalf && lane - kHalf < nb) x = col[eb + (lane - kHalf)];
+    return x;
+  };
+  int src = __ffsll((long long)lm) - 1;
+  lm &= lm - 1;
+  int na = 0;
+  int x = fetch(src, na);
+  while (true) {
+    // the next edge's rows are in flight while this edge's ids are compared
+    const bool more = lm != 0;
+    int src2 = 0, na2 = 0, x2 = -2;
+    if (more) {
+      src2 = __ffsll((long long)lm) - 1;
+      lm &= lm - 1;
+      x2 = fetch(src2, na2);
+    }
     uint32_t c = 0;
     for (int i = 0; i < na; ++i) {
       const int ai = __builtin_amdgcn_readlane(x, i);
       c += (uint32_t)__popcll(__ballot(lane >= kHalf && x == ai));
     }
     if (lane == src) mine = c;
+    if (!more) break;
+    src = src2;
+    na = na2;
+    x = x2;
   }
   return mine;
 }
