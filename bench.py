@@ -45,6 +45,20 @@ def exact_bytes(row_ptr: np.ndarray, col: np.ndarray, eu: np.ndarray, ev: np.nda
     return int(per.sum()), int(per[heavy].sum()), int(heavy.sum())
 
 
+def cpu_model() -> str:
+    """Host CPU model name (from /proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(rp, col, budget_s=20.0):
     """Time the C oracle (the cpu_exp exact policy, OpenMP) beside the GPU run.
 
@@ -78,7 +92,8 @@ def cpu_baseline(rp, col, budget_s=20.0):
             times.append(time.perf_counter() - t0)
         dt = float(np.median(times))
         return {"value": E / dt, "unit": "oriented edges/s", "cores": threads,
-                "kind": "port", "seconds": dt, "runs": len(times), "count": int(c),
+                "cpu": cpu_model(), "kind": "port", "seconds": dt, "runs": len(times),
+                "count": int(c),
                 "sample": f"all {E} oriented edges (full graph), median of {len(times)}",
                 "single_thread": single}
     stride = int(np.ceil(est_full / budget_s))
@@ -87,8 +102,8 @@ def cpu_baseline(rp, col, budget_s=20.0):
     t0 = time.perf_counter()
     cbind.exact_count(rp, col, eu_all[sel], col[sel], policy=0, nthreads=threads)
     dt = time.perf_counter() - t0
-    return {"value": n_s / dt, "unit": "oriented edges/s", "cores": threads, "kind": "port",
-            "seconds": dt, "runs": 1, "count": None,
+    return {"value": n_s / dt, "unit": "oriented edges/s", "cores": threads, "cpu": cpu_model(),
+            "kind": "port", "seconds": dt, "runs": 1, "count": None,
             "sample": f"every {stride}th oriented edge ({n_s} of {E}), u-major order",
             "single_thread": single}
 
