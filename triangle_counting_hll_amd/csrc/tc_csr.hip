@@ -4,8 +4,9 @@
 //   canonical (min,max) keys packed in 2 x bit_length(max id) bits -> LSD
 //   radix sort -> adjacent-unique
 //   -> presence flags + scan = dense relabel by sorted unique endpoint
-//   -> undirected degrees -> orientation -> forward degrees -> scan = row_ptr
-//   -> oriented (src, dst) keys radix-sorted: the rows in order, each sorted.
+//   -> undirected degrees -> orientation
+//   -> oriented (src, dst) keys radix-sorted: the rows in order, each sorted
+//   -> row_ptr by a lower bound per vertex over the sorted keys.
 // The output equals the lexsort-based loader of experiment/exp_gpu.py:301-365.
 #include <algorithm>
 #include <climits>
@@ -110,9 +111,10 @@ __global__ void k_relabel_degree(uint64_t *__restrict__ uniq, int64_t nu,
   }
 }
 
-// u -> v iff (deg u, u) < (deg v, v); keys become (src << 32 | dst).
-__global__ void k_orient(uint64_t *__restrict__ uniq, int64_t nu, const int *__restrict__ udeg,
-                         int *__restrict__ fdeg) {
+// u -> v iff (deg u, u) < (deg v, v); keys become (src << 32 | dst).  The
+// forward degrees come later from the sorted keys (k_row_starts), so there is
+// no per-edge atomic here.
+__global__ void k_orient(uint64_t *__restrict__ uniq, int64_t nu, const int *__restrict__ udeg) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nu;
        i += (int64_t)gridDim.x * blockDim.x) {
     uint64_t k = uniq[i];
@@ -121,7 +123,22 @@ __global__ void k_orient(uint64_t *__restrict__ uniq, int64_t nu, const int *__r
     bool a_first = (da < db) || (da == db && a < b);
     uint32_t s = a_first ? a : b, d = a_first ? b : a;
     uniq[i] = ((uint64_t)s << 32) | d;
-    atomicAdd(&fdeg[s], 1);
+  }
+}
+
+// row_ptr[v] = number of sorted (s << B | d) keys with s < v, for v in [0, n]
+// (a lower bound per vertex; vertices without forward edges get the next
+// row's start).
+__global__ void k_row_starts(const uint64_t *__restrict__ keys, int64_t E, int B, int64_t n,
+                             int64_t *__restrict__ row_ptr) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    int64_t lo = 0, hi = E;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if ((int64_t)(keys[mid] >> B) < v) lo = mid + 1; else hi = mid;
+    }
+    row_ptr[v] = lo;
   }
 }
 
@@ -152,7 +169,7 @@ __global__ void k_unpack_rows(const uint64_t *__restrict__ keys, int64_t E, int 
 
 __global__ void k_finish_arrays(const int64_t *__restrict__ row_ptr64, int64_t n,
                                 const int32_t *__restrict__ node_list,
-                                const int *__restrict__ fdeg, const int *__restrict__ udeg,
+                                const int *__restrict__ udeg,
                                 void *__restrict__ row_ptr_out, int row_ptr_64,
                                 int32_t *__restrict__ degrees, int64_t *__restrict__ node_ids,
                                 int32_t *__restrict__ udeg_out) {
@@ -165,7 +182,7 @@ __global__ void k_finish_arrays(const int64_t *__restrict__ row_ptr64, int64_t n
         ((int32_t *)row_ptr_out)[i] = (int32_t)row_ptr64[i];
     }
     if (i < n) {
-      if (degrees) degrees[i] = fdeg[i];
+      if (degrees) degrees[i] = (int32_t)(row_ptr64[i + 1] - row_ptr64[i]);
       if (node_ids) node_ids[i] = node_list[i];
       if (udeg_out) udeg_out[i] = udeg[i];
     }
@@ -181,11 +198,11 @@ struct tc_csr_build {
   hipStream_t stream = nullptr;
   int64_t n = 0, E = 0;
   uint64_t *keys = nullptr;      // oriented (src<<32 | dst), dense ids
-  int *udeg = nullptr, *fdeg = nullptr;
-  int64_t *row_ptr64 = nullptr;  // n+1
+  int *udeg = nullptr;
+  int64_t *row_ptr64 = nullptr;  // n+1, filled by tc_csr_build_finish
   int32_t *node_list = nullptr;  // n original ids
   ~tc_csr_build() {
-    for (void *p : {(void *)keys, (void *)udeg, (void *)fdeg, (void *)row_ptr64, (void *)node_list})
+    for (void *p : {(void *)keys, (void *)udeg, (void *)row_ptr64, (void *)node_list})
       if (p) (void)hipFree(p);
   }
 };
@@ -305,14 +322,12 @@ extern "C" int tc_csr_build_begin(const int32_t *pairs, int64_t m, void *stream,
   }
   b->n = n;
   if (hipMalloc(&b->udeg, sizeof(int) * std::max<int64_t>(n, 1)) != hipSuccess ||
-      hipMalloc(&b->fdeg, sizeof(int) * std::max<int64_t>(n, 1)) != hipSuccess ||
       hipMalloc(&b->row_ptr64, sizeof(int64_t) * (n + 1)) != hipSuccess ||
       hipMalloc(&b->node_list, sizeof(int32_t) * std::max<int64_t>(n, 1)) != hipSuccess) {
     set_error(TC_ERR_OOM, "cannot allocate CSR build state for %lld nodes", (long long)n);
     return fail(TC_ERR_OOM);
   }
-  if (hipMemsetAsync(b->udeg, 0, sizeof(int) * std::max<int64_t>(n, 1), s) != hipSuccess ||
-      hipMemsetAsync(b->fdeg, 0, sizeof(int) * std::max<int64_t>(n, 1), s) != hipSuccess) {
+  if (hipMemsetAsync(b->udeg, 0, sizeof(int) * std::max<int64_t>(n, 1), s) != hipSuccess) {
     set_error(TC_ERR_HIP, "memset failed");
     return fail(TC_ERR_HIP);
   }
@@ -324,10 +339,8 @@ extern "C" int tc_csr_build_begin(const int32_t *pairs, int64_t m, void *stream,
     hipLaunchKernelGGL(k_relabel_degree, dim3(grid_for(nu, kBlock, 8192)), dim3(kBlock), 0, s,
                        b->keys, nu, (const int64_t *)nidx.as<int64_t>(), b->udeg);
     hipLaunchKernelGGL(k_orient, dim3(grid_for(nu, kBlock, 8192)), dim3(kBlock), 0, s, b->keys,
-                       nu, (const int *)b->udeg, b->fdeg);
+                       nu, (const int *)b->udeg);
   }
-  rc = exclusive_scan<int32_t>(b->fdeg, b->row_ptr64, n, nullptr, s);
-  if (rc) return fail(rc);
   if (hipStreamSynchronize(s) != hipSuccess) {
     set_error(TC_ERR_HIP, "CSR build phase 1 failed: %s", hipGetErrorString(hipGetLastError()));
     return fail(TC_ERR_HIP);
@@ -350,11 +363,12 @@ extern "C" int tc_csr_build_finish(tc_csr_build *b, void *row_ptr, int row_ptr_6
            "%lld oriented edges do not fit int32 row offsets; use int64", (long long)b->E);
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = b->n, E = b->E;
+  int B = 1;
+  while ((int64_t(1) << B) < n) ++B;
+  const uint64_t *sorted_keys = b->keys;  // read only when E > 0
+  Scratch tmp;
   if (E > 0) {
     // rows in order and sorted inside: one radix sort of the oriented keys
-    int B = 1;
-    while ((int64_t(1) << B) < n) ++B;
-    Scratch tmp;
     TC_TRY(tmp.alloc(sizeof(uint64_t) * E, s));
     hipLaunchKernelGGL(k_repack_oriented, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock), 0, s,
                        b->keys, E, B);
@@ -364,11 +378,16 @@ extern "C" int tc_csr_build_finish(tc_csr_build *b, void *row_ptr, int row_ptr_6
     hipLaunchKernelGGL(k_unpack_rows, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock), 0, s,
                        (const uint64_t *)sorted, E, B, col_idx, edge_u, edge_v);
     TC_HIP(hipGetLastError());
+    sorted_keys = sorted;
   }
+  // row offsets (and forward degrees) from the sorted keys
+  hipLaunchKernelGGL(k_row_starts, dim3(grid_for(n + 1)), dim3(kBlock), 0, s, sorted_keys, E, B,
+                     n, b->row_ptr64);
+  TC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_finish_arrays, dim3(grid_for(n + 1)), dim3(kBlock), 0, s,
                      (const int64_t *)b->row_ptr64, n, (const int32_t *)b->node_list,
-                     (const int *)b->fdeg, (const int *)b->udeg, row_ptr, row_ptr_64, degrees,
-                     node_ids, undirected_deg);
+                     (const int *)b->udeg, row_ptr, row_ptr_64, degrees, node_ids,
+                     undirected_deg);
   TC_HIP(hipGetLastError());
   return TC_OK;
 }
