@@ -236,3 +236,16 @@ def test_rmat23_full_size_planners_agree(device):
     counts = {str(p): _count(csr, params=p)
               for p in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"unroll": 4, "chunk": 64})}
     assert set(counts.values()) == {4_672_556_999}, counts
+
+
+def test_rgg21_full_size_matches_cpu_oracle(device):
+    # the configured random geometric graph (n = 2^21, mean degree 2.8, seed
+    # 1): every edge is merged in the classify pass; eager (early exit after
+    # the merge pass) and the id layout agree with the C oracle
+    csr = gpu_graph.build_csr_gpu(config_pairs("rgg21"))
+    rp = csr.row_ptr.cpu().numpy()
+    col = csr.col_idx.cpu().numpy()
+    truth = cbind.exact_count(rp, col)
+    assert truth == _truth(config_pairs("rgg21"))
+    for params in ({}, {"pack_ids": True}, {"light_max": 2}):
+        assert _count(csr, params=params) == truth, params
