@@ -206,6 +206,69 @@ def run_hybrid_extra(csr, truth, reps=3):
     return out
 
 
+def run_doulion_extra(csr, truth):
+    """DOULION on the same graph (experiment/exp_gpu.py:396-450): q = 0.10 and
+    0.05 over seeds 42..51, each estimate timed end to end (sampling mask,
+    sampled CSR, exact count, readback); the C oracle counts one q = 0.10
+    sample on the host cores."""
+    import torch
+
+    from oracle import cbind
+    from triangle_counting_hll_amd import doulion, gpu_hybrid
+
+    out = {}
+    for q in doulion.DOULION_QS:
+        r = doulion.doulion_forward_gpu(csr.row_ptr, csr.col_idx, q)
+        out[f"q{q:g}"] = {"mean_ms": r["mean_time"] * 1e3, "mean_estimate": r["mean_estimate"],
+                          "rel_err_of_mean": gpu_hybrid.relative_error(r["mean_estimate"], truth),
+                          "seeds": r["n_seeds"]}
+    keep = doulion.sample_mask_gpu(csr.n_oriented, 0.10, np.random.default_rng(42))
+    srp, scol = doulion.edge_subgraph(csr.row_ptr, csr.col_idx, keep)
+    torch.cuda.synchronize()
+    rp_h = srp.cpu().numpy().astype(np.int64)
+    col_h = scol.cpu().numpy()
+    t0 = time.perf_counter()
+    c = cbind.exact_count(rp_h, col_h, nthreads=len(os.sched_getaffinity(0)))
+    out["cpu_count_of_one_q0.1_sample"] = {"seconds": time.perf_counter() - t0,
+                                           "sampled_edges": int(col_h.size), "count": int(c),
+                                           "threads": len(os.sched_getaffinity(0))}
+    return out
+
+
+def run_degree_stats_extra(workload):
+    """Degree statistics of the raw edge list (experiment/gpu_graph_stats.py:89-320):
+    degree count, summary and histograms on the device; numpy bincount of the
+    same pairs on the host beside it."""
+    import torch
+
+    from triangle_counting_hll_amd import graph_stats
+
+    pairs = workload_pairs(workload)
+    n_slots = int(pairs.max().item()) + 1
+
+    def run():
+        deg = graph_stats.degree_count_gpu(pairs, n_slots)
+        summ = graph_stats.degree_summary_gpu(deg)
+        hist, log_hist = graph_stats.degree_histograms_gpu(deg, summ["max_degree"])
+        return deg, summ, hist
+
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    deg, summ, hist = run()
+    gpu_s = time.perf_counter() - t0
+    host = pairs.cpu().numpy()
+    t0 = time.perf_counter()
+    ref = np.bincount(host.reshape(-1), minlength=n_slots)
+    cpu_s = time.perf_counter() - t0
+    ref_hist = np.bincount(ref, minlength=len(hist))
+    return {"pairs": int(host.shape[0]), "gpu_ms": gpu_s * 1e3,
+            "numpy_bincount_ms": cpu_s * 1e3, "numpy_threads": 1,
+            "max_degree": summ["max_degree"], "active_nodes": summ["active_nodes"],
+            "degrees_equal_numpy": bool(np.array_equal(deg.cpu().numpy(), ref)),
+            "histogram_equal_numpy": bool(np.array_equal(hist[1:], ref_hist[1:len(hist)]))}
+
+
 def run_prototype_extra(csr, truth):
     """The prototype's timing table (experiment1_gpu/gpu_experiment_fixed.py:337-701)
     on the same graph: p = 8, gammas 32..256, its own hash and estimator."""
@@ -463,7 +526,8 @@ def main():
     ap.add_argument("--unroll", type=int, default=16)
     ap.add_argument("--extras", default="all",
                     help="comma-separated extra measurements to run (hybrid_fixed_threshold, "
-                         "ingest_text, batch4096, learned_router, prototype_table) or 'all'")
+                         "ingest_text, batch4096, learned_router, prototype_table, doulion, "
+                         "degree_stats) or 'all'")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the hybrid and many-graph measurements added to the line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -560,7 +624,9 @@ def main():
                          ("ingest_text", lambda: run_ingest_extra(args.workload, csr)),
                          ("batch4096", lambda: run_batch_extra(args)),
                          ("learned_router", lambda: run_router_extra()),
-                         ("prototype_table", lambda: run_prototype_extra(csr, gpu_count))):
+                         ("prototype_table", lambda: run_prototype_extra(csr, gpu_count)),
+                         ("doulion", lambda: run_doulion_extra(csr, gpu_count)),
+                         ("degree_stats", lambda: run_degree_stats_extra(args.workload))):
             if args.extras != "all" and name not in args.extras.split(","):
                 continue
             try:
