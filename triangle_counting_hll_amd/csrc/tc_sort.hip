@@ -120,7 +120,9 @@ k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int
       c[ww] = has_digit ? wpos[ww][d] : 0;
       tot += c[ww];
     }
-    const int st = block_exclusive_scan_int(tot, wave_tot, &s_kept);
+    int kept;
+    const int st = block_exclusive_scan_int(tot, wave_tot, &kept);
+    if (threadIdx.x == 0) s_kept = kept;
     if (has_digit) {
       tstart[d] = st;
       gstart[d] = offs[(int64_t)d * ntiles + blockIdx.x];
