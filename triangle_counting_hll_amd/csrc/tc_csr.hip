@@ -4,7 +4,9 @@
 //   canonical (min,max) keys packed in 2 x bit_length(max id) bits -> LSD
 //   radix sort -> adjacent-unique
 //   -> presence flags + scan = dense relabel by sorted unique endpoint
-//   -> undirected degrees -> orientation
+//   -> undirected degrees (first ids per run of equal ids; second ids grouped
+//   by a radix sort on their high bits and counted in LDS histograms)
+//   -> orientation
 //   -> oriented (src, dst) keys radix-sorted: the rows in order, each sorted
 //   -> row_ptr by a lower bound per vertex over the sorted keys.
 // The output equals the lexsort-based loader of experiment/exp_gpu.py:301-365.
@@ -77,13 +79,15 @@ __global__ void k_mark_present(const uint64_t *__restrict__ uniq, int64_t nu,
   }
 }
 
-// Relabel to dense ids in place (key -> dense a<<32 | dense b) and count
-// undirected degrees.  The keys are sorted, so equal first ids sit in
-// consecutive lanes: one atomic per run of them (hub rows would otherwise
-// serialise thousands of same-address atomics); the second ids are spread out
-// and get one atomic each.
+// Relabel to dense ids in place (key -> dense a<<32 | dense b) and count the
+// first ids' share of the undirected degrees.  The keys are sorted, so equal
+// first ids sit in consecutive lanes: one atomic per run of them (hub rows
+// would otherwise serialise thousands of same-address atomics).  The second
+// ids are spread out; they are written to `second` and counted by bucket
+// (k_bucket_degrees) instead of by one device-scope atomic each.
 __global__ void k_relabel_degree(uint64_t *__restrict__ uniq, int64_t nu,
-                                 const int64_t *__restrict__ nidx, int *__restrict__ udeg) {
+                                 const int64_t *__restrict__ nidx, int *__restrict__ udeg,
+                                 uint64_t *__restrict__ second) {
   const int lane = lane_id();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // the loop bound is uniform per wavefront, so every lane joins the ballot
@@ -107,7 +111,55 @@ __global__ void k_relabel_degree(uint64_t *__restrict__ uniq, int64_t nu,
       const int next = above ? (__ffsll((unsigned long long)above) - 1) : n_valid;
       atomicAdd(&udeg[a], next - lane);
     }
-    if (ok) atomicAdd(&udeg[b], 1);
+    if (ok) second[i] = b;
+  }
+}
+
+// The second ids' share of the degrees.  The ids are grouped by bucket (their
+// bits from kDegBucketBits up, radix-sorted); bounds[j] is the first position
+// of bucket j.  A workgroup takes a fixed chunk of the grouped ids (buckets
+// are skewed: the first ids of a power-law graph are its hubs), counts each
+// bucket segment of it in an LDS histogram of 2^kDegBucketBits counters and
+// adds the nonzero counts to udeg: one atomic per (chunk, id) instead of one
+// per edge, on consecutive addresses.
+constexpr int kDegBucketBits = 14;  // 16384 counters, 64 KiB of LDS
+constexpr int kDegBucketBlock = 1024;
+constexpr int64_t kDegChunk = int64_t(1) << 18;
+
+__global__ void k_bucket_bounds(const uint64_t *__restrict__ ids, int64_t nu, int64_t n_buckets,
+                                int64_t *__restrict__ bounds) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= n_buckets;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = (uint64_t)j << kDegBucketBits;
+    int64_t lo = 0, hi = nu;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (ids[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    bounds[j] = lo;
+  }
+}
+
+__global__ void __launch_bounds__(kDegBucketBlock)
+k_bucket_degrees(const uint64_t *__restrict__ ids, int64_t nu, const int64_t *__restrict__ bounds,
+                 int64_t n, int *__restrict__ udeg) {
+  __shared__ int hist[1 << kDegBucketBits];
+  const uint64_t low = (1ull << kDegBucketBits) - 1;
+  const int64_t c1 = min(nu, ((int64_t)blockIdx.x + 1) * kDegChunk);
+  int64_t pos = (int64_t)blockIdx.x * kDegChunk;
+  while (pos < c1) {  // uniform: every thread reads the same ids[pos]
+    const int64_t j = (int64_t)(ids[pos] >> kDegBucketBits);
+    const int64_t seg_end = min(c1, bounds[j + 1]);
+    for (int k = threadIdx.x; k < (1 << kDegBucketBits); k += kDegBucketBlock) hist[k] = 0;
+    __syncthreads();
+    for (int64_t i = pos + threadIdx.x; i < seg_end; i += kDegBucketBlock)
+      atomicAdd(&hist[(int)(ids[i] & low)], 1);
+    __syncthreads();
+    const int64_t v0 = j << kDegBucketBits;
+    for (int k = threadIdx.x; k < (1 << kDegBucketBits); k += kDegBucketBlock)
+      if (hist[k] && v0 + k < n) atomicAdd(&udeg[v0 + k], hist[k]);
+    __syncthreads();
+    pos = seg_end;
   }
 }
 
@@ -336,8 +388,25 @@ extern "C" int tc_csr_build_begin(const int32_t *pairs, int64_t m, void *stream,
     if (rc) return fail(rc);
   }
   if (nu > 0) {
+    // the canonical-sort buffers (m >= nu words each) hold the second ids
+    uint64_t *second = packed.as<uint64_t>();
     hipLaunchKernelGGL(k_relabel_degree, dim3(grid_for(nu, kBlock, 8192)), dim3(kBlock), 0, s,
-                       b->keys, nu, (const int64_t *)nidx.as<int64_t>(), b->udeg);
+                       b->keys, nu, (const int64_t *)nidx.as<int64_t>(), b->udeg, second);
+    int nb_bits = 1;
+    while ((int64_t(1) << nb_bits) < n) ++nb_bits;
+    uint64_t *grouped = second;
+    if (nb_bits > kDegBucketBits &&
+        (rc = radix_sort_u64(second, tmp.as<uint64_t>(), nu, kDegBucketBits, nb_bits, &grouped,
+                             s)))
+      return fail(rc);
+    const int64_t n_buckets = (n + (int64_t(1) << kDegBucketBits) - 1) >> kDegBucketBits;
+    Scratch bounds;
+    if ((rc = bounds.alloc(sizeof(int64_t) * (n_buckets + 1), s))) return fail(rc);
+    hipLaunchKernelGGL(k_bucket_bounds, dim3(grid_for(n_buckets + 1)), dim3(kBlock), 0, s,
+                       (const uint64_t *)grouped, nu, n_buckets, bounds.as<int64_t>());
+    hipLaunchKernelGGL(k_bucket_degrees, dim3((unsigned)((nu + kDegChunk - 1) / kDegChunk)),
+                       dim3(kDegBucketBlock), 0, s, (const uint64_t *)grouped, nu,
+                       (const int64_t *)bounds.as<int64_t>(), n, b->udeg);
     hipLaunchKernelGGL(k_orient, dim3(grid_for(nu, kBlock, 8192)), dim3(kBlock), 0, s, b->keys,
                        nu, (const int *)b->udeg);
   }
