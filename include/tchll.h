@@ -147,7 +147,10 @@ typedef struct tc_exact_params {
                          0 = they carry its row start and length when the
                          fields fit (no row-offset reads in the intersection
                          kernel), else its id                                 */
-  int reserved[1];
+  int wide_offsets;  /* 1 = the intersection kernels address ids with 64-bit
+                         offsets even when every row lies within the first
+                         2^30 ids (the form graphs above 2^30 oriented edges
+                         take; a test knob); 0 = chosen from row_ptr[n]       */
 } tc_exact_params;
 
 /* How heavy edges are grouped by center before the intersection kernel:

@@ -47,17 +47,19 @@ def test_clique_beyond_lds_table(device):
 
 
 @pytest.mark.parametrize("plan,pack_ids", [("sorted", False), ("sorted", True), ("atomic", False)])
+@pytest.mark.parametrize("wide", [False, True])
 @pytest.mark.parametrize("n", [2100, 4400])
-def test_big_centers_mixed_with_small(device, plan, pack_ids, n):
+def test_big_centers_mixed_with_small(device, plan, pack_ids, wide, n):
     # a K_n block plus an R-MAT graph on disjoint ids: forward rows of
     # 1024..4095 ids take the joined workgroup table, rows of 4096 ids and up
     # (n = 4400) the 128 KiB workgroup kernel, the R-MAT rows one wavefront
-    # each; counts add up
+    # each; counts add up.  wide: 64-bit id addressing, which graphs above
+    # 2^30 oriented edges take
     clique = np.stack(np.triu_indices(n, 1), axis=1)
     other = rmat(12, 16, seed=8).astype(np.int64) + n
     pairs = np.concatenate([clique, other]).astype(np.int32)
     csr = gpu_graph.build_csr_gpu(pairs)
-    assert (_count(csr, params={"plan": plan, "pack_ids": pack_ids})
+    assert (_count(csr, params={"plan": plan, "pack_ids": pack_ids, "wide_offsets": wide})
             == math.comb(n, 3) + _truth(other))
 
 
@@ -79,15 +81,18 @@ def test_rmat14_config(device):
                                                             (64, 32, 32, 8, 10),
                                                             (32, 256, 16, 16, 10),
                                                             (10**6, 256, 16, 4, 9)])
+@pytest.mark.parametrize("wide", [False, True])
 def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc, unroll, blk, plan,
-                                          pack_ids):
+                                          pack_ids, wide):
     # pack_ids: the sorted plan's packed words carry the other's id instead of
-    # its row start and length (the layout large graphs fall back to)
+    # its row start and length (the layout large graphs fall back to); wide:
+    # 64-bit id addressing in the intersection kernels (graphs above 2^30
+    # oriented edges)
     pairs = rmat(12, 16, seed=7)
     truth = _truth(pairs)
     csr = gpu_graph.build_csr_gpu(pairs)
     params = {"light_max": light_max, "chunk": chunk, "waves_per_cu": wpc, "unroll": unroll,
-              "plan": plan, "block_min_bits": blk, "pack_ids": pack_ids}
+              "plan": plan, "block_min_bits": blk, "pack_ids": pack_ids, "wide_offsets": wide}
     assert _count(csr, params=params) == truth
 
 
@@ -221,7 +226,7 @@ def test_chunglu_full_size_matches_cpu_oracle(device):
     col = csr.col_idx.cpu().numpy()
     truth = cbind.exact_count(rp, col)
     assert truth == 1_836_795_280
-    for params in ({}, {"pack_ids": True}, {"plan": "atomic"}):
+    for params in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"wide_offsets": True}):
         assert _count(csr, params=params) == truth, params
 
 
@@ -234,7 +239,8 @@ def test_rmat23_full_size_planners_agree(device):
     csr = gpu_graph.build_csr_gpu(batch.rmat_gpu(23, 16, seed=4), row_ptr_dtype="int64")
     assert csr.n_oriented == 129_335_302
     counts = {str(p): _count(csr, params=p)
-              for p in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"unroll": 4, "chunk": 64})}
+              for p in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"unroll": 4, "chunk": 64},
+                        {"wide_offsets": True})}
     assert set(counts.values()) == {4_672_556_999}, counts
 
 

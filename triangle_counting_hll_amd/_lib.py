@@ -84,7 +84,7 @@ class ExactParams(ctypes.Structure):
     _fields_ = [("light_max", ctypes.c_int), ("chunk", ctypes.c_int),
                 ("waves_per_cu", ctypes.c_int), ("unroll", ctypes.c_int),
                 ("plan", ctypes.c_int), ("block_min_bits", ctypes.c_int),
-                ("pack_ids", ctypes.c_int), ("reserved", ctypes.c_int * 1)]
+                ("pack_ids", ctypes.c_int), ("wide_offsets", ctypes.c_int)]
 
 
 PLANS = {"sorted": 0, "atomic": 1}

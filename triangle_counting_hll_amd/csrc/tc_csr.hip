@@ -34,7 +34,16 @@ __global__ void k_canon_max(const int32_t *__restrict__ pairs, int64_t m, int *_
     if (x != y) local_max = max(local_max, max(x, y));
   }
   for (int off = 32; off > 0; off >>= 1) local_max = max(local_max, __shfl_xor(local_max, off));
-  if (lane_id() == 0 && local_max >= 0) atomicMax(max_id, local_max);
+  // one same-address atomic per workgroup: thousands per wavefront serialise
+  __shared__ int wave_max[kBlock / kWave];
+  if (lane_id() == 0) wave_max[threadIdx.x / kWave] = local_max;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int bm = -1;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) bm = max(bm, wave_max[w]);
+    if (bm >= 0) atomicMax(max_id, bm);
+  }
 }
 
 // (min << B | max) for a non-loop pair, all-ones (sorts last) for a loop.
@@ -295,7 +304,7 @@ extern "C" int tc_csr_build_begin(const int32_t *pairs, int64_t m, void *stream,
     return fail(TC_ERR_HIP);
   }
   if (m > 0)
-    hipLaunchKernelGGL(k_canon_max, dim3(grid_for(m, kBlock, 8192)), dim3(kBlock), 0, s, pairs,
+    hipLaunchKernelGGL(k_canon_max, dim3(grid_for(m, kBlock, 2048)), dim3(kBlock), 0, s, pairs,
                        m, max_id, bad);
   struct {
     unsigned long long nu;

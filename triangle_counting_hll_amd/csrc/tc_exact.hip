@@ -305,7 +305,8 @@ __device__ __forceinline__ uint32_t global_probe(const int32_t *__restrict__ col
 
 // Row statistics measured on the device before a count (k_row_stats):
 // stats[0] = the largest forward degree, stats[1] = row_ptr[n], the number of
-// ids in col[] that the rows reach.  Every choice below is made from them
+// ids in col[] that the rows reach, stats[2] = 1 to force 64-bit id addressing
+// (the wide_offsets parameter).  Every choice below is made from them
 // inside the kernels, so it needs no host synchronisation and holds for any
 // memory the arrays live in.
 __device__ __forceinline__ int bit_length_u64(unsigned long long x) {
@@ -313,9 +314,9 @@ __device__ __forceinline__ int bit_length_u64(unsigned long long x) {
 }
 
 // Narrow id addressing: every id position of the rows fits a 32-bit byte
-// offset.
+// offset (and the caller did not ask for the wide form).
 __device__ __forceinline__ bool rows_narrow(const unsigned long long *stats) {
-  return stats[1] <= (1ull << 30);
+  return stats[2] == 0 && stats[1] <= (1ull << 30);
 }
 
 // Layout of the sorted plan's packed words.  The center id fills the top
@@ -761,7 +762,8 @@ static auto heavy_kernel(int unroll) -> decltype(&k_heavy<I, 4, O>) {
 
 __global__ void __launch_bounds__(kBlock)
 k_row_stats(const int64_t *__restrict__ rp64, const int32_t *__restrict__ rp32, int64_t n,
-            unsigned long long *__restrict__ stats) {
+            int force_wide, unsigned long long *__restrict__ stats) {
+  __shared__ unsigned long long wave_max[kBlock / kWave];
   unsigned long long m = 0;
   for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
        u += (int64_t)gridDim.x * blockDim.x) {
@@ -770,17 +772,29 @@ k_row_stats(const int64_t *__restrict__ rp64, const int32_t *__restrict__ rp32, 
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned long long)__shfl_xor(m, off, kWave));
-  if (lane_id() == 0 && m) atomicMax(&stats[0], m);
-  if (blockIdx.x == 0 && threadIdx.x == 0)
+  if (lane_id() == 0) wave_max[threadIdx.x / kWave] = m;
+  __syncthreads();
+  // one same-address atomic per workgroup: thousands per wavefront serialise
+  if (threadIdx.x == 0) {
+    unsigned long long bm = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) bm = max(bm, wave_max[w]);
+    if (bm) atomicMax(&stats[0], bm);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     stats[1] = (unsigned long long)(rp64 ? rp64[n] : (int64_t)rp32[n]);
+    stats[2] = force_wide ? 1ull : 0ull;
+  }
 }
 
-// stats[0..1] = (max forward degree, row_ptr[n]); stats[0] must be zero
-static int launch_row_stats(const void *row_ptr, int row_ptr_64, int64_t n,
+// stats[0..2] = (max forward degree, row_ptr[n], force_wide); stats[0] must
+// be zero
+static int launch_row_stats(const void *row_ptr, int row_ptr_64, int64_t n, bool force_wide,
                             unsigned long long *stats, hipStream_t s) {
-  hipLaunchKernelGGL(k_row_stats, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, s,
+  hipLaunchKernelGGL(k_row_stats, dim3(grid_for(n, kBlock, 1024)), dim3(kBlock), 0, s,
                      row_ptr_64 ? (const int64_t *)row_ptr : nullptr,
-                     row_ptr_64 ? nullptr : (const int32_t *)row_ptr, n, stats);
+                     row_ptr_64 ? nullptr : (const int32_t *)row_ptr, n, force_wide ? 1 : 0,
+                     stats);
   TC_HIP(hipGetLastError());
   return TC_OK;
 }
@@ -933,7 +947,8 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              const int32_t *eu, const int32_t *ev, int64_t E,
                              const uint8_t *skip, const int32_t *graph_of,
                              unsigned long long *count, int light_max, int chunk, int wpc,
-                             int unroll, int blk_bits, bool pack_ids, hipStream_t s) {
+                             int unroll, int blk_bits, bool pack_ids, bool wide,
+                             hipStream_t s) {
   Scratch eu_buf;
   if (!eu) {
     TC_TRY(eu_buf.alloc(sizeof(int32_t) * E, s));
@@ -965,18 +980,18 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   TC_TRY(iptr.alloc(sizeof(int64_t) * (max_runs + 1), s));
   const int64_t max_items = max_runs + E / chunk + 1;
   TC_TRY(items.alloc(sizeof(int2) * max_items, s));
-  TC_TRY(ctr.alloc(6 * sizeof(unsigned long long), s));
+  TC_TRY(ctr.alloc(7 * sizeof(unsigned long long), s));
   unsigned long long *next_item = ctr.as<unsigned long long>();
   unsigned long long *n_heavy = next_item + 1;
-  unsigned long long *row_stats = next_item + 4;  // 2 entries
+  unsigned long long *row_stats = next_item + 4;  // 3 entries
   const PackLayout layout{shift, pack_ids ? 1 : 0, row_stats};
 
   const unsigned gr = grid_for(max_runs + 1, kBlock, 16384);
   const unsigned ge = grid_for(E, kBlock, 16384);
   unsigned long long *next_big = next_item + 2;
   unsigned long long *next_blk = next_item + 3;
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, next_item, 6, 0ull);
-  TC_TRY(launch_row_stats(row_ptr, row_ptr_64, n, row_stats, s));
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, next_item, 7, 0ull);
+  TC_TRY(launch_row_stats(row_ptr, row_ptr_64, n, wide, row_stats, s));
   TC_DISPATCH_ROWS(row_ptr_64, I, {
     Rows<I> R{(const I *)row_ptr};
     {
@@ -1078,7 +1093,8 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
   if (plan == TC_PLAN_SORTED)
     return exact_sorted_plan(row_ptr, row_ptr_64, col, n, eu, ev, E, skip, graph_of, count,
                              light_max, chunk, wpc, unroll, blk_bits,
-                             params && params->pack_ids != 0, s);
+                             params && params->pack_ids != 0,
+                             params && params->wide_offsets != 0, s);
 
   Scratch eu_buf;
   if (!eu) {
@@ -1103,14 +1119,15 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
   TC_TRY(iptr.alloc(sizeof(int64_t) * (n + 1), s));
   const int64_t max_items = n + E / chunk + 1;
   TC_TRY(items.alloc(sizeof(int2) * max_items, s));
-  TC_TRY(ctr.alloc(5 * sizeof(unsigned long long), s));
-  unsigned long long *row_stats = ctr.as<unsigned long long>() + 3;  // 2 entries
+  TC_TRY(ctr.alloc(6 * sizeof(unsigned long long), s));
+  unsigned long long *row_stats = ctr.as<unsigned long long>() + 3;  // 3 entries
 
   const unsigned gn = grid_for(n, kBlock, 16384);
   const unsigned ge = grid_for(E, kBlock, 16384);
   hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(kBlock), 0, s, cnt.as<int32_t>(), n, 0);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr.as<unsigned long long>(), 5, 0ull);
-  TC_TRY(launch_row_stats(row_ptr, row_ptr_64, n, row_stats, s));
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr.as<unsigned long long>(), 6, 0ull);
+  TC_TRY(launch_row_stats(row_ptr, row_ptr_64, n, params && params->wide_offsets != 0,
+                          row_stats, s));
   TC_DISPATCH_ROWS(row_ptr_64, I, {
     Rows<I> R{(const I *)row_ptr};
     {

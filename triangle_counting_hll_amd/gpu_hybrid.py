@@ -79,7 +79,8 @@ def _exact_params(params: Optional[dict]):
                             int(params.get("waves_per_cu", 0)), int(params.get("unroll", 0)),
                             _lib.PLANS[plan] if isinstance(plan, str) else int(plan),
                             int(params.get("block_min_bits", 0)),
-                            int(bool(params.get("pack_ids", False))))
+                            int(bool(params.get("pack_ids", False))),
+                            int(bool(params.get("wide_offsets", False))))
 
 
 def exact_count(row_ptr, col_idx, edge_u=None, edge_v=None, count=None, params=None,
