@@ -216,3 +216,73 @@ def test_reference_launch_objects(device):
         assert [int(x) for x in st] == [ref["stats"]["n_approx"], ref["stats"]["n_exact"],
                                         ref["stats"]["n_clamped"]]
         assert math.isclose(float(counts.item()), ref["total_fsum"], rel_tol=1e-12)
+
+
+class _SlotRows:
+    """Dense-by-vertex view of compact register rows: ``rows[ids]`` returns
+    the rows of those vertices (the oracle indexes registers by vertex id)."""
+
+    def __init__(self, regs, slot):
+        self.regs, self.slot = regs, slot
+
+    def __getitem__(self, ids):
+        s = self.slot[ids]
+        assert (s >= 0).all()
+        return self.regs[s]
+
+
+def test_chunglu_full_size_sketches_and_hybrid(device):
+    # the configured 117M-edge power-law graph with the hybrid's p = 12 and
+    # sketches for forward degree >= 128 in compact slots (125k vertices):
+    # every register equals the C oracle's; per-edge hybrid supports equal the
+    # C oracle's exact supports on every exact-routed edge and the numpy
+    # oracle's clamped HLL supports on a seeded sample of HLL-routed edges
+    import torch
+
+    from oracle import cbind
+
+    p, gamma = 12, 128
+    csr = gpu_graph.build_csr_gpu(config_pairs("chunglu"), row_ptr_dtype="int64")
+    rp = csr.row_ptr.cpu().numpy()
+    col = csr.col_idx.cpu().numpy()
+    deg = csr.degrees
+    assert np.array_equal(deg.cpu().numpy(), np.diff(rp))
+    sk = deg >= gamma
+    n_sk = int(sk.sum().item())
+    assert n_sk == 125_233
+    slot = torch.full_like(deg, -1)
+    slot[sk] = torch.arange(n_sk, dtype=torch.int32, device=deg.device)
+    regs = torch.zeros(n_sk << p, dtype=torch.uint8, device=deg.device)
+    has = torch.zeros_like(deg, dtype=torch.uint8)
+    gpu_hll.build_sketches(csr.row_ptr, csr.col_idx, csr.n_nodes, gamma, p, regs, has,
+                           slot=slot, accumulate=False)
+    slot_h = slot.cpu().numpy().astype(np.int64)
+    regs_h = regs.cpu().numpy().reshape(n_sk, 1 << p)
+    assert np.array_equal(has.cpu().numpy(), (np.diff(rp) >= gamma).astype(np.uint8))
+    assert np.array_equal(regs_h, cbind.hll_registers(rp, col, p, slot_h, n_sk))
+
+    eu, ev = csr.edge_u, csr.edge_v
+    per = torch.zeros(csr.n_oriented, dtype=torch.float64, device=deg.device)
+    c, s, fb = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, eu, ev, deg, regs, has, gamma,
+                                       p, slot=slot, per_edge=per)
+    torch.cuda.synchronize()
+    per_h = per.cpu().numpy()
+    eu_h = eu.cpu().numpy().astype(np.int64)
+    ev_h = ev.cpu().numpy().astype(np.int64)
+    d = np.diff(rp)
+    use = (np.minimum(d[eu_h], d[ev_h]) >= gamma)
+    st = s.cpu().numpy()
+    assert int(st[0]) == int(use.sum()) == 30_997_202
+    assert int(st[1]) == csr.n_oriented - int(use.sum())
+    exact = cbind.edge_supports(rp, col, eu_h, ev_h)
+    assert np.array_equal(per_h[~use], exact[~use].astype(np.float64))
+    # HLL-routed sample against the numpy oracle (register max, FP64 estimate,
+    # support d+u + d+v - union clamped to [0, min])
+    idx = np.sort(np.random.default_rng(5).choice(np.nonzero(use)[0], 20_000, replace=False))
+    ref = O.hybrid_count(rp, col, eu_h[idx], ev_h[idx], _SlotRows(regs_h, slot_h),
+                         has.cpu().numpy(), gamma)
+    assert ref["use_hll"].all()
+    np.testing.assert_allclose(per_h[idx], ref["supports"], rtol=1e-12, atol=1e-9)
+    # the device total is the sum of its per-edge supports (numpy's pairwise
+    # sum is within ~1e-15 relative here)
+    assert math.isclose(float(c.item()), float(np.sum(per_h)), rel_tol=1e-12)
