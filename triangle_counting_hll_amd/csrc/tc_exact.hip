@@ -100,12 +100,49 @@ __device__ __forceinline__ int64_t readlane_i64(int64_t v, int l) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+// One light edge per lane without a chain of dependent loads: the shorter
+// row (at most K ids) is loaded into registers at once, the longer one
+// streamed four ids per step, and every streamed id is compared with the
+// K registers (ids are non-negative; the fill values match nothing).
+template <int K>
+__device__ __forceinline__ uint32_t lane_intersection(const int32_t *__restrict__ col, int64_t s0,
+                                                      int ds, int64_t l0, int dl) {
+  int S[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) S[i] = (i < ds) ? col[s0 + i] : -1;
+  uint32_t c = 0;
+  for (int j = 0; j < dl; j += 4) {
+    int y[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) y[t] = (j + t < dl) ? col[l0 + j + t] : -2;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int i = 0; i < K; ++i) c += (S[i] == y[t]) ? 1u : 0u;
+  }
+  return c;
+}
+
 __device__ __forceinline__ uint32_t light_intersections(const int32_t *__restrict__ col, bool coop,
                                                         int64_t a0, int la, int64_t b0, int lb) {
   uint64_t lm = __ballot(coop);
   if (!lm) return 0;
-  if (__popcll(lm) > kCoopMaxPerRound)
+  if (__popcll(lm) > kCoopMaxPerRound) {
+    // many light edges: one per lane, the register width chosen per wavefront
+    int64_t s0 = a0, l0 = b0;
+    int ds = la, dl = lb;
+    if (la > lb) {
+      s0 = b0;
+      l0 = a0;
+      ds = lb;
+      dl = la;
+    }
+    if (!coop) ds = dl = 0;
+    if (!__ballot(ds > 4)) return lane_intersection<4>(col, s0, ds, l0, dl);
+    if (!__ballot(ds > 8)) return lane_intersection<8>(col, s0, ds, l0, dl);
+    if (!__ballot(ds > 16)) return lane_intersection<16>(col, s0, ds, l0, dl);
     return coop ? merge_count(col, a0, a0 + la, b0, b0 + lb) : 0u;
+  }
   const int lane = lane_id();
   constexpr int kHalf = kWave / 2;
   uint32_t mine = 0;
