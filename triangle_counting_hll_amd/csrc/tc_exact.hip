@@ -842,14 +842,15 @@ static int launch_row_stats(const void *row_ptr, int row_ptr_64, int64_t n, bool
 // a radix sort on the center bits whose first pass drops the sentinels groups
 // the others of each center (in edge order).  No per-edge global atomics.
 // ---------------------------------------------------------------------------
-template <class I>
+template <class I, bool kRoute>
 __global__ void __launch_bounds__(kBlock)
 k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict__ eu,
                 const int32_t *__restrict__ ev, int64_t E, const uint8_t *__restrict__ skip,
                 int light_max, PackLayout L, uint64_t *__restrict__ packed,
                 const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count,
-                unsigned long long *__restrict__ n_heavy, int64_t n) {
+                unsigned long long *__restrict__ n_heavy, int64_t n, HybridRoute hr) {
   uint64_t acc = 0, nh = 0;
+  uint64_t n_use = 0, n_fb = 0;  // kRoute: HLL-routed edges, wanted HLL without sketches
   const int lane = lane_id();
   const bool rows_layout = L.rows();
   const int lbits = L.lbits();
@@ -871,7 +872,24 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
         u = eu[e];
         const int64_t ub = R.begin(u), ue = R.end(u), vb = R.begin(v), ve = R.end(v);
         const int64_t du = ue - ub, dv = ve - vb;
-        if (du > 0 && dv > 0) {
+        bool to_hll = false;
+        if constexpr (kRoute) {
+          // reads only what decides: D(v) once D(u) reaches gamma, the sketch
+          // flags only for edges that want HLL
+          bool wants;
+          if (hr.route) {
+            wants = hr.route[e] != 0;
+          } else {
+            wants = (hr.degrees ? (int64_t)hr.degrees[u] : du) >= hr.gamma &&
+                    (hr.degrees ? (int64_t)hr.degrees[v] : dv) >= hr.gamma;
+          }
+          const bool both = wants && hr.has[u] == 1 && hr.has[v] == 1;
+          to_hll = wants && both;
+          hr.use[e] = to_hll ? 1 : 0;
+          n_use += to_hll ? 1 : 0;
+          n_fb += (wants && !both) ? 1 : 0;
+        }
+        if (!to_hll && du > 0 && dv > 0) {
           if (du + dv <= light_max) {
             if (du <= kWave / 2 && dv <= kWave / 2) {
               coop = true;
@@ -916,22 +934,34 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
   }
   // one atomic per workgroup and counter: same-address atomics from every
   // wavefront of a large grid serialise
-  __shared__ unsigned long long s_red[2][kBlock / kWave];
+  __shared__ unsigned long long s_red[4][kBlock / kWave];
   nh = wave_sum(nh);
   acc = wave_sum(acc);
+  if constexpr (kRoute) {
+    n_use = wave_sum(n_use);
+    n_fb = wave_sum(n_fb);
+  }
   if (lane == 0) {
     s_red[0][threadIdx.x / kWave] = nh;
     s_red[1][threadIdx.x / kWave] = acc;
+    s_red[2][threadIdx.x / kWave] = n_use;
+    s_red[3][threadIdx.x / kWave] = n_fb;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long a = 0, b = 0;
+    unsigned long long a = 0, b = 0, c = 0, d = 0;
     for (int w = 0; w < kBlock / kWave; ++w) {
       a += s_red[0][w];
       b += s_red[1][w];
+      c += s_red[2][w];
+      d += s_red[3][w];
     }
     if (a) atomicAdd(n_heavy, a);
     if (!graph_of && b) atomicAdd(count, b);
+    if constexpr (kRoute) {
+      if (c) atomicAdd(&hr.counters[0], c);
+      if (d) atomicAdd(&hr.counters[1], d);
+    }
   }
 }
 
@@ -985,7 +1015,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              const uint8_t *skip, const int32_t *graph_of,
                              unsigned long long *count, int light_max, int chunk, int wpc,
                              int unroll, int blk_bits, bool pack_ids, bool wide,
-                             hipStream_t s) {
+                             hipStream_t s, const HybridRoute *hr) {
   Scratch eu_buf;
   if (!eu) {
     TC_TRY(eu_buf.alloc(sizeof(int32_t) * E, s));
@@ -1033,9 +1063,14 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     Rows<I> R{(const I *)row_ptr};
     {
       StageTimer _timer(TC_STAGE_CLASSIFY, s);
-      hipLaunchKernelGGL(k_classify_pack<I>, dim3(ge), dim3(kBlock), 0, s, R, col, eu, ev, E,
-                         skip, light_max, layout, packed.as<uint64_t>(), graph_of, count, n_heavy,
-                         n);
+      if (hr)
+        hipLaunchKernelGGL((k_classify_pack<I, true>), dim3(ge), dim3(kBlock), 0, s, R, col, eu,
+                           ev, E, skip, light_max, layout, packed.as<uint64_t>(), graph_of, count,
+                           n_heavy, n, *hr);
+      else
+        hipLaunchKernelGGL((k_classify_pack<I, false>), dim3(ge), dim3(kBlock), 0, s, R, col, eu,
+                           ev, E, skip, light_max, layout, packed.as<uint64_t>(), graph_of, count,
+                           n_heavy, n, HybridRoute{});
     }
     TC_HIP(hipGetLastError());
     // Small inputs are launch-bound: when classify found no heavy edge, the
@@ -1113,7 +1148,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
 int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, int64_t n,
                      const int32_t *eu, const int32_t *ev, int64_t E, const uint8_t *skip,
                      const int32_t *graph_of, unsigned long long *count,
-                     const tc_exact_params *params, hipStream_t s) {
+                     const tc_exact_params *params, hipStream_t s, const HybridRoute *hr) {
   const int light_max = (params && params->light_max > 0) ? params->light_max : 32;
   // at most 65536 others per work item keeps each lane's 32-bit hit count exact
   const int chunk = (params && params->chunk > 0) ? std::min(params->chunk, 65536) : 256;
@@ -1126,12 +1161,14 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                                                               : kBlkMinBits;
   TC_CHECK(blk_bits >= kBigCenterBits && blk_bits <= kBigMinBits, TC_ERR_INVALID,
            "block_min_bits must be in [%d, %d], got %d", kBigCenterBits, kBigMinBits, blk_bits);
+  TC_CHECK(!hr || (plan == TC_PLAN_SORTED && !skip && eu && ev), TC_ERR_INVALID,
+           "hybrid routing in the exact pass needs the sorted plan, an edge list and no skip mask");
   if (E <= 0 || n <= 0) return TC_OK;
   if (plan == TC_PLAN_SORTED)
     return exact_sorted_plan(row_ptr, row_ptr_64, col, n, eu, ev, E, skip, graph_of, count,
                              light_max, chunk, wpc, unroll, blk_bits,
                              params && params->pack_ids != 0,
-                             params && params->wide_offsets != 0, s);
+                             params && params->wide_offsets != 0, s, hr);
 
   Scratch eu_buf;
   if (!eu) {

@@ -6,13 +6,31 @@
 
 namespace tc {
 
+// Hybrid routing taken in the exact pipeline's edge pass (src/gpu_hybrid.py:80-89
+// rule): edge e goes to HLL -- use[e] = 1, no exact count -- iff it wants HLL
+// (route[e] != 0 when route is given, else min(D(u), D(v)) >= gamma with D the
+// degrees array, or the row length when degrees is null) and both endpoints
+// are sketched (has == 1); otherwise use[e] = 0.  counters[0] += HLL edges,
+// counters[1] += edges that wanted HLL without both sketches.
+struct HybridRoute {
+  const int32_t *degrees;
+  const uint8_t *has;
+  int64_t gamma;
+  const uint8_t *route;
+  uint8_t *use;
+  unsigned long long *counters;
+};
+
 // count[0] (or count[graph_of[center]] when graph_of != nullptr) +=
 // sum over edges e with !(skip && skip[e]) of |N+(eu[e]) ∩ N+(ev[e])|.
-// eu == nullptr means the CSR's own COO (ev is then ignored).
+// eu == nullptr means the CSR's own COO (ev is then ignored).  With hr (sorted
+// plan only, skip must be null) the edges hr routes to HLL are left out and
+// hr->use and hr->counters are written for every edge.
 int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, int64_t n,
                      const int32_t *eu, const int32_t *ev, int64_t E, const uint8_t *skip,
                      const int32_t *graph_of, unsigned long long *count,
-                     const tc_exact_params *params, hipStream_t s);
+                     const tc_exact_params *params, hipStream_t s,
+                     const HybridRoute *hr = nullptr);
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v);
 __global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v);

@@ -693,12 +693,17 @@ extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_
     Rows<I> R{(const I *)row_ptr};
     Degrees<I> D{R, degrees};
     if (E > 0) {
-      hipLaunchKernelGGL(k_route<I>, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock), 0, s, D,
-                         edge_u, edge_v, E, has_sketch, gamma, route, use.as<uint8_t>(), counters);
-      TC_HIP(hipGetLastError());
-      if (part != TC_PART_HLL) {
+      if (part == TC_PART_HLL) {
+        hipLaunchKernelGGL(k_route<I>, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock), 0, s, D,
+                           edge_u, edge_v, E, has_sketch, gamma, route, use.as<uint8_t>(),
+                           counters);
+        TC_HIP(hipGetLastError());
+      } else {
+        // the exact pipeline's edge pass routes each edge (same rule as
+        // k_route) and counts only the edges it keeps
+        const HybridRoute hr{degrees, has_sketch, gamma, route, use.as<uint8_t>(), counters};
         TC_TRY(exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, E,
-                                use.as<uint8_t>(), nullptr, exact_sum, nullptr, s));
+                                nullptr, nullptr, exact_sum, nullptr, s, &hr));
         if (per_edge)
           hipLaunchKernelGGL(k_exact_per_edge<I>, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock),
                              0, s, R, col_idx, edge_u, edge_v, E,
