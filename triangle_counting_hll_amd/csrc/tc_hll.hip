@@ -666,6 +666,7 @@ extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_
   TC_CHECK(counts && stats && has_sketch && n_edges >= 0 && n_nodes >= 0, TC_ERR_INVALID,
            "invalid arguments to tc_hybrid_count");
   TC_CHECK(n_edges == 0 || (edge_u && edge_v), TC_ERR_INVALID, "edge_u and edge_v are required");
+  TC_CHECK(n_edges == 0 || n_nodes > 0, TC_ERR_INVALID, "edges need n_nodes > 0");
   TC_CHECK(part == TC_PART_HLL || n_edges == 0 || (row_ptr && col_idx), TC_ERR_INVALID,
            "row_ptr and col_idx are required for the exact part");
   TC_CHECK(row_ptr || degrees, TC_ERR_INVALID, "row_ptr or degrees is required");
