@@ -85,13 +85,13 @@ __device__ __forceinline__ uint32_t merge_count(const int32_t *__restrict__ col,
   return c;
 }
 
-// Light edges of one wavefront round, each with rows of at most 32 ids.  A
-// lane's serial merge is a chain of dependent loads, and the whole wavefront
-// waits for its slowest lane; when a round holds few light edges, the
-// wavefront intersects them one after the other instead: one load puts row a
-// in lanes [0, la) and row b in lanes [32, 32 + lb), then one ballot per id of
-// a counts its match in b.  With many light edges in the round, each lane
-// merges its own.  Returns |a ∩ b| in the edge's lane, 0 elsewhere.
+// Light edges of one wavefront round, each with rows of at most 32 ids.  When
+// a round holds few light edges, the wavefront intersects them one after the
+// other: one load puts row a in lanes [0, la) and row b in lanes [32, 32 + lb),
+// then one ballot per id of a counts its match in b.  With many light edges in
+// the round, each lane intersects its own from registers (lane_intersection;
+// a serial merge, a chain of dependent loads, remains for rows too long for
+// them).  Returns |a ∩ b| in the edge's lane, 0 elsewhere.
 constexpr int kCoopMaxPerRound = 16;
 
 __device__ __forceinline__ int64_t readlane_i64(int64_t v, int l) {
