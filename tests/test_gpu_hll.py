@@ -189,6 +189,42 @@ def test_hybrid_per_edge_route_and_determinism(device):
     assert math.isclose(outs[0], ref["total_fsum"], rel_tol=1e-12)
 
 
+@pytest.mark.parametrize("gamma,theta", [(0, 0), (8, 0), (8, 12), (40, 0), (10**9, 0)])
+def test_hybrid_row_length_degrees_and_parts(device, gamma, theta):
+    # degrees=None routes and estimates with the row lengths, which are the
+    # forward degrees: every output equals the call with the degrees array.
+    # The exact part (routed inside the exact pipeline's edge pass) and the HLL
+    # part (routed by its own pass) add up to the combined call.
+    import torch
+
+    from triangle_counting_hll_amd import _lib
+
+    p = 10
+    rp, col, eu, ev, csr, regs, has, ro, ho = _hybrid_setup(p, theta)
+    E = len(eu)
+
+    def run(degrees, part=_lib.TC_PART_BOTH):
+        per = torch.zeros(E, dtype=torch.float64, device="cuda")
+        c, s, fb = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
+                                           degrees, regs, has, gamma, p, part=part, per_edge=per)
+        torch.cuda.synchronize()
+        return float(c.item()), s.cpu().numpy(), int(fb.item()), per.cpu().numpy()
+
+    c1, s1, f1, p1 = run(csr.degrees)
+    c0, s0, f0, p0 = run(None)
+    assert c0 == c1 and f0 == f1
+    assert np.array_equal(s0, s1) and np.array_equal(p0, p1)
+    ref = O.hybrid_count(rp, col, eu, ev, ro, ho, gamma)
+    assert int(s1[0]) == ref["stats"]["n_approx"] and f1 == ref["stats"]["n_fallback"]
+    ce, se, fe, pe = run(csr.degrees, _lib.TC_PART_EXACT)
+    ch, sh, fh, ph = run(csr.degrees, _lib.TC_PART_HLL)
+    assert math.isclose(ce + ch, c1, rel_tol=1e-12, abs_tol=1e-9)
+    # each part adds its own statistics: exact-routed edges, HLL edges and clamps
+    assert se[1] == s1[1] and se[0] == 0 and se[2] == 0
+    assert sh[0] == s1[0] and sh[2] == s1[2] and sh[1] == 0
+    assert fe == f1 and fh == f1
+
+
 def test_reference_launch_objects(device):
     import torch
 
