@@ -358,13 +358,15 @@ __global__ void k_route(Degrees<I> D, const int32_t *__restrict__ eu, const int3
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int32_t u = eu[e], v = ev[e];
+    // min(D(u), D(v)) >= gamma, reading D(v) only once D(u) reaches gamma and
+    // the sketch flags only for edges that want HLL
     bool wants;
     if (route) {
       wants = route[e] != 0;
     } else {
-      wants = min(D(u), D(v)) >= gamma;
+      wants = D(u) >= gamma && D(v) >= gamma;
     }
-    const bool both = has[u] == 1 && has[v] == 1;
+    const bool both = wants && has[u] == 1 && has[v] == 1;
     const bool hll = wants && both;
     use[e] = hll ? 1 : 0;
     n_use += hll;
