@@ -276,8 +276,11 @@ __global__ void k_item_fill(const int32_t *__restrict__ nitems, const int32_t *_
 // LDS hash set of one center row: buckets of 4 int slots read with one
 // 16-byte LDS load.  A bucket's slots fill in order 0..3 and a key moves to the
 // next bucket only when its home bucket is full, so a lookup stops at the
-// first bucket whose last slot is empty.  Load factor <= 1/4 up to 512 ids and
-// <= 1/2 up to kTblMaxDeg ids.
+// first bucket whose last slot is empty.  Every table uses all the buckets its
+// LDS holds, whatever the row length: the fewer keys per bucket, the rarer a
+// full home bucket, and a wavefront skips the continuation probes of a step
+// when none of its lanes met one.  Load factor <= 1/2 (kTblMaxDeg ids in a
+// 512-bucket wave table).
 __device__ __forceinline__ uint32_t bucket_of(uint32_t x, int shift) {
   return (x * 0x9E3779B1u) >> shift;
 }
@@ -287,14 +290,14 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t x, int shift) {
 template <int kInts, bool kBlockWide>
 __device__ __forceinline__ void table_build(int *__restrict__ tbl, const int32_t *__restrict__ row,
                                             int dc, int &shift, uint32_t &mask) {
+  static_assert(kInts >= 16 && (kInts & (kInts - 1)) == 0, "table of 2^k buckets of 4 ints");
+  constexpr int NB = kInts / 4;
   const int t0 = kBlockWide ? (int)threadIdx.x : lane_id();
   const int stride = kBlockWide ? (int)blockDim.x : kWave;
-  int logNB = 2;
-  while ((1 << logNB) < dc && (4 << logNB) < kInts) ++logNB;
-  const int NB = 1 << logNB;
-  shift = 32 - logNB;
+  shift = 32 - __builtin_ctz((unsigned)NB);
   mask = (uint32_t)NB - 1;
-  for (int i = t0; i < 4 * NB; i += stride) tbl[i] = -1;
+  int4 *tbl4 = reinterpret_cast<int4 *>(tbl);  // the tables are int4 arrays
+  for (int i = t0; i < NB; i += stride) tbl4[i] = make_int4(-1, -1, -1, -1);
   if constexpr (kBlockWide) __syncthreads(); else wave_lds_sync();
   for (int i = t0; i < dc; i += stride) {
     const int x = row[i];
