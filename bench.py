@@ -521,7 +521,7 @@ def main():
     ap.add_argument("--scale-down", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--light-max", type=int, default=32)
-    ap.add_argument("--chunk", type=int, default=256)
+    ap.add_argument("--chunk", type=int, default=512)
     ap.add_argument("--waves-per-cu", type=int, default=16)
     ap.add_argument("--unroll", type=int, default=16)
     ap.add_argument("--extras", default="all",

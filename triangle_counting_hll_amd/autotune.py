@@ -21,9 +21,9 @@ from typing import Callable, Dict, Optional
 
 from . import _lib
 
-DEFAULTS = {"chunk": 256, "unroll": 16, "light_max": 32, "waves_per_cu": 16, "plan": "sorted"}
+DEFAULTS = {"chunk": 512, "unroll": 16, "light_max": 32, "waves_per_cu": 16, "plan": "sorted"}
 SPACE = {
-    "chunk": (128, 256, 512),
+    "chunk": (256, 512, 1024),
     "unroll": (8, 16),
     "light_max": (16, 32, 64),
     "waves_per_cu": (8, 16),

@@ -1154,7 +1154,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                      const tc_exact_params *params, hipStream_t s, const HybridRoute *hr) {
   const int light_max = (params && params->light_max > 0) ? params->light_max : 32;
   // at most 65536 others per work item keeps each lane's 32-bit hit count exact
-  const int chunk = (params && params->chunk > 0) ? std::min(params->chunk, 65536) : 256;
+  const int chunk = (params && params->chunk > 0) ? std::min(params->chunk, 65536) : 512;
   const int wpc = (params && params->waves_per_cu > 0) ? params->waves_per_cu : 16;
   const int unroll = (params && params->unroll > 0) ? params->unroll : kDefaultUnroll;
   TC_CHECK(unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8 || unroll == 16,
