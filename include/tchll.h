@@ -161,7 +161,9 @@ typedef struct tc_exact_params {
 /* count[0] += sum over the given oriented edges (edge_u[e] -> edge_v[e]) of
  * |N+(u) ∩ N+(v)|, with N+(x) = col_idx[row_ptr[x] : row_ptr[x+1]] sorted.
  * `count` is a device uint64.  edge_u/edge_v may both be NULL: the edge list
- * is then the CSR's own COO.  params may be NULL.  Stream-ordered, except that
+ * is then the CSR's own COO, and n_edges must equal row_ptr[n_nodes] (the
+ * pass reads only the positions below both, so a mismatch miscounts but
+ * never reads outside the rows).  params may be NULL.  Stream-ordered, except that
  * for at most 2^24 edges outside stream capture the call synchronises the
  * stream once, after the merge pass, and returns early when no edge needs the
  * hash-probe path (small inputs are bound by launch latency); the same holds

@@ -130,3 +130,37 @@ def test_graph_replays_with_only_light_edges(device):
         torch.cuda.synchronize()
         assert np.array_equal(got.cpu().numpy(), truth), f"replay {i}"
     g.close()
+
+
+def test_batch4096_full_size(device):
+    # the configured many-graph workload (4,096 graphs, seed 3: half ER with n
+    # in [200, 50,000] and p = 8/n, half R-MAT scale 8..16, edge factor 16),
+    # 607M oriented edges in one packed CSR: the eager pipeline and two
+    # replays of the captured launch give every graph the count the C oracle
+    # finds on that graph's edge range of the same CSR
+    import torch
+
+    from triangle_counting_hll_amd import autotune
+
+    spec = batch.many_graphs_spec()
+    assert spec.n_graphs == 4096
+    b = batch.build_batch(spec)
+    rp = b.csr.row_ptr.cpu().numpy().astype(np.int64)
+    col = b.csr.col_idx.cpu().numpy()
+    eu = b.csr.edge_u.cpu().numpy()
+    truth = np.array([cbind.exact_count(rp, col, eu[b.edge_off[g]:b.edge_off[g + 1]],
+                                        col[b.edge_off[g]:b.edge_off[g + 1]])
+                      for g in range(b.n_graphs)], dtype=np.int64)
+    assert (truth > 0).sum() > 4000
+    eager = batch.batch_exact_count(b).cpu().numpy()
+    assert np.array_equal(eager, truth)
+    params = dict(autotune.DEFAULTS, chunk=256, block_min_bits=9)
+    g = batch.BatchCountGraph(b, params)
+    for i in range(2):
+        x = torch.arange(1000, device="cuda").sum()
+        torch.cuda.synchronize()
+        got = g.launch().clone()
+        torch.cuda.synchronize()
+        assert np.array_equal(got.cpu().numpy(), truth), f"replay {i}"
+        assert int(x.item()) == 499500
+    g.close()

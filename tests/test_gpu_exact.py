@@ -1,6 +1,8 @@
 """Exact-count parity on the GPU against the CPU oracle (bit-exact integers)."""
 
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -9,6 +11,10 @@ from oracle import cbind
 from oracle import tc_oracle as O
 from triangle_counting_hll_amd import gpu_graph, gpu_hybrid
 from triangle_counting_hll_amd.graphgen import config_pairs, erdos_renyi, random_geometric, rmat
+
+# oracle counts of the full-size graphs (tests/golden/make_full_size_counts.py)
+with open(os.path.join(os.path.dirname(__file__), "golden", "full_size_counts.json")) as _f:
+    FULL = json.load(_f)
 
 pytestmark = pytest.mark.gpu
 
@@ -225,7 +231,7 @@ def test_chunglu_full_size_matches_cpu_oracle(device):
     rp = csr.row_ptr.cpu().numpy()
     col = csr.col_idx.cpu().numpy()
     truth = cbind.exact_count(rp, col)
-    assert truth == 1_836_795_280
+    assert truth == FULL["chunglu"]["triangles"]
     for params in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"wide_offsets": True}):
         assert _count(csr, params=params) == truth, params
 
@@ -233,15 +239,15 @@ def test_chunglu_full_size_matches_cpu_oracle(device):
 def test_rmat23_full_size_planners_agree(device):
     # the 129M-edge R-MAT scale 23 graph (seed 4) built and counted on the
     # device: the planners and packed-word layouts agree with each other and
-    # with the count a 16-core run of the C oracle gave for this graph
+    # with the C oracle's count of this graph (golden file)
     from triangle_counting_hll_amd import batch
 
     csr = gpu_graph.build_csr_gpu(batch.rmat_gpu(23, 16, seed=4), row_ptr_dtype="int64")
-    assert csr.n_oriented == 129_335_302
+    assert csr.n_oriented == FULL["rmat23"]["oriented_edges"]
     counts = {str(p): _count(csr, params=p)
               for p in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"unroll": 4, "chunk": 64},
                         {"wide_offsets": True})}
-    assert set(counts.values()) == {4_672_556_999}, counts
+    assert set(counts.values()) == {FULL["rmat23"]["triangles"]}, counts
 
 
 def test_rgg21_full_size_matches_cpu_oracle(device):

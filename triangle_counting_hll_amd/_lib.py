@@ -8,6 +8,8 @@ functions here raise ``RuntimeError``.
 from __future__ import annotations
 
 import ctypes
+import functools
+import inspect
 import os
 import threading
 from typing import Optional
@@ -153,6 +155,38 @@ def stream_ptr(stream=None) -> int:
     t = torch()
     s = stream if stream is not None else t.cuda.current_stream()
     return int(s.cuda_stream)
+
+
+def on_stream(fn):
+    """Run a wrapper's whole body on its ``stream`` argument.
+
+    A wrapper that takes ``stream=`` allocates and zeroes accumulators,
+    converts inputs and reads results back through torch, which works on
+    torch's current stream.  When ``stream`` is given and is not the current
+    stream, the body runs with ``stream`` made current (so those steps and the
+    library's kernels are ordered on one stream and a readback waits for the
+    kernels), after ``stream`` has waited for the work already queued on the
+    caller's current stream (which produced the inputs).  Device tensors the
+    wrapper returns belong to ``stream``: a caller that uses them on another
+    stream synchronises with ``stream`` first.
+    """
+    sig = inspect.signature(fn)
+
+    @functools.wraps(fn)
+    def wrapper(*args, **kwargs):
+        bound = sig.bind_partial(*args, **kwargs)
+        s = bound.arguments.get("stream")
+        if s is None:
+            return fn(*args, **kwargs)
+        t = torch()
+        cur = t.cuda.current_stream()
+        if s == cur:
+            return fn(*args, **kwargs)
+        s.wait_stream(cur)
+        with t.cuda.stream(s):
+            return fn(*args, **kwargs)
+
+    return wrapper
 
 
 def ptr(x) -> Optional[int]:

@@ -104,6 +104,7 @@ def _er_from_key(key: int, n: int, m: int) -> np.ndarray:
     return np.stack([u, v], axis=1).astype(np.int32)
 
 
+@_lib.on_stream
 def generate_pairs_gpu(spec: BatchSpec, stream=None):
     """All raw pairs of the batch on the device (int32 [total, 2], global ids)."""
     t = _lib.require_device()
@@ -144,6 +145,7 @@ class BatchedCSR:
     vert_off: np.ndarray   # int64 [G + 1]: graph g owns dense vertices [vert_off[g], vert_off[g+1])
 
 
+@_lib.on_stream
 def build_batch(spec: BatchSpec, pairs=None, stream=None) -> BatchedCSR:
     """Generate (unless ``pairs`` is given) and build the packed CSR on the device."""
     t = _lib.require_device()
@@ -165,6 +167,7 @@ def build_batch(spec: BatchSpec, pairs=None, stream=None) -> BatchedCSR:
     return BatchedCSR(csr, graph_of, G, edge_off, vert_off)
 
 
+@_lib.on_stream
 def batch_exact_count(b: BatchedCSR, params: Optional[dict] = None, counts=None, stream=None):
     """Per-graph exact counts (device uint64 [G] viewed as int64), one eager pipeline."""
     t = _lib.require_device()
@@ -185,6 +188,7 @@ class BatchCountGraph:
     After a launch completes, ``counts`` holds the per-graph counts.
     """
 
+    @_lib.on_stream
     def __init__(self, b: BatchedCSR, params: Optional[dict] = None, stream=None):
         t = _lib.require_device()
         self._b = b
@@ -202,6 +206,7 @@ class BatchCountGraph:
         self._h = h
         self.workspace_bytes = ws.value
 
+    @_lib.on_stream
     def launch(self, stream=None):
         _lib.check(_lib.load().tc_batch_graph_launch(self._h, _lib.stream_ptr(stream)))
         return self.counts

@@ -54,13 +54,27 @@ __global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long 
     p[i] = v;
 }
 
+// CSR mode (no edge list): eu[k] = the row of position k, for the positions
+// k < E that the rows reach.  The caller's n_edges (E) should equal
+// row_ptr[n]; positions past row_ptr[n] are left unwritten and the edge
+// passes skip them (edge_limit), positions the rows reach past E are never
+// written.
 template <class I>
-__global__ void k_expand_rows(const I *__restrict__ rp, int64_t n, int32_t *__restrict__ eu) {
+__global__ void k_expand_rows(const I *__restrict__ rp, int64_t n, int64_t E,
+                              int32_t *__restrict__ eu) {
   const int lane = lane_id();
   const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
-  for (int64_t u = wid; u < n; u += nw)
-    for (int64_t k = (int64_t)rp[u] + lane; k < (int64_t)rp[u + 1]; k += kWave) eu[k] = (int32_t)u;
+  for (int64_t u = wid; u < n; u += nw) {
+    const int64_t hi = min((int64_t)rp[u + 1], E);
+    for (int64_t k = (int64_t)rp[u] + lane; k < hi; k += kWave) eu[k] = (int32_t)u;
+  }
+}
+
+// Number of edges an edge pass may read: E, or in CSR mode min(E, row_ptr[n])
+// (row_ptr[n] measured on the device, stats[1] of k_row_stats).
+__device__ __forceinline__ int64_t edge_limit(int64_t E, const unsigned long long *rows_total) {
+  return rows_total ? min(E, (int64_t)*rows_total) : E;
 }
 
 __device__ __forceinline__ uint32_t merge_count(const int32_t *__restrict__ col, int64_t a0,
@@ -188,10 +202,11 @@ __device__ __forceinline__ uint32_t light_intersections(const int32_t *__restric
 template <class I>
 __global__ void __launch_bounds__(kBlock)
 k_classify(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict__ eu,
-           const int32_t *__restrict__ ev, int64_t E, const uint8_t *__restrict__ skip,
-           int light_max, int32_t *__restrict__ center_cnt,
+           const int32_t *__restrict__ ev, int64_t E, const unsigned long long *rows_total,
+           const uint8_t *__restrict__ skip, int light_max, int32_t *__restrict__ center_cnt,
            const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count) {
   uint64_t acc = 0;
+  E = edge_limit(E, rows_total);
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
        e += (int64_t)gridDim.x * blockDim.x) {
     if (skip && skip[e]) continue;
@@ -219,9 +234,11 @@ k_classify(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict
 template <class I>
 __global__ void __launch_bounds__(kBlock)
 k_scatter_others(Rows<I> R, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
-                 int64_t E, const uint8_t *__restrict__ skip, int light_max,
+                 int64_t E, const unsigned long long *rows_total,
+                 const uint8_t *__restrict__ skip, int light_max,
                  const int64_t *__restrict__ cptr, int32_t *__restrict__ cursor,
                  int32_t *__restrict__ others) {
+  E = edge_limit(E, rows_total);
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
        e += (int64_t)gridDim.x * blockDim.x) {
     if (skip && skip[e]) continue;
@@ -848,8 +865,9 @@ static int launch_row_stats(const void *row_ptr, int row_ptr_64, int64_t n, bool
 template <class I, bool kRoute>
 __global__ void __launch_bounds__(kBlock)
 k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict__ eu,
-                const int32_t *__restrict__ ev, int64_t E, const uint8_t *__restrict__ skip,
-                int light_max, PackLayout L, uint64_t *__restrict__ packed,
+                const int32_t *__restrict__ ev, int64_t E, const unsigned long long *rows_total,
+                const uint8_t *__restrict__ skip, int light_max, PackLayout L,
+                uint64_t *__restrict__ packed,
                 const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count,
                 unsigned long long *__restrict__ n_heavy, int64_t n, HybridRoute hr) {
   uint64_t acc = 0, nh = 0;
@@ -858,6 +876,7 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
   const bool rows_layout = L.rows();
   const int lbits = L.lbits();
   const uint64_t sentinel = (uint64_t)n << L.shift;
+  const int64_t E_read = edge_limit(E, rows_total);  // edges past it get the sentinel
   // the loop bound is uniform per wavefront, so every lane stays active for
   // the cross-lane aggregation below
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < E;
@@ -870,7 +889,7 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
     int la = 0, lb = 0;
     if (e < E) {
       uint64_t word = sentinel;
-      if (!(skip && skip[e])) {
+      if (e < E_read && !(skip && skip[e])) {
         const int32_t v = ev[e];
         u = eu[e];
         const int64_t ub = R.begin(u), ue = R.end(u), vb = R.begin(v), ve = R.end(v);
@@ -1020,11 +1039,12 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              int unroll, int blk_bits, bool pack_ids, bool wide,
                              hipStream_t s, const HybridRoute *hr) {
   Scratch eu_buf;
+  const bool csr_mode = eu == nullptr;
   if (!eu) {
     TC_TRY(eu_buf.alloc(sizeof(int32_t) * E, s));
     TC_DISPATCH_ROWS(row_ptr_64, I, {
       hipLaunchKernelGGL(k_expand_rows<I>, dim3(grid_for(n * kWave, kBlock, 65536)),
-                         dim3(kBlock), 0, s, (const I *)row_ptr, n, eu_buf.as<int32_t>());
+                         dim3(kBlock), 0, s, (const I *)row_ptr, n, E, eu_buf.as<int32_t>());
     });
     TC_HIP(hipGetLastError());
     eu = eu_buf.as<int32_t>();
@@ -1055,6 +1075,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   unsigned long long *n_heavy = next_item + 1;
   unsigned long long *row_stats = next_item + 4;  // 3 entries
   const PackLayout layout{shift, pack_ids ? 1 : 0, row_stats};
+  const unsigned long long *rows_total = csr_mode ? row_stats + 1 : nullptr;
 
   const unsigned gr = grid_for(max_runs + 1, kBlock, 16384);
   const unsigned ge = grid_for(E, kBlock, 16384);
@@ -1068,12 +1089,12 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
       StageTimer _timer(TC_STAGE_CLASSIFY, s);
       if (hr)
         hipLaunchKernelGGL((k_classify_pack<I, true>), dim3(ge), dim3(kBlock), 0, s, R, col, eu,
-                           ev, E, skip, light_max, layout, packed.as<uint64_t>(), graph_of, count,
-                           n_heavy, n, *hr);
+                           ev, E, rows_total, skip, light_max, layout, packed.as<uint64_t>(),
+                           graph_of, count, n_heavy, n, *hr);
       else
         hipLaunchKernelGGL((k_classify_pack<I, false>), dim3(ge), dim3(kBlock), 0, s, R, col, eu,
-                           ev, E, skip, light_max, layout, packed.as<uint64_t>(), graph_of, count,
-                           n_heavy, n, HybridRoute{});
+                           ev, E, rows_total, skip, light_max, layout, packed.as<uint64_t>(),
+                           graph_of, count, n_heavy, n, HybridRoute{});
     }
     TC_HIP(hipGetLastError());
     // Small inputs are launch-bound: when classify found no heavy edge, the
@@ -1174,11 +1195,12 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                              params && params->wide_offsets != 0, s, hr);
 
   Scratch eu_buf;
+  const bool csr_mode = eu == nullptr;
   if (!eu) {
     TC_TRY(eu_buf.alloc(sizeof(int32_t) * E, s));
     TC_DISPATCH_ROWS(row_ptr_64, I, {
       hipLaunchKernelGGL(k_expand_rows<I>, dim3(grid_for(n * kWave, kBlock, 65536)),
-                         dim3(kBlock), 0, s, (const I *)row_ptr, n, eu_buf.as<int32_t>());
+                         dim3(kBlock), 0, s, (const I *)row_ptr, n, E, eu_buf.as<int32_t>());
     });
     TC_HIP(hipGetLastError());
     eu = eu_buf.as<int32_t>();
@@ -1198,6 +1220,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
   TC_TRY(items.alloc(sizeof(int2) * max_items, s));
   TC_TRY(ctr.alloc(6 * sizeof(unsigned long long), s));
   unsigned long long *row_stats = ctr.as<unsigned long long>() + 3;  // 3 entries
+  const unsigned long long *rows_total = csr_mode ? row_stats + 1 : nullptr;
 
   const unsigned gn = grid_for(n, kBlock, 16384);
   const unsigned ge = grid_for(E, kBlock, 16384);
@@ -1209,15 +1232,15 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
     Rows<I> R{(const I *)row_ptr};
     {
       StageTimer _timer(TC_STAGE_CLASSIFY, s);
-      hipLaunchKernelGGL(k_classify<I>, dim3(ge), dim3(kBlock), 0, s, R, col, eu, ev, E, skip,
-                         light_max, cnt.as<int32_t>(), graph_of, count);
+      hipLaunchKernelGGL(k_classify<I>, dim3(ge), dim3(kBlock), 0, s, R, col, eu, ev, E,
+                         rows_total, skip, light_max, cnt.as<int32_t>(), graph_of, count);
     }
     TC_HIP(hipGetLastError());
     TC_TRY(exclusive_scan<int32_t>(cnt.as<int32_t>(), cptr.as<int64_t>(), n, nullptr, s));
     // reuse nitems as the scatter cursor, then as the per-center item count
     hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(kBlock), 0, s, nitems.as<int32_t>(), n, 0);
-    hipLaunchKernelGGL(k_scatter_others<I>, dim3(ge), dim3(kBlock), 0, s, R, eu, ev, E, skip,
-                       light_max, (const int64_t *)cptr.as<int64_t>(), nitems.as<int32_t>(),
+    hipLaunchKernelGGL(k_scatter_others<I>, dim3(ge), dim3(kBlock), 0, s, R, eu, ev, E,
+                       rows_total, skip, light_max, (const int64_t *)cptr.as<int64_t>(), nitems.as<int32_t>(),
                        others.as<int32_t>());
     TC_HIP(hipGetLastError());
     // order centers longest row first (stable bucket sort on bit length)

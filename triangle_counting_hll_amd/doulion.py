@@ -33,6 +33,7 @@ def _pcg64_state(rng: np.random.Generator):
     return int(st["state"]["state"]), int(st["state"]["inc"])
 
 
+@_lib.on_stream
 def sample_mask_gpu(n: int, q: float, rng: np.random.Generator, stream=None):
     """``rng.random(n) < q`` as a uint8 CUDA tensor; ``rng`` advances by ``n`` draws."""
     t = _lib.require_device()
@@ -45,6 +46,7 @@ def sample_mask_gpu(n: int, q: float, rng: np.random.Generator, stream=None):
     return keep
 
 
+@_lib.on_stream
 def edge_subgraph(row_ptr, col_idx, keep, stream=None):
     """Forward CSR of the kept edges (CSR order): ``(row_ptr int64[n+1], col_idx int32)``."""
     t = _lib.require_device()

@@ -59,6 +59,7 @@ class DeviceCSR(NamedTuple):
     n_oriented: int
 
 
+@_lib.on_stream
 def build_csr_gpu(pairs, row_ptr_dtype: str = "auto", stream=None) -> DeviceCSR:
     """Raw undirected pairs (``int32[m, 2]``, host or device) -> oriented CSR on the GPU.
 
@@ -96,6 +97,7 @@ def build_csr_gpu(pairs, row_ptr_dtype: str = "auto", stream=None) -> DeviceCSR:
     return DeviceCSR(row_ptr, col, deg, nid, eu, ev, udeg, n_nodes, n_edges)
 
 
+@_lib.on_stream
 def degree_buckets(row_ptr, stream=None):
     """Degree-bucket sort of the vertices: ``(bucket_start int64[34], perm int32[n])``.
 

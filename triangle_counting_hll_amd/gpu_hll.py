@@ -64,6 +64,7 @@ def alloc_sketch_arrays_device(n_rows: int, p: int, n_nodes: int = None, device=
             t.zeros(n_nodes, dtype=t.uint8, device=dev))
 
 
+@_lib.on_stream
 def build_sketches(row_ptr, col_idx, n_nodes: int, theta: int, p: int, sketch_regs,
                    has_sketch, slot=None, accumulate: bool = True, hash_mode: str = "gpu",
                    seed: int = 0, stream=None) -> None:
@@ -130,6 +131,7 @@ class _BuildSketchesLauncher:
 build_sketches_kernel = _BuildSketchesLauncher()
 
 
+@_lib.on_stream
 def union_estimate_pairs(sketch_regs, us, vs, p: int, slot=None, hash_mode: str = "gpu",
                          stream=None):
     """FP64 estimates of ``|N+(u) ∪ N+(v)|`` for each pair (device float64 tensor).
@@ -153,6 +155,7 @@ def union_estimate_pairs(sketch_regs, us, vs, p: int, slot=None, hash_mode: str 
     return out
 
 
+@_lib.on_stream
 def splitmix64(x, stream=None):
     """Device splitmix64 finaliser of a uint64/int64 tensor (src/gpu_hll.py:12-17)."""
     t = _lib.require_device()

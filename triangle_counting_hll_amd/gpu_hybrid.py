@@ -53,6 +53,7 @@ def _warn_if_float64_atomic_slow(cuda_module=None) -> None:
     _FLOAT64_ATOMIC_WARNED = True
 
 
+@_lib.on_stream
 def flag_indices(flags, stream=None):
     """Indices of the nonzero entries of a uint8/bool device array, ascending
     (int32 device tensor): the device form of the prototype's edge partition
@@ -83,6 +84,7 @@ def _exact_params(params: Optional[dict]):
                             int(bool(params.get("wide_offsets", False))))
 
 
+@_lib.on_stream
 def exact_count(row_ptr, col_idx, edge_u=None, edge_v=None, count=None, params=None,
                 stream=None):
     """Enqueue the exact count; returns the device uint64 accumulator (``count`` if given)."""
@@ -91,8 +93,12 @@ def exact_count(row_ptr, col_idx, edge_u=None, edge_v=None, count=None, params=N
     col = _lib.as_device(col_idx, t.int32, "col_idx")
     n = rp.numel() - 1
     if edge_u is None:
+        # CSR mode: the edge list is the CSR's own COO, so col_idx must hold
+        # exactly row_ptr[n] ids
         eu = ev = None
         E = col.numel()
+        if int(rp[-1].item()) != E:
+            raise ValueError(f"row_ptr[-1] = {int(rp[-1].item())} but col_idx has {E} ids")
     else:
         eu = _lib.as_device(edge_u, t.int32, "edge_u")
         ev = _lib.as_device(edge_v, t.int32, "edge_v")
@@ -109,6 +115,7 @@ def exact_count(row_ptr, col_idx, edge_u=None, edge_v=None, count=None, params=N
     return count
 
 
+@_lib.on_stream
 def edge_supports(row_ptr, col_idx, edge_u, edge_v, stream=None):
     """Per-edge ``|N+(u) ∩ N+(v)|`` as a device int32 tensor."""
     t = _lib.require_device()
@@ -123,6 +130,7 @@ def edge_supports(row_ptr, col_idx, edge_u, edge_v, stream=None):
     return out
 
 
+@_lib.on_stream
 def hybrid_count(row_ptr, col_idx, edge_u, edge_v, degrees, sketch_regs, has_sketch, gamma,
                  p, part=_lib.TC_PART_BOTH, counts=None, stats=None, fallback=None,
                  route=None, slot=None, per_edge=None, hash_mode: str = "gpu", stream=None):

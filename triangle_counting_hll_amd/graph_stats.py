@@ -26,6 +26,7 @@ from . import _lib, gpu_graph, ingest
 COMMENT_CHARS = "#%"  # experiment/gpu_graph_stats.py:32
 
 
+@_lib.on_stream
 def degree_count_gpu(pairs, n_slots: int, stream=None):
     """Undirected degree per id (int32[n_slots]) of raw pairs ``int32[m, 2]`` on the device."""
     t = _lib.require_device()
@@ -38,6 +39,7 @@ def degree_count_gpu(pairs, n_slots: int, stream=None):
     return deg
 
 
+@_lib.on_stream
 def degree_summary_gpu(degree, stream=None) -> dict:
     """Summary of the positive degrees, with the reference's formulas
     (gpu_graph_stats.py:243-257)."""
@@ -70,6 +72,7 @@ def n_log_buckets_for(max_degree: int) -> int:
     return max(1, int(math.floor(math.log2(max_degree))) + 1) if max_degree > 0 else 1
 
 
+@_lib.on_stream
 def degree_histograms_gpu(degree, max_degree: int, stream=None):
     """(hist int64[max_degree + 1], log_hist int64[n_log_buckets]) as host arrays."""
     t = _lib.require_device()

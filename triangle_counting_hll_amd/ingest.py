@@ -63,6 +63,7 @@ def _device_text(text, t):
     return t.from_numpy(arr.copy()).cuda()
 
 
+@_lib.on_stream
 def _parse_device(buf, n_bytes: int, comment_chars: str, out, stream=None):
     """Run tc_parse_edge_list on ``buf[:n_bytes]`` writing into ``out`` (int32 [cap, 2]).
 
@@ -87,6 +88,7 @@ def max_pairs(n_bytes: int) -> int:
     return (n_bytes + 1) // 4 + 1
 
 
+@_lib.on_stream
 def parse_edge_list_gpu(text, comment_chars: str = COMMENT_CHARS, stream=None):
     """Edge-list text (bytes / str / uint8 array on the host, or a uint8 CUDA tensor)
     -> raw pairs ``int32[m, 2]`` on the device, in file order, self-loops dropped."""
@@ -104,6 +106,7 @@ def parse_edge_list_gpu(text, comment_chars: str = COMMENT_CHARS, stream=None):
     return out[:n]
 
 
+@_lib.on_stream
 def count_edge_list_gpu(text, comment_chars: str = COMMENT_CHARS, stream=None) -> int:
     """Number of pairs ``parse_edge_list_gpu`` would return (the first, counting pass only)."""
     t = _lib.require_device()
@@ -114,6 +117,7 @@ def count_edge_list_gpu(text, comment_chars: str = COMMENT_CHARS, stream=None) -
     return n
 
 
+@_lib.on_stream
 def format_edge_list_gpu(pairs, sep: str = "\t", newline: str = "\n", stream=None):
     """Raw pairs (``int32[m, 2]``, ids >= 0, host or device) -> edge-list text as a
     uint8 CUDA tensor, one ``u<sep>v<newline>`` line per pair (``tc_format_edge_list``)."""

@@ -138,6 +138,7 @@ def candidate_edges(csr, sk: Sketches, min_cost_degree: int = 0):
     return t.nonzero(ok).flatten().to(t.int32)
 
 
+@_lib.on_stream
 def features(csr, sk: Sketches, cand, sample_bits: int = 8, stream=None):
     t = _lib.require_device()
     n = cand.numel()
@@ -180,6 +181,7 @@ def feature_norm(feats) -> np.ndarray:
 # Training (GPU kernels) and inference
 # ---------------------------------------------------------------------------
 
+@_lib.on_stream
 def mlp_forward(params, norm, feats, stream=None):
     t = _lib.require_device()
     pr = _lib.as_device(params, t.float32)
@@ -191,6 +193,7 @@ def mlp_forward(params, norm, feats, stream=None):
     return out
 
 
+@_lib.on_stream
 def mlp_grad(params, norm, feats, target, pos_weight: float = 1.0, stream=None):
     """(grad float32 [513], loss float32 [1]) of the mean weighted BCE."""
     t = _lib.require_device()
@@ -206,6 +209,7 @@ def mlp_grad(params, norm, feats, target, pos_weight: float = 1.0, stream=None):
     return g, loss
 
 
+@_lib.on_stream
 def mlp_adam(params, grad, m, v, step: int, lr: float = 1e-3, beta1: float = 0.9,
              beta2: float = 0.999, eps: float = 1e-8, stream=None):
     _lib.check(_lib.load().tc_mlp_adam(_lib.ptr(params), _lib.ptr(grad), _lib.ptr(m), _lib.ptr(v),
@@ -213,6 +217,7 @@ def mlp_adam(params, grad, m, v, step: int, lr: float = 1e-3, beta1: float = 0.9
                                        _lib.stream_ptr(stream)))
 
 
+@_lib.on_stream
 def mlp_route_bf16(params, norm, feats, threshold: float, cand=None, route=None, prob=True,
                    stream=None):
     """BF16 inference; returns probabilities (when ``prob``) and fills ``route``."""
@@ -277,6 +282,7 @@ def choose_threshold(prob, err, truth_total: float, budget: float) -> float:
     return float(prob[order[k - 1]].item())
 
 
+@_lib.on_stream
 def route(model: RouterModel, csr, sk: Sketches, stream=None):
     """Per-edge uint8 route (1 = HLL) from the network; candidates only."""
     t = _lib.require_device()
@@ -289,8 +295,14 @@ def route(model: RouterModel, csr, sk: Sketches, stream=None):
     return r, int(cand.numel())
 
 
-def routed_count(model: RouterModel, csr, sk: Optional[Sketches] = None, stream=None):
-    """Hybrid count with the learned router; returns (estimate, stats dict)."""
+@_lib.on_stream
+def routed_count(model: RouterModel, csr, sk: Optional[Sketches] = None, per_edge=None,
+                 stream=None):
+    """Hybrid count with the learned router; returns (estimate, stats dict).
+
+    ``per_edge`` (float64 device [E], optional) receives every edge's support.
+    ``stats["use"]`` is the per-edge route taken (uint8 device, 1 = HLL).
+    """
     t = _lib.require_device()
     if sk is None:
         # only vertices that can be an endpoint of a candidate need a sketch
@@ -298,14 +310,18 @@ def routed_count(model: RouterModel, csr, sk: Optional[Sketches] = None, stream=
         if int(csr.degrees.max().item() if csr.n_nodes else 0) < need:
             count = gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
                                            stream=stream)
-            return float(count.item()), {"n_approx": 0, "n_exact": csr.n_oriented,
-                                         "n_clamped": 0, "n_candidates": 0, "n_sketched": 0}
+            if per_edge is not None:
+                per_edge.copy_(gpu_hybrid.edge_supports(csr.row_ptr, csr.col_idx, csr.edge_u,
+                                                        csr.edge_v).double())
+            return float(count.item()), {
+                "n_approx": 0, "n_exact": csr.n_oriented, "n_clamped": 0, "n_candidates": 0,
+                "n_sketched": 0, "use": t.zeros(csr.n_oriented, dtype=t.uint8, device="cuda")}
         sk = build_sketches(csr, need, model.p)
     r, n_cand = route(model, csr, sk, stream)
     c, s, fb = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
                                        csr.degrees, sk.regs, sk.has, 0, model.p, route=r,
-                                       slot=sk.slot, stream=stream)
+                                       slot=sk.slot, per_edge=per_edge, stream=stream)
     st = s.cpu().numpy()
     return float(c.item()), {"n_approx": int(st[0]), "n_exact": int(st[1]),
                              "n_clamped": int(st[2]), "n_candidates": n_cand,
-                             "n_sketched": sk.n_sketched}
+                             "n_sketched": sk.n_sketched, "use": r}
