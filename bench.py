@@ -550,14 +550,27 @@ def run_batch_extra(args, reps=10):
     }
 
 
-def run_router_extra(reps=3, p=12, theta=128, budget=0.005):
-    """Learned router: trained on R-MAT scale 20 (seed 5), evaluated on R-MAT
-    scale 23 (seed 4), HLL p = 12, against the fixed-threshold hybrid."""
+def run_router_extra(reps=3, p=12, theta=256, rel_tol=0.02, budget=0.005, gates=(0, 512)):
+    """Learned router on R-MAT scale 23 (seed 4), HLL p = 12, against the
+    fixed-threshold hybrid on the same graph.
+
+    Training: labels from R-MAT scale 20 (seed 5): an edge is positive iff its
+    clamped HLL support is within max(1, rel_tol * exact) of its exact
+    support; FP32 training with the library's gradient and Adam kernels.
+    Threshold: calibrated on a held-out graph, R-MAT scale 21 (seed 6), as the
+    lowest network output whose routed edges keep the sum of absolute per-edge
+    errors within ``budget`` of that graph's count.  Evaluation: the network
+    runs inside the exact pass (``tc_router_hybrid_count``) on the scale-23
+    graph, with and without a cost gate (candidates need min(d+u, d+v) >=
+    gate); timed = sketch build + the routed count.  Beside it: the degree
+    rule routing the same number of edges (the candidates with the largest
+    min(d+u, d+v)), which shows what the network adds at equal coverage."""
     import torch
 
     from triangle_counting_hll_amd import batch, gpu_graph, gpu_hybrid, router
 
-    out = {"p": p, "theta": theta, "error_budget_train": budget}
+    out = {"p": p, "theta": theta, "label_rel_tol": rel_tol, "calibration_budget": budget,
+           "calibration_rule": "absolute"}
     # ---- training data from the scale-20 graph
     t0 = time.perf_counter()
     csr = gpu_graph.build_csr_gpu(batch.rmat_gpu(20, 16, seed=5))
@@ -565,16 +578,26 @@ def run_router_extra(reps=3, p=12, theta=128, budget=0.005):
     cand = router.candidate_edges(csr, sk)
     feats = router.features(csr, sk, cand)
     exact, s_hll = router.edge_errors(csr, sk, cand)
-    labels = router.make_labels(exact, s_hll)
-    truth20 = int(gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v).item())
+    labels = router.make_labels(exact, s_hll, rel_tol=rel_tol)
     params, norm, hist = router.train(feats, labels)
-    prob = router.mlp_route_bf16(params, norm, feats, 0.5)
-    thr = router.choose_threshold(prob, s_hll - exact, truth20, budget)
-    model = router.RouterModel(params, norm, thr, p, 8, theta)
-    out["train"] = {"edges": csr.n_oriented, "candidates": int(cand.numel()),
+    out["train"] = {"graph": "rmat20 seed 5", "edges": csr.n_oriented,
+                    "candidates": int(cand.numel()),
                     "positive_frac": float(labels.mean().item()), "loss": hist,
-                    "threshold": thr, "seconds": time.perf_counter() - t0}
-    del csr, sk, cand, feats, exact, s_hll, labels, prob
+                    "seconds": time.perf_counter() - t0}
+    del csr, sk, cand, feats, exact, s_hll, labels
+    torch.cuda.empty_cache()
+
+    # ---- threshold on the held-out scale-21 graph
+    csr = gpu_graph.build_csr_gpu(batch.rmat_gpu(21, 16, seed=6))
+    sk = router.build_sketches(csr, theta, p)
+    cand = router.candidate_edges(csr, sk)
+    exact, s_hll = router.edge_errors(csr, sk, cand)
+    truth21 = int(gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v).item())
+    prob = router.mlp_route_bf16(params, norm, router.features(csr, sk, cand), 0.5)
+    thr = router.choose_threshold(prob, s_hll - exact, truth21, budget, rule="absolute")
+    out["calibration"] = {"graph": "rmat21 seed 6", "edges": csr.n_oriented,
+                          "candidates": int(cand.numel()), "threshold": thr}
+    del csr, sk, cand, exact, s_hll, prob
     torch.cuda.empty_cache()
 
     # ---- evaluation graph
@@ -587,7 +610,8 @@ def run_router_extra(reps=3, p=12, theta=128, budget=0.005):
 
     _, exact_ms = _timed(exact_fn, reps)
     truth = int(count.item())
-    out["eval"] = {"graph": "rmat23 seed 4", "n_nodes": csr.n_nodes, "edges": csr.n_oriented,
+    E = csr.n_oriented
+    out["eval"] = {"graph": "rmat23 seed 4", "n_nodes": csr.n_nodes, "edges": E,
                    "triangles": truth, "exact_ms": exact_ms}
 
     def fixed(gamma):
@@ -600,28 +624,55 @@ def run_router_extra(reps=3, p=12, theta=128, budget=0.005):
         (c, s), ms = _timed(fn, reps)
         est = float(c.item())
         return {"ms": ms, "rel_err": gpu_hybrid.relative_error(est, truth),
-                "coverage": int(s[0].item()) / csr.n_oriented}
+                "n_approx": int(s[0].item()), "coverage": int(s[0].item()) / E}
 
-    out["fixed_threshold"] = {f"gamma{g}": fixed(g) for g in (64, 128, 256, 512, 1024)}
+    out["fixed_threshold"] = {f"gamma{g}": fixed(g) for g in (64, 128, 256, 512, 768, 1024)}
+
+    sk = router.build_sketches(csr, theta, p)
+    deg = csr.degrees
+    mind = torch.minimum(deg[csr.edge_u.long()], deg[csr.edge_v.long()])
+    both = (sk.has[csr.edge_u.long()] == 1) & (sk.has[csr.edge_v.long()] == 1)
 
     def learned(gate):
         m = router.RouterModel(params, norm, thr, p, 8, theta, min_cost_degree=gate)
 
         def fn():
-            return router.routed_count(m, csr)
+            skl = router.build_sketches(csr, max(theta, gate), p)
+            return router.routed_count(m, csr, skl)
         (est, st), ms = _timed(fn, reps)
-        return {"ms": ms, "rel_err": gpu_hybrid.relative_error(est, truth),
-                "coverage": st["n_approx"] / csr.n_oriented, "candidates": st["n_candidates"],
-                "min_cost_degree": gate}
+        r = {"ms": ms, "rel_err": gpu_hybrid.relative_error(est, truth),
+             "n_approx": st["n_approx"], "coverage": st["n_approx"] / E,
+             "network_evaluated_on": st["n_candidates"], "min_cost_degree": gate,
+             "fused": True}
+        # the degree rule sending as many edges to HLL: the candidates with
+        # the largest min(d+u, d+v) (ties broken by edge order)
+        k = st["n_approx"]
+        if k:
+            key = torch.where(both & (mind >= gate), mind.to(torch.int64), torch.full_like(
+                mind, -1, dtype=torch.int64))
+            top = torch.topk(key * E + (E - 1 - torch.arange(E, device="cuda")), k).indices
+            route = torch.zeros(E, dtype=torch.uint8, device="cuda")
+            route[top] = 1
+            c, s, _ = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
+                                              deg, sk.regs, sk.has, 0, p, route=route,
+                                              slot=sk.slot)
+            r["degree_rule_same_coverage_rel_err"] = gpu_hybrid.relative_error(float(c.item()),
+                                                                               truth)
+            del route, top, key
+        return r
 
-    out["learned"] = {"accuracy_policy": learned(0), "cost_gated": learned(1 << (p - 1))}
-    ok_fixed = [v["ms"] for v in out["fixed_threshold"].values() if v["rel_err"] <= 0.01]
-    ok_learned = [v["ms"] for v in out["learned"].values() if v["rel_err"] <= 0.01]
-    if ok_fixed and ok_learned:
-        out["speedup_learned_vs_fixed_at_1pct"] = min(ok_fixed) / min(ok_learned)
+    out["learned"] = {f"gate{g}": learned(g) for g in gates}
+    ok_fixed = {k: v for k, v in out["fixed_threshold"].items() if v["rel_err"] <= 0.01}
+    ok_learned = {k: v for k, v in out["learned"].items() if v["rel_err"] <= 0.01}
+    if ok_fixed:
+        kf = min(ok_fixed, key=lambda k: ok_fixed[k]["ms"])
+        out["fastest_fixed_within_1pct"] = dict(ok_fixed[kf], rule=kf)
     if ok_learned:
-        out["speedup_learned_vs_fastest_fixed"] = (
-            min(v["ms"] for v in out["fixed_threshold"].values()) / min(ok_learned))
+        kl = min(ok_learned, key=lambda k: ok_learned[k]["ms"])
+        out["fastest_learned_within_1pct"] = dict(ok_learned[kl], policy=kl)
+    if ok_fixed and ok_learned:
+        out["wall_time_learned_over_fixed_within_1pct"] = (
+            out["fastest_learned_within_1pct"]["ms"] / out["fastest_fixed_within_1pct"]["ms"])
     return out
 
 

@@ -307,6 +307,35 @@ int tc_mlp_route_bf16(const float *params, const float *norm, const float *feats
                       float threshold, const int32_t *cand, uint8_t *route, float *prob,
                       void *stream);
 
+/* The network and its inputs for tc_router_hybrid_count (device pointers). */
+typedef struct tc_router_net {
+  const float *params;      /* 513 float32 (layout of tc_router_n_params)        */
+  const float *norm;        /* 12 float32: mean[6], inv_std[6]                   */
+  const float *card;        /* float32 [n_nodes]: single-sketch estimate          */
+  float threshold;          /* HLL iff the BF16 output >= threshold              */
+  int sample_bits;          /* features read the first 2^sample_bits registers    */
+  int64_t min_cost_degree;  /* candidates need min(deg u, deg v) >= this         */
+} tc_router_net;
+
+/* Hybrid count routed by the network inside the exact pipeline's edge pass.
+ * Edge e is a candidate iff has_sketch[u] == has_sketch[v] == 1 and
+ * min(deg u, deg v) >= net->min_cost_degree (deg as in tc_hybrid_count);
+ * for each candidate the pass computes the six features of
+ * tc_router_features and the BF16 output of tc_mlp_route_bf16, and sends the
+ * edge to HLL iff the output is >= net->threshold (the same decisions those
+ * two entry points give).  Every other edge is counted exactly.  Accumulates
+ * counts and stats as tc_hybrid_count with part TC_PART_BOTH; n_candidates
+ * (device int64, may be NULL) += candidates; use (uint8 [n_edges], may be
+ * NULL) receives 1 for HLL-routed edges, else 0; per_edge as in
+ * tc_hybrid_count.  Replaces the per-edge predicate of src/gpu_hybrid.py:80-89
+ * (and src/hybrid.py:107-112) inside the counting pass. */
+int tc_router_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                           int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                           int64_t n_edges, const int32_t *degrees, const uint8_t *regs,
+                           const int32_t *slot, const uint8_t *has_sketch, int p,
+                           const tc_router_net *net, double *counts, int64_t *stats,
+                           int64_t *n_candidates, uint8_t *use, double *per_edge, void *stream);
+
 /* ---- batched counting of many graphs ------------------------------------- */
 /* No reference counterpart: the reference counts one graph per process
  * (experiment/exp_gpu.py:632-935).  Graphs are packed into one CSR over

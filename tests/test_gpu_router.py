@@ -146,3 +146,56 @@ def test_routed_count_is_exact_when_nothing_routes(device):
     assert st2["n_approx"] == st2["n_candidates"] > 0
     back = router.RouterModel.from_json(model.to_json())
     assert np.array_equal(back.params, model.params) and back.threshold == -1.0
+
+
+@pytest.mark.parametrize("threshold,gate,sample_bits", [(0.5, 0, 8), (0.3, 40, 8), (0.6, 0, 6),
+                                                        (0.5, 0, 10), (0.5, 10_000, 8)])
+def test_fused_routing_equals_separate_steps(device, threshold, gate, sample_bits):
+    # the network evaluated inside the exact pass takes the same decisions as
+    # features + BF16 inference as separate kernels, so the estimate, stats and
+    # per-edge supports are identical
+    import torch
+
+    p, theta = 10, 16
+    pairs, csr, sk = _graph(13, 6, p, theta)
+    cand = router.candidate_edges(csr, sk)
+    norm = router.feature_norm(router.features(csr, sk, cand, sample_bits))
+    model = router.RouterModel(router.init_params(5), norm, threshold, p, sample_bits, theta,
+                               min_cost_degree=gate)
+    pa = torch.zeros(csr.n_oriented, dtype=torch.float64, device="cuda")
+    pb = torch.zeros_like(pa)
+    ea, sa = router.routed_count(model, csr, sk, per_edge=pa, fused=True)
+    eb, sb = router.routed_count(model, csr, sk, per_edge=pb, fused=False)
+    assert torch.equal(sa["use"], sb["use"])
+    for k in ("n_approx", "n_exact", "n_clamped", "n_candidates"):
+        assert sa[k] == sb[k], k
+    assert ea == eb and torch.equal(pa, pb)
+    if gate == 10_000:
+        assert sa["n_candidates"] == 0 and sa["n_approx"] == 0
+    else:
+        assert 0 < sa["n_approx"] < sa["n_candidates"]
+    # exact-routed supports are the oracle's
+    rp, col, fdeg, nid, eu, ev, _ = O.csr_from_pairs(pairs)
+    use = sa["use"].cpu().numpy().astype(bool)
+    sup = O.edge_supports(rp, col, eu, ev)
+    assert np.array_equal(pa.cpu().numpy()[~use], sup[~use].astype(np.float64))
+
+
+def test_fused_router_rejects_bad_arguments(device):
+    import torch
+
+    p, theta = 10, 16
+    _, csr, sk = _graph(10, 1, p, theta)
+    model = router.RouterModel(router.init_params(0), np.array([0] * 6 + [1] * 6, np.float32),
+                               0.5, p, 12, theta)
+    with pytest.raises(ValueError, match="sample_bits"):
+        router.fused_count(model, csr, sk)
+    model.sample_bits = 8
+    model.min_cost_degree = -1
+    with pytest.raises(ValueError, match="min_cost_degree"):
+        router.fused_count(model, csr, sk)
+    model.min_cost_degree = 0
+    model.params = model.params[:100]
+    with pytest.raises(ValueError, match="513"):
+        router.fused_count(model, csr, sk)
+    torch.cuda.synchronize()

@@ -67,6 +67,11 @@ def test_router_full_size_matches_oracle(device, name, scale, seed):
     per = torch.zeros(csr.n_oriented, dtype=torch.float64, device="cuda")
     est, st = router.routed_count(model, csr, sk, per_edge=per)
     torch.cuda.synchronize()
+    # the network inside the exact pass decides as the separate feature and
+    # BF16 inference kernels do
+    est2, st2 = router.routed_count(model, csr, sk, fused=False)
+    assert torch.equal(st["use"], st2["use"]) and est2 == est
+    assert st2["n_candidates"] == st["n_candidates"]
     # the network decided: some candidates go to HLL, some stay exact
     assert st["n_candidates"] == n_cand > 0
     assert 0 < st["n_approx"] < n_cand

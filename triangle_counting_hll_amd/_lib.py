@@ -62,6 +62,8 @@ _SIGNATURES = {
     "tc_mlp_adam": (I32, [P, P, P, P, I32, I32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                           ctypes.c_float, P]),
     "tc_mlp_route_bf16": (I32, [P, P, P, I64, ctypes.c_float, P, P, P, P]),
+    "tc_router_hybrid_count": (I32, [P, I32, P, I64, P, P, I64, P, P, P, P, I32, P, P, P, P, P,
+                                     P, P]),
     "tc_vertex_graph_ids": (I32, [P, I64, P, I32, P, P]),
     "tc_batch_exact_count": (I32, [P, I32, P, I64, P, P, I64, P, I32, P, P, P]),
     "tc_batch_graph_create": (I32, [P, I32, P, I64, P, P, I64, P, I32, P, P, P, P, P]),
@@ -87,6 +89,14 @@ class ExactParams(ctypes.Structure):
                 ("waves_per_cu", ctypes.c_int), ("unroll", ctypes.c_int),
                 ("plan", ctypes.c_int), ("block_min_bits", ctypes.c_int),
                 ("pack_ids", ctypes.c_int), ("wide_offsets", ctypes.c_int)]
+
+
+class RouterNet(ctypes.Structure):
+    """``tc_router_net`` of include/tchll.h (device pointers)."""
+
+    _fields_ = [("params", ctypes.c_void_p), ("norm", ctypes.c_void_p),
+                ("card", ctypes.c_void_p), ("threshold", ctypes.c_float),
+                ("sample_bits", ctypes.c_int), ("min_cost_degree", ctypes.c_int64)]
 
 
 PLANS = {"sorted": 0, "atomic": 1}

@@ -28,6 +28,8 @@
 
 #include "tc_common.hpp"
 #include "tc_exact.hpp"
+#include "tc_hll.hpp"
+#include "tc_router.hpp"
 
 namespace tc {
 
@@ -862,7 +864,51 @@ static int launch_row_stats(const void *row_ptr, int row_ptr_64, int64_t n, bool
 // a radix sort on the center bits whose first pass drops the sentinels groups
 // the others of each center (in edge order).  No per-edge global atomics.
 // ---------------------------------------------------------------------------
-template <class I, bool kRoute>
+// Learned routing of one wavefront round (kRoute == 2): the round's
+// candidate edges are compacted into batch slots (LDS list of their lanes),
+// their sampled register statistics read cooperatively (batch_stats), and
+// each slot's features and BF16 network output computed in the slot's lane;
+// the decision goes back to the candidate's lane.  Every lane of the
+// wavefront calls this.
+template <int G>
+__device__ __forceinline__ bool net_route_round(const RouterNet &net, const float *sp,
+                                                const float *snorm, int *s_src, bool cand,
+                                                int32_t u, int32_t v, int Du, int Dv) {
+  const uint64_t cm = __ballot(cand);
+  if (!cm) return false;
+  const int lane = lane_id();
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int my_slot = __popcll(cm & lt);
+  const int n_c = __popcll(cm);
+  if (cand) s_src[my_slot] = lane;
+  wave_lds_sync();
+  const int src = lane < n_c ? s_src[lane] : 0;
+  wave_lds_sync();  // the list is rewritten by the next round
+  const int32_t su = __shfl(u, src), sv = __shfl(v, src);
+  const int sDu = __shfl(Du, src), sDv = __shfl(Dv, src);
+  int32_t a = 0, b = 0;
+  if (lane < n_c) {
+    a = net.slot ? net.slot[su] : su;
+    b = net.slot ? net.slot[sv] : sv;
+  }
+  const int ms = 1 << net.sample_bits;
+  const RegStats st = batch_stats<G, false>(net.regs, 1 << net.p, ms, a, b, n_c);
+  int dec = 0;
+  if (lane < n_c) {
+    const double us = hll_estimate(ms, st.zeros[2], st.z[2]) *
+                      (double)(1 << (net.p - net.sample_bits));
+    float f[kIn];
+    router_features(sDu, sDv, net.card[su], net.card[sv], us, f);
+    dec = mlp_prob_bf16(sp, snorm, f) >= net.threshold ? 1 : 0;
+  }
+  const int back = __shfl(dec, my_slot);
+  return cand && back != 0;
+}
+
+// kRoute: 0 = plain exact pass, 1 = hybrid routing by the degree rule or a
+// route array, 2 = hybrid routing by the learned network (G: lanes per edge of
+// its sampled-register read, group_lanes(net.sample_bits)).
+template <class I, int kRoute, int G>
 __global__ void __launch_bounds__(kBlock)
 k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict__ eu,
                 const int32_t *__restrict__ ev, int64_t E, const unsigned long long *rows_total,
@@ -871,72 +917,104 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
                 const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count,
                 unsigned long long *__restrict__ n_heavy, int64_t n, HybridRoute hr) {
   uint64_t acc = 0, nh = 0;
-  uint64_t n_use = 0, n_fb = 0;  // kRoute: HLL-routed edges, wanted HLL without sketches
+  // kRoute: HLL-routed edges, wanted HLL without sketches, network candidates
+  uint64_t n_use = 0, n_fb = 0, n_cand = 0;
   const int lane = lane_id();
   const bool rows_layout = L.rows();
   const int lbits = L.lbits();
   const uint64_t sentinel = (uint64_t)n << L.shift;
   const int64_t E_read = edge_limit(E, rows_total);  // edges past it get the sentinel
+  float *sp = nullptr, *snorm = nullptr;
+  int *s_src = nullptr;
+  if constexpr (kRoute == 2) {
+    __shared__ float s_params[kParams];
+    __shared__ float s_norm[2 * kIn];
+    __shared__ int s_list[kBlock / kWave][kWave];
+    load_params_bf16(hr.net.params, hr.net.norm, s_params, s_norm);
+    sp = s_params;
+    snorm = s_norm;
+    s_src = s_list[threadIdx.x / kWave];
+  }
   // the loop bound is uniform per wavefront, so every lane stays active for
-  // the cross-lane aggregation below
+  // the cross-lane work below
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < E;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t e = base + threadIdx.x;
     uint32_t tri = 0;
-    int32_t u = 0;
+    int32_t u = 0, v = 0;
+    int64_t ub = 0, ue = 0, vb = 0, ve = 0;
+    const bool active = e < E_read && !(skip && skip[e]);
+    if (active) {
+      v = ev[e];
+      u = eu[e];
+      ub = R.begin(u);
+      ue = R.end(u);
+      vb = R.begin(v);
+      ve = R.end(v);
+    }
+    const int64_t du = ue - ub, dv = ve - vb;
+    bool to_hll = false;
+    if constexpr (kRoute == 1) {
+      if (active) {
+        // reads only what decides: D(v) once D(u) reaches gamma, the sketch
+        // flags only for edges that want HLL
+        bool wants;
+        if (hr.route) {
+          wants = hr.route[e] != 0;
+        } else {
+          wants = (hr.degrees ? (int64_t)hr.degrees[u] : du) >= hr.gamma &&
+                  (hr.degrees ? (int64_t)hr.degrees[v] : dv) >= hr.gamma;
+        }
+        const bool both = wants && hr.has[u] == 1 && hr.has[v] == 1;
+        to_hll = wants && both;
+        n_use += to_hll ? 1 : 0;
+        n_fb += (wants && !both) ? 1 : 0;
+      }
+    }
+    if constexpr (kRoute == 2) {
+      int Du = 0, Dv = 0;
+      bool cand = false;
+      if (active && hr.has[u] == 1 && hr.has[v] == 1) {
+        Du = hr.degrees ? hr.degrees[u] : (int)du;
+        Dv = hr.degrees ? hr.degrees[v] : (int)dv;
+        // a sketched vertex without a register row is never a candidate
+        cand = (int64_t)min(Du, Dv) >= hr.net.min_cost_degree &&
+               (!hr.net.slot || (hr.net.slot[u] >= 0 && hr.net.slot[v] >= 0));
+      }
+      to_hll = net_route_round<G>(hr.net, sp, snorm, s_src, cand, u, v, Du, Dv);
+      n_cand += cand ? 1 : 0;
+      n_use += to_hll ? 1 : 0;
+    }
+    if constexpr (kRoute != 0) {
+      if (e < E) hr.use[e] = to_hll ? 1 : 0;
+    }
     bool coop = false;           // light edge whose rows fit 32 lanes each
-    int64_t la0 = 0, lb0 = 0;    // its rows
     int la = 0, lb = 0;
     if (e < E) {
       uint64_t word = sentinel;
-      if (e < E_read && !(skip && skip[e])) {
-        const int32_t v = ev[e];
-        u = eu[e];
-        const int64_t ub = R.begin(u), ue = R.end(u), vb = R.begin(v), ve = R.end(v);
-        const int64_t du = ue - ub, dv = ve - vb;
-        bool to_hll = false;
-        if constexpr (kRoute) {
-          // reads only what decides: D(v) once D(u) reaches gamma, the sketch
-          // flags only for edges that want HLL
-          bool wants;
-          if (hr.route) {
-            wants = hr.route[e] != 0;
+      if (active && !to_hll && du > 0 && dv > 0) {
+        if (du + dv <= light_max) {
+          if (du <= kWave / 2 && dv <= kWave / 2) {
+            coop = true;
+            la = (int)du;
+            lb = (int)dv;
           } else {
-            wants = (hr.degrees ? (int64_t)hr.degrees[u] : du) >= hr.gamma &&
-                    (hr.degrees ? (int64_t)hr.degrees[v] : dv) >= hr.gamma;
+            tri = merge_count(col, ub, ue, vb, ve);
           }
-          const bool both = wants && hr.has[u] == 1 && hr.has[v] == 1;
-          to_hll = wants && both;
-          hr.use[e] = to_hll ? 1 : 0;
-          n_use += to_hll ? 1 : 0;
-          n_fb += (wants && !both) ? 1 : 0;
-        }
-        if (!to_hll && du > 0 && dv > 0) {
-          if (du + dv <= light_max) {
-            if (du <= kWave / 2 && dv <= kWave / 2) {
-              coop = true;
-              la0 = ub;
-              lb0 = vb;
-              la = (int)du;
-              lb = (int)dv;
-            } else {
-              tri = merge_count(col, ub, ue, vb, ve);
-            }
-          } else {
-            const bool cv = dv >= du;
-            const uint32_t c = (uint32_t)(cv ? v : u), o = (uint32_t)(cv ? u : v);
-            word = (uint64_t)c << L.shift;
-            if (rows_layout)
-              word |= ((uint64_t)(cv ? ub : vb) << lbits) | (uint64_t)(cv ? du : dv);
-            else
-              word |= o;
-            ++nh;
-          }
+        } else {
+          const bool cv = dv >= du;
+          const uint32_t c = (uint32_t)(cv ? v : u), o = (uint32_t)(cv ? u : v);
+          word = (uint64_t)c << L.shift;
+          if (rows_layout)
+            word |= ((uint64_t)(cv ? ub : vb) << lbits) | (uint64_t)(cv ? du : dv);
+          else
+            word |= o;
+          ++nh;
         }
       }
       packed[e] = word;
     }
-    tri += light_intersections(col, coop, la0, la, lb0, lb);
+    tri += light_intersections(col, coop, ub, la, vb, lb);
     if (graph_of) {
       // one atomic per wavefront when all its triangles belong to one graph
       const int g = tri ? graph_of[u] : -1;
@@ -956,33 +1034,32 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
   }
   // one atomic per workgroup and counter: same-address atomics from every
   // wavefront of a large grid serialise
-  __shared__ unsigned long long s_red[4][kBlock / kWave];
+  __shared__ unsigned long long s_red[5][kBlock / kWave];
   nh = wave_sum(nh);
   acc = wave_sum(acc);
-  if constexpr (kRoute) {
+  if constexpr (kRoute != 0) {
     n_use = wave_sum(n_use);
     n_fb = wave_sum(n_fb);
+    n_cand = wave_sum(n_cand);
   }
   if (lane == 0) {
     s_red[0][threadIdx.x / kWave] = nh;
     s_red[1][threadIdx.x / kWave] = acc;
     s_red[2][threadIdx.x / kWave] = n_use;
     s_red[3][threadIdx.x / kWave] = n_fb;
+    s_red[4][threadIdx.x / kWave] = n_cand;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long a = 0, b = 0, c = 0, d = 0;
-    for (int w = 0; w < kBlock / kWave; ++w) {
-      a += s_red[0][w];
-      b += s_red[1][w];
-      c += s_red[2][w];
-      d += s_red[3][w];
-    }
-    if (a) atomicAdd(n_heavy, a);
-    if (!graph_of && b) atomicAdd(count, b);
-    if constexpr (kRoute) {
-      if (c) atomicAdd(&hr.counters[0], c);
-      if (d) atomicAdd(&hr.counters[1], d);
+    unsigned long long t[5] = {0, 0, 0, 0, 0};
+    for (int w = 0; w < kBlock / kWave; ++w)
+      for (int q = 0; q < 5; ++q) t[q] += s_red[q][w];
+    if (t[0]) atomicAdd(n_heavy, t[0]);
+    if (!graph_of && t[1]) atomicAdd(count, t[1]);
+    if constexpr (kRoute != 0) {
+      if (t[2]) atomicAdd(&hr.counters[0], t[2]);
+      if (t[3]) atomicAdd(&hr.counters[1], t[3]);
+      if (kRoute == 2 && t[4]) atomicAdd(&hr.counters[3], t[4]);
     }
   }
 }
@@ -1087,14 +1164,21 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     Rows<I> R{(const I *)row_ptr};
     {
       StageTimer _timer(TC_STAGE_CLASSIFY, s);
-      if (hr)
-        hipLaunchKernelGGL((k_classify_pack<I, true>), dim3(ge), dim3(kBlock), 0, s, R, col, eu,
+      if (hr && hr->net.params) {
+        TC_DISPATCH_GROUP(hr->net.sample_bits, G, {
+          hipLaunchKernelGGL((k_classify_pack<I, 2, G>), dim3(ge), dim3(kBlock), 0, s, R, col,
+                             eu, ev, E, rows_total, skip, light_max, layout,
+                             packed.as<uint64_t>(), graph_of, count, n_heavy, n, *hr);
+        });
+      } else if (hr) {
+        hipLaunchKernelGGL((k_classify_pack<I, 1, 1>), dim3(ge), dim3(kBlock), 0, s, R, col, eu,
                            ev, E, rows_total, skip, light_max, layout, packed.as<uint64_t>(),
                            graph_of, count, n_heavy, n, *hr);
-      else
-        hipLaunchKernelGGL((k_classify_pack<I, false>), dim3(ge), dim3(kBlock), 0, s, R, col, eu,
+      } else {
+        hipLaunchKernelGGL((k_classify_pack<I, 0, 1>), dim3(ge), dim3(kBlock), 0, s, R, col, eu,
                            ev, E, rows_total, skip, light_max, layout, packed.as<uint64_t>(),
                            graph_of, count, n_heavy, n, HybridRoute{});
+      }
     }
     TC_HIP(hipGetLastError());
     // Small inputs are launch-bound: when classify found no heavy edge, the

@@ -12,6 +12,23 @@ namespace tc {
 // degrees array, or the row length when degrees is null) and both endpoints
 // are sketched (has == 1); otherwise use[e] = 0.  counters[0] += HLL edges,
 // counters[1] += edges that wanted HLL without both sketches.
+// With net.params != null the learned router decides instead: an edge is a
+// candidate iff both endpoints are sketched and min(D(u), D(v)) >=
+// net.min_cost_degree, and a candidate goes to HLL iff the network's BF16
+// output on its six features (tc_router.hpp) is >= net.threshold;
+// counters[3] += candidates.
+struct RouterNet {
+  const float *params;   // kParams float32 (device)
+  const float *norm;     // 12 float32 (device)
+  const float *card;     // float32 [n]: single-sketch estimate of each vertex
+  const uint8_t *regs;   // sketch rows of 2^p bytes, 16-byte aligned
+  const int32_t *slot;   // compact row of each vertex (null: row = vertex id)
+  int p;
+  int sample_bits;       // features read the first 2^sample_bits registers
+  float threshold;
+  int64_t min_cost_degree;
+};
+
 struct HybridRoute {
   const int32_t *degrees;
   const uint8_t *has;
@@ -19,6 +36,7 @@ struct HybridRoute {
   const uint8_t *route;
   uint8_t *use;
   unsigned long long *counters;
+  RouterNet net;
 };
 
 // count[0] (or count[graph_of[center]] when graph_of != nullptr) +=

@@ -23,6 +23,8 @@
 
 #include "tc_common.hpp"
 #include "tc_exact.hpp"
+#include "tc_hll.hpp"
+#include "tc_router.hpp"
 
 namespace tc {
 
@@ -119,149 +121,6 @@ k_hll_build(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restric
   }
 }
 
-__device__ __forceinline__ double hll_alpha(int m) {
-  if (m == 16) return 0.673;
-  if (m == 32) return 0.697;
-  if (m == 64) return 0.709;
-  return 0.7213 / (1.0 + 1.079 / (double)m);
-}
-
-// Estimate from the merged register statistics (src/hll.py:37-54):
-// Z = sum_j 2^-r_j (zero registers included), zeros = #{j : r_j = 0}.
-// TC_HLL_CPU_EXP (cpu_exp/triangle_hll_experiment.py:356-365): LinearCounting
-// or the 2^32 large-range correction of the raw estimate, each returned as is.
-// TC_HLL_PROTO32 (experiment1_gpu/gpu_experiment_fixed.py:236-249): LinearCounting
-// only.
-__device__ __forceinline__ double hll_estimate(int m, uint64_t zeros, double Z,
-                                              int mode = TC_HLL_GPU) {
-  const double md = (double)m;
-  double E = hll_alpha(m) * md * md / Z;
-  if (mode == TC_HLL_PROTO32) {
-    if (E <= 2.5 * md && zeros > 0) E = md * log(md / (double)zeros);
-    return E;
-  }
-  if (mode == TC_HLL_CPU_EXP) {
-    if (E <= 2.5 * md && zeros > 0) return md * log(md / (double)zeros);
-    const double two32 = 4294967296.0;
-    if (E > two32 / 30.0) return -two32 * log(1.0 - E / two32);
-    return E;
-  }
-  if (E <= 2.5 * md && zeros > 0) E = md * log(md / (double)zeros);
-  const double two64 = 18446744073709551616.0;
-  if (E > two64 / 30.0) E = -two64 * log(1.0 - E / two64);
-  return E;
-}
-
-// 2^-r as a double built from its exponent bits (r <= 61).
-__device__ __forceinline__ double pow2_neg(uint32_t r) {
-  return __hiloint2double((int)((1023u - r) << 20), 0);
-}
-
-// ---------------------------------------------------------------------------
-// Batched per-edge register statistics.
-//
-// A wavefront takes 64 edges at a time ("batch slots"), one slot per lane.  An
-// edge's two register rows are read by a group of G = min(64, m / 16) lanes,
-// lane s of the group reading the 16-byte chunks s, s + G, s + 2G, ... of both
-// rows, so a wave step covers 64 / G edges and small sketches (m < 1024) keep
-// every lane loading.  The group's sums are reduced with log2(G) butterfly
-// steps and handed to the lane that owns the slot; after G steps every lane
-// holds the statistics of its own edge and the 64 estimates (FP64 divide and
-// log) run side by side instead of on one lane per edge.  Each lane sums its
-// chunks in a fixed order and the butterfly is fixed, so results are
-// deterministic.
-// ---------------------------------------------------------------------------
-struct RegStats {
-  uint32_t zeros[3];  // [row a, row b, merged]; a and b only when kTriple
-  double z[3];
-};
-
-// Number of zero bytes in a 32-bit word: a byte's top bit in t is set iff the
-// byte is non-zero (adding 0x7F carries into bit 7 for any non-zero low 7 bits).
-__device__ __forceinline__ uint32_t zero_bytes(uint32_t w) {
-  const uint32_t t = ((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w;
-  return 4u - (uint32_t)__popc(t & 0x80808080u);
-}
-
-// Adds this lane's chunks of two register rows: Z terms per byte position in
-// four accumulators (independent FP64 add chains), combined in a fixed order.
-template <int G, bool kTriple>
-__device__ __forceinline__ void group_partials(const uint8_t *__restrict__ ra,
-                                               const uint8_t *__restrict__ rb, int m, int sub,
-                                               RegStats &st) {
-  constexpr int kQ = kTriple ? 3 : 1;  // statistics kept: [merged] or [a, b, merged]
-  double z4[kQ][4] = {};
-  uint32_t zeros[kQ] = {};
-#pragma unroll 2
-  for (int j = sub * 16; j < m; j += 16 * G) {
-    const uint4 a = *(const uint4 *)(ra + j);
-    const uint4 b = *(const uint4 *)(rb + j);
-    const uint32_t aw[4] = {a.x, a.y, a.z, a.w}, bw[4] = {b.x, b.y, b.z, b.w};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      zeros[kQ - 1] += zero_bytes(aw[q] | bw[q]);  // max is 0 iff both are 0
-      if (kTriple) {
-        zeros[0] += zero_bytes(aw[q]);
-        zeros[1] += zero_bytes(bw[q]);
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t x = (aw[q] >> (8 * k)) & 0xffu, y = (bw[q] >> (8 * k)) & 0xffu;
-        z4[kQ - 1][k] += pow2_neg(max(x, y));
-        if (kTriple) {
-          z4[0][k] += pow2_neg(x);
-          z4[1][k] += pow2_neg(y);
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < kQ; ++q) {
-    const int o = kTriple ? q : 2;
-    st.zeros[o] += zeros[q];
-    st.z[o] += (z4[q][0] + z4[q][1]) + (z4[q][2] + z4[q][3]);
-  }
-}
-
-// Statistics of the edge in this lane's batch slot over the first `count`
-// registers of its two rows (rows `stride` bytes apart).  row_a / row_b: this
-// lane's slot's register rows (any value for unused slots).  n_valid: slots in
-// use.
-template <int G, bool kTriple>
-__device__ __forceinline__ RegStats batch_stats(const uint8_t *__restrict__ regs, int stride,
-                                                int count, int32_t row_a, int32_t row_b,
-                                                int n_valid) {
-  constexpr int kEpw = kWave / G;  // edges per wave step
-  const int lane = lane_id();
-  const int sub = lane % G, grp = lane / G;
-  RegStats mine = {};
-  for (int t = 0; t < G; ++t) {
-    const int src = t * kEpw + grp;  // slot handled by this lane's group
-    const int32_t a = __shfl(row_a, src, kWave), b = __shfl(row_b, src, kWave);
-    RegStats st = {};
-    if (src < n_valid)
-      group_partials<G, kTriple>(regs + (int64_t)a * stride, regs + (int64_t)b * stride, count,
-                                 sub, st);
-#pragma unroll
-    for (int q = kTriple ? 0 : 2; q < 3; ++q) {
-#pragma unroll
-      for (int off = G / 2; off > 0; off >>= 1) {
-        st.zeros[q] += __shfl_xor(st.zeros[q], off, kWave);
-        st.z[q] += __shfl_xor(st.z[q], off, kWave);
-      }
-      // slot `lane` is handled at step lane / kEpw by group lane % kEpw
-      const int from = (lane % kEpw) * G;
-      const uint32_t zz = __shfl(st.zeros[q], from, kWave);
-      const double zv = __shfl(st.z[q], from, kWave);
-      if (lane / kEpw == t) {
-        mine.zeros[q] = zz;
-        mine.z[q] = zv;
-      }
-    }
-  }
-  return mine;
-}
-
 template <int G>
 __global__ void __launch_bounds__(kBlock)
 k_union_estimate(const uint8_t *__restrict__ regs, const int32_t *__restrict__ slot, int p,
@@ -321,15 +180,10 @@ k_router_features(const int32_t *__restrict__ degrees, const int32_t *__restrict
     const RegStats st = batch_stats<G, false>(regs, m, ms, a, b, n_valid);
     if (!valid) continue;
     const double us = hll_estimate(ms, st.zeros[2], st.z[2]) * scale;
-    const float f0 = log2f(1.0f + (float)degrees[u]);
-    const float f1 = log2f(1.0f + (float)degrees[v]);
-    float *f = feats + i * 6;
-    f[0] = f0;
-    f[1] = f1;
-    f[2] = f0 - f1;
-    f[3] = log2f(1.0f + card[u]);
-    f[4] = log2f(1.0f + card[v]);
-    f[5] = (float)log2(1.0 + us);
+    float f[kIn];
+    router_features(degrees[u], degrees[v], card[u], card[v], us, f);
+#pragma unroll
+    for (int q = 0; q < kIn; ++q) feats[i * kIn + q] = f[q];
   }
 }
 
@@ -539,19 +393,6 @@ k_hybrid_finish(const double *__restrict__ partial, const int64_t *__restrict__ 
   }
 }
 
-// Lanes per edge for m = 2^p registers: 16 bytes per lane per step, at most a wave.
-inline int group_lanes(int p) { return p >= 10 ? 64 : 1 << (p - 4); }
-
-#define TC_DISPATCH_GROUP(p, G, ...)                                 \
-  switch (group_lanes(p)) {                                          \
-    case 1: { constexpr int G = 1; __VA_ARGS__; } break;             \
-    case 2: { constexpr int G = 2; __VA_ARGS__; } break;             \
-    case 4: { constexpr int G = 4; __VA_ARGS__; } break;             \
-    case 8: { constexpr int G = 8; __VA_ARGS__; } break;             \
-    case 16: { constexpr int G = 16; __VA_ARGS__; } break;           \
-    case 32: { constexpr int G = 32; __VA_ARGS__; } break;           \
-    default: { constexpr int G = 64; __VA_ARGS__; } break;           \
-  }
 
 }  // namespace tc
 
@@ -651,15 +492,25 @@ extern "C" int tc_router_features(const int32_t *degrees, const int32_t *edge_u,
   return TC_OK;
 }
 
-extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
-                               int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
-                               int64_t n_edges, const int32_t *degrees, const uint8_t *regs,
-                               const int32_t *slot, const uint8_t *has_sketch, int64_t gamma, int p,
-                               const uint8_t *route, int part, double *counts, int64_t *stats,
-                               int64_t *fallback, double *per_edge, int hash_mode,
-                               void *stream) {
-  using namespace tc;
-  Frame _frame((hipStream_t)stream);
+namespace tc {
+
+// target[0] += source[0] (a device counter handed to the caller).
+__global__ void k_add_counter(const unsigned long long *__restrict__ source,
+                              int64_t *__restrict__ target) {
+  target[0] += (int64_t)source[0];
+}
+
+// tc_hybrid_count, and with net (net->params != null) the learned router's
+// count: routing by the network inside the exact pass (part must be BOTH).
+// use_out (uint8 [E], optional) receives the per-edge route taken; n_cand
+// (device int64, optional) += the network's candidate edges.
+static int hybrid_impl(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                       int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                       int64_t n_edges, const int32_t *degrees, const uint8_t *regs,
+                       const int32_t *slot, const uint8_t *has_sketch, int64_t gamma, int p,
+                       const uint8_t *route, int part, double *counts, int64_t *stats,
+                       int64_t *fallback, double *per_edge, int hash_mode, hipStream_t s,
+                       const RouterNet *net, uint8_t *use_out, int64_t *n_cand) {
   TC_CHECK(valid_hash_mode(hash_mode), TC_ERR_INVALID, "unknown hash mode %d", hash_mode);
   TC_CHECK(part == TC_PART_BOTH || part == TC_PART_EXACT || part == TC_PART_HLL, TC_ERR_INVALID,
            "unknown part %d", part);
@@ -677,19 +528,23 @@ extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_
   TC_CHECK(((uintptr_t)regs & 15) == 0, TC_ERR_INVALID, "regs must be 16-byte aligned");
   TC_CHECK(n_edges < INT32_MAX, TC_ERR_INVALID, "edge ids must fit int32");
   TC_TRY(ensure_init());
-  hipStream_t s = (hipStream_t)stream;
   const int64_t E = n_edges;
-  Scratch use, list, nlist, svals, partial, acc;
-  TC_TRY(use.alloc(std::max<int64_t>(E, 1), s));
+  Scratch use_buf, list, nlist, svals, partial, acc;
+  uint8_t *use = use_out;
+  if (!use) {
+    TC_TRY(use_buf.alloc(std::max<int64_t>(E, 1), s));
+    use = use_buf.as<uint8_t>();
+  }
   TC_TRY(list.alloc(sizeof(int32_t) * std::max<int64_t>(E, 1), s));
   TC_TRY(nlist.alloc(sizeof(int64_t), s));
   TC_TRY(svals.alloc(sizeof(double) * std::max<int64_t>(E, 1), s));
   const int64_t max_tiles = (E + kSumTile - 1) / kSumTile + 1;
   TC_TRY(partial.alloc(sizeof(double) * max_tiles, s));
-  TC_TRY(acc.alloc(sizeof(unsigned long long) * 4, s));
+  TC_TRY(acc.alloc(sizeof(unsigned long long) * 5, s));
   unsigned long long *exact_sum = acc.as<unsigned long long>();
-  unsigned long long *counters = exact_sum + 1;  // [n_hll, n_fallback, n_clamped]
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, exact_sum, 4, 0ull);
+  // [n_hll, n_fallback, n_clamped, n_candidates]
+  unsigned long long *counters = exact_sum + 1;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, exact_sum, 5, 0ull);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t *)nlist.ptr, 2, 0);
   TC_HIP(hipGetLastError());
   TC_DISPATCH_ROWS(row_ptr_64, I, {
@@ -698,22 +553,22 @@ extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_
     if (E > 0) {
       if (part == TC_PART_HLL) {
         hipLaunchKernelGGL(k_route<I>, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock), 0, s, D,
-                           edge_u, edge_v, E, has_sketch, gamma, route, use.as<uint8_t>(),
-                           counters);
+                           edge_u, edge_v, E, has_sketch, gamma, route, use, counters);
         TC_HIP(hipGetLastError());
       } else {
         // the exact pipeline's edge pass routes each edge (same rule as
         // k_route) and counts only the edges it keeps
-        const HybridRoute hr{degrees, has_sketch, gamma, route, use.as<uint8_t>(), counters};
+        HybridRoute hr{degrees, has_sketch, gamma, route, use, counters, RouterNet{}};
+        if (net) hr.net = *net;
         TC_TRY(exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, E,
                                 nullptr, nullptr, exact_sum, nullptr, s, &hr));
         if (per_edge)
           hipLaunchKernelGGL(k_exact_per_edge<I>, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock),
                              0, s, R, col_idx, edge_u, edge_v, E,
-                             (const uint8_t *)use.as<uint8_t>(), per_edge);
+                             (const uint8_t *)use, per_edge);
       }
       if (part != TC_PART_EXACT) {
-        TC_TRY(compact_indices(use.as<uint8_t>(), E, list.as<int32_t>(), nlist.as<int64_t>(), s));
+        TC_TRY(compact_indices(use, E, list.as<int32_t>(), nlist.as<int64_t>(), s));
         // the HLL edge count is on the device: size the grid for all E edges
         const unsigned gw = (unsigned)std::max<int64_t>(
             1, std::min<int64_t>((E + kBlock - 1) / kBlock, num_cus() * 16));
@@ -747,5 +602,49 @@ extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_
                      (const unsigned long long *)exact_sum, (const unsigned long long *)counters, E,
                      part, counts, stats, fallback);
   TC_HIP(hipGetLastError());
+  if (n_cand)
+    hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, s,
+                       (const unsigned long long *)(counters + 3), n_cand);
+  TC_HIP(hipGetLastError());
   return TC_OK;
+}
+
+}  // namespace tc
+
+extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                               int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                               int64_t n_edges, const int32_t *degrees, const uint8_t *regs,
+                               const int32_t *slot, const uint8_t *has_sketch, int64_t gamma, int p,
+                               const uint8_t *route, int part, double *counts, int64_t *stats,
+                               int64_t *fallback, double *per_edge, int hash_mode,
+                               void *stream) {
+  using namespace tc;
+  Frame _frame((hipStream_t)stream);
+  return hybrid_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges, degrees,
+                     regs, slot, has_sketch, gamma, p, route, part, counts, stats, fallback,
+                     per_edge, hash_mode, (hipStream_t)stream, nullptr, nullptr, nullptr);
+}
+
+extern "C" int tc_router_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                                      int64_t n_nodes, const int32_t *edge_u,
+                                      const int32_t *edge_v, int64_t n_edges,
+                                      const int32_t *degrees, const uint8_t *regs,
+                                      const int32_t *slot, const uint8_t *has_sketch, int p,
+                                      const tc_router_net *net, double *counts, int64_t *stats,
+                                      int64_t *n_candidates, uint8_t *use, double *per_edge,
+                                      void *stream) {
+  using namespace tc;
+  Frame _frame((hipStream_t)stream);
+  TC_CHECK(net && net->params && net->norm && net->card, TC_ERR_INVALID,
+           "the router network needs params, norm and card");
+  TC_CHECK(p >= 4 && p <= 16 && net->sample_bits >= 4 && net->sample_bits <= p, TC_ERR_INVALID,
+           "need 4 <= sample_bits <= p <= 16 (got p=%d, sample_bits=%d)", p, net->sample_bits);
+  TC_CHECK(net->min_cost_degree >= 0, TC_ERR_INVALID, "min_cost_degree must be >= 0");
+  TC_CHECK(n_edges == 0 || (row_ptr && col_idx && edge_u && edge_v && regs), TC_ERR_INVALID,
+           "row_ptr, col_idx, edge_u, edge_v and regs are required");
+  const RouterNet rn{net->params, net->norm, net->card, regs, slot, p, net->sample_bits,
+                     net->threshold, net->min_cost_degree};
+  return hybrid_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges, degrees,
+                     regs, slot, has_sketch, 0, p, nullptr, TC_PART_BOTH, counts, stats, nullptr,
+                     per_edge, TC_HLL_GPU, (hipStream_t)stream, &rn, use, n_candidates);
 }

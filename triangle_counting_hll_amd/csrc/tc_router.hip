@@ -2,24 +2,14 @@
 // FP32 training (forward + backward with deterministic reductions, Adam) and
 // BF16 inference.  No MFMA: the network is 513 parameters and every sample is
 // independent, so each lane evaluates one sample from parameters held in LDS.
-//
-// Parameter vector (float32, kParams entries):
-//   W1[j][i] at j*6 + i (j < 64, i < 6), b1[j] at 384 + j, w2[j] at 448 + j,
-//   b2 at 512.
-// Normalisation vector (float32, 12 entries): mean[6] then inv_std[6]; the
-// network sees x = (feature - mean) * inv_std.
+// Parameter layout and the shared inference code: tc_router.hpp.
 #include <algorithm>
 
 #include "tc_common.hpp"
+#include "tc_router.hpp"
 
 namespace tc {
 
-constexpr int kIn = 6;
-constexpr int kHid = 64;
-constexpr int kOffB1 = kHid * kIn;    // 384
-constexpr int kOffW2 = kOffB1 + kHid;  // 448
-constexpr int kOffB2 = kOffW2 + kHid;  // 512
-constexpr int kParams = kOffB2 + 1;    // 513
 constexpr int kSlots = (kParams + kWave - 1) / kWave;  // per-lane gradient slots (9)
 
 __device__ __forceinline__ float softplus_f(float x) {
@@ -170,13 +160,6 @@ __global__ void k_adam(float *__restrict__ params, const float *__restrict__ gra
   params[k] -= lr * (mk / corr1) / (sqrtf(vk / corr2) + eps);
 }
 
-// float32 -> bfloat16, round to nearest even (NaN not expected)
-__device__ __forceinline__ float round_bf16(float f) {
-  uint32_t b = __float_as_uint(f);
-  b += 0x7FFFu + ((b >> 16) & 1u);
-  return __uint_as_float(b & 0xFFFF0000u);
-}
-
 // BF16 inference: weights, inputs and hidden activations rounded to bf16,
 // products accumulated in FP32.  route[cand[i]] = prob >= threshold (or
 // route[i] when cand is NULL).
@@ -187,26 +170,13 @@ k_mlp_route_bf16(const float *__restrict__ params, const float *__restrict__ nor
                  float *__restrict__ prob) {
   __shared__ float sp[kParams];
   __shared__ float snorm[2 * kIn];
-  load_params(params, norm, sp, snorm);
-  for (int k = threadIdx.x; k < kHid * kIn; k += blockDim.x) sp[k] = round_bf16(sp[k]);
-  for (int k = threadIdx.x; k < kHid; k += blockDim.x) sp[kOffW2 + k] = round_bf16(sp[kOffW2 + k]);
-  __syncthreads();
+  load_params_bf16(params, norm, sp, snorm);
   for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n;
        s += (int64_t)gridDim.x * blockDim.x) {
-    float x[kIn];
-    normalise(feats, s, snorm, x);
+    float f[kIn];
 #pragma unroll
-    for (int i = 0; i < kIn; ++i) x[i] = round_bf16(x[i]);
-    float z = 0.f;
-    for (int j = 0; j < kHid; ++j) {
-      float a = 0.f;
-#pragma unroll
-      for (int i = 0; i < kIn; ++i) a = fmaf(sp[j * kIn + i], x[i], a);
-      a += sp[kOffB1 + j];
-      z = fmaf(sp[kOffW2 + j], round_bf16(fmaxf(a, 0.f)), z);
-    }
-    z += sp[kOffB2];
-    const float y = 1.f / (1.f + expf(-z));
+    for (int i = 0; i < kIn; ++i) f[i] = feats[s * kIn + i];
+    const float y = mlp_prob_bf16(sp, snorm, f);
     if (prob) prob[s] = y;
     if (route) route[cand ? (int64_t)cand[s] : s] = (y >= threshold) ? 1 : 0;
   }

@@ -29,6 +29,7 @@ exact path, which stays the fallback and the judge of the router's error.
 
 from __future__ import annotations
 
+import ctypes
 import json
 import math
 from dataclasses import dataclass, field
@@ -268,13 +269,23 @@ def train(feats, target, epochs: int = 12, batch: int = 8192, lr: float = 1e-2,
     return params.cpu().numpy(), np.asarray(norm, np.float32), history
 
 
-def choose_threshold(prob, err, truth_total: float, budget: float) -> float:
+def choose_threshold(prob, err, truth_total: float, budget: float,
+                     rule: str = "signed") -> float:
     """Lowest threshold whose HLL-routed prefix (by descending probability)
-    keeps every partial |sum of signed errors| within budget * truth_total."""
+    keeps the error of the routed edges within budget * truth_total.
+
+    ``rule="signed"``: every partial |sum of signed errors| stays within the
+    budget (errors of both signs cancel).  ``rule="absolute"``: the sum of the
+    absolute per-edge errors stays within it, a bound on the total error that
+    does not rely on cancellation, so it carries over to other graphs with a
+    similar error distribution (the calibration graph is not the graph
+    counted)."""
     t = _lib.torch()
+    if rule not in ("signed", "absolute"):
+        raise ValueError(f"rule must be 'signed' or 'absolute', got {rule!r}")
     order = t.argsort(prob, descending=True)
     e = err[order].double()
-    c = t.cumsum(e, 0).abs()
+    c = t.cumsum(e.abs() if rule == "absolute" else e, 0).abs()
     bad = (c > budget * max(truth_total, 1.0)).nonzero()
     k = int(bad[0].item()) if bad.numel() else int(c.numel())
     if k == 0:
@@ -297,9 +308,15 @@ def route(model: RouterModel, csr, sk: Sketches, stream=None):
 
 @_lib.on_stream
 def routed_count(model: RouterModel, csr, sk: Optional[Sketches] = None, per_edge=None,
-                 stream=None):
+                 fused: bool = True, stream=None):
     """Hybrid count with the learned router; returns (estimate, stats dict).
 
+    ``fused`` (default): the network is evaluated per candidate edge inside
+    the exact pipeline's edge pass (``tc_router_hybrid_count``), where the
+    reference evaluates its degree predicate (src/gpu_hybrid.py:80-89).
+    ``fused=False`` runs the same decisions as separate steps (candidate
+    list, ``tc_router_features``, ``tc_mlp_route_bf16``, then the hybrid count
+    with the route array).
     ``per_edge`` (float64 device [E], optional) receives every edge's support.
     ``stats["use"]`` is the per-edge route taken (uint8 device, 1 = HLL).
     """
@@ -317,11 +334,51 @@ def routed_count(model: RouterModel, csr, sk: Optional[Sketches] = None, per_edg
                 "n_approx": 0, "n_exact": csr.n_oriented, "n_clamped": 0, "n_candidates": 0,
                 "n_sketched": 0, "use": t.zeros(csr.n_oriented, dtype=t.uint8, device="cuda")}
         sk = build_sketches(csr, need, model.p)
-    r, n_cand = route(model, csr, sk, stream)
-    c, s, fb = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
-                                       csr.degrees, sk.regs, sk.has, 0, model.p, route=r,
-                                       slot=sk.slot, per_edge=per_edge, stream=stream)
+    if not fused:
+        r, n_cand = route(model, csr, sk, stream)
+        c, s, fb = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
+                                           csr.degrees, sk.regs, sk.has, 0, model.p, route=r,
+                                           slot=sk.slot, per_edge=per_edge, stream=stream)
+        st = s.cpu().numpy()
+        return float(c.item()), {"n_approx": int(st[0]), "n_exact": int(st[1]),
+                                 "n_clamped": int(st[2]), "n_candidates": n_cand,
+                                 "n_sketched": sk.n_sketched, "use": r}
+    c, s, n_cand, use = fused_count(model, csr, sk, per_edge=per_edge, stream=stream)
     st = s.cpu().numpy()
     return float(c.item()), {"n_approx": int(st[0]), "n_exact": int(st[1]),
-                             "n_clamped": int(st[2]), "n_candidates": n_cand,
-                             "n_sketched": sk.n_sketched, "use": r}
+                             "n_clamped": int(st[2]), "n_candidates": int(n_cand.item()),
+                             "n_sketched": sk.n_sketched, "use": use}
+
+
+@_lib.on_stream
+def fused_count(model: RouterModel, csr, sk: Sketches, counts=None, stats=None, n_cand=None,
+                use=None, per_edge=None, stream=None):
+    """Enqueue ``tc_router_hybrid_count``: the network routes each candidate
+    inside the exact pipeline's edge pass.  Returns the device accumulators
+    ``(counts float64 [1], stats int64 [3], n_candidates int64 [1], use uint8 [E])``."""
+    t = _lib.require_device()
+    rp, is64 = _lib.row_ptr_arg(csr.row_ptr)
+    dev = rp.device
+    params = _lib.as_device(model.params, t.float32)
+    norm = _lib.as_device(model.norm, t.float32)
+    card = _lib.as_device(sk.card, t.float32)
+    if params.numel() != N_PARAMS or norm.numel() != 2 * N_IN:
+        raise ValueError("router parameters must be 513 floats and norm 12 floats")
+    if counts is None:
+        counts = t.zeros(1, dtype=t.float64, device=dev)
+    if stats is None:
+        stats = t.zeros(3, dtype=t.int64, device=dev)
+    if n_cand is None:
+        n_cand = t.zeros(1, dtype=t.int64, device=dev)
+    if use is None:
+        use = t.empty(max(csr.n_oriented, 1), dtype=t.uint8, device=dev)
+    net = _lib.RouterNet(_lib.ptr(params), _lib.ptr(norm), _lib.ptr(card),
+                         float(model.threshold), int(model.sample_bits),
+                         int(model.min_cost_degree))
+    _lib.check(_lib.load().tc_router_hybrid_count(
+        _lib.ptr(rp), is64, _lib.ptr(csr.col_idx), csr.n_nodes, _lib.ptr(csr.edge_u),
+        _lib.ptr(csr.edge_v), csr.n_oriented, _lib.ptr(csr.degrees), _lib.ptr(sk.regs),
+        _lib.ptr(sk.slot), _lib.ptr(sk.has), int(sk.p), ctypes.byref(net), _lib.ptr(counts),
+        _lib.ptr(stats), _lib.ptr(n_cand), _lib.ptr(use), _lib.ptr(per_edge),
+        _lib.stream_ptr(stream)))
+    return counts, stats, n_cand, use[: csr.n_oriented]
