@@ -175,6 +175,20 @@ int tc_exact_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
                    int64_t n_edges, uint64_t *count, const tc_exact_params *params,
                    void *stream);
 
+/* tc_exact_count without any allocation: no reference counterpart; it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * result and stream behaviour as tc_exact_count (sorted plan only), but the
+ * pipeline's scratch is carved from one 128 MiB array the library holds on
+ * each device it runs on until the process ends; a graph whose scratch would
+ * not fit (about 5M edges at 4 edges per vertex) is rejected with
+ * TC_ERR_INVALID before any device work.  Calls that overlap on one device
+ * (other streams, other host threads) share that array and race. */
+int tc_exact_count_noalloc(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                           int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                           int64_t n_edges, uint64_t *count, const tc_exact_params *params,
+                           void *stream);
+
 /* Per-edge supports: out[e] = |N+(edge_u[e]) ∩ N+(edge_v[e])| (int32).
  * Replaces the per-edge records of src/exact.py:87-91. */
 int tc_edge_supports(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,

@@ -48,6 +48,7 @@ _SIGNATURES = {
     "tc_format_edge_list": (I32, [P, I64, ctypes.c_char_p, ctypes.c_char_p, P, I64, P, P]),
     "tc_degree_buckets": (I32, [P, I32, I64, P, P, P]),
     "tc_exact_count": (I32, [P, I32, P, I64, P, P, I64, P, P, P]),
+    "tc_exact_count_noalloc": (I32, [P, I32, P, I64, P, P, I64, P, P, P]),
     "tc_edge_supports": (I32, [P, I32, P, P, P, I64, P, P]),
     "tc_hash64": (I32, [P, P, I64, P]),
     "tc_compact_flags": (I32, [P, I64, P, P, P]),

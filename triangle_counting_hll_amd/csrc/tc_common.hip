@@ -232,6 +232,16 @@ k_tile_scan(const T *__restrict__ in, int64_t n, const int64_t *__restrict__ off
 
 __global__ void k_zero_i64(int64_t *p) { *p = 0; }
 
+// Scratch bytes one exclusive_scan of n elements carves (the allocation
+// sequence below, every piece rounded as the arena rounds it).
+size_t scan_scratch_bytes(int64_t n) {
+  if (n <= 0) return 0;
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles == 1) return 0;
+  return arena_round(sizeof(int64_t) * tiles) + arena_round(sizeof(int64_t) * (tiles + 1)) +
+         scan_scratch_bytes(tiles);
+}
+
 template <class T>
 int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t * /*unused*/, hipStream_t s) {
   if (n <= 0) {
@@ -307,6 +317,12 @@ __global__ void k_bucket_starts(const int64_t *__restrict__ offs, int64_t ntiles
                                 int64_t *__restrict__ starts) {
   const int k = threadIdx.x;
   if (k <= nb) starts[k] = offs[(int64_t)k * ntiles];
+}
+
+size_t bucket_sort_scratch_bytes(int64_t n, int nb) {
+  const int64_t ntiles = std::max<int64_t>((n + kWave - 1) / kWave, 1);
+  return arena_round(sizeof(int32_t) * ntiles * nb) +
+         arena_round(sizeof(int64_t) * (ntiles * nb + 1)) + scan_scratch_bytes(ntiles * nb);
 }
 
 int bucket_sort_u8(const uint8_t *key, int64_t n, int nb, int32_t *perm, int64_t *starts,
@@ -397,6 +413,12 @@ k_compact_write(const uint8_t *__restrict__ flag, int64_t n, int64_t nchunks,
 #pragma unroll
   for (int k = 0; k < kCompactPerLane; ++k)
     if ((w[k >> 2] >> (8 * (k & 3))) & 0xFFu) idx[o++] = (int32_t)(first + k);
+}
+
+size_t compact_scratch_bytes(int64_t n) {
+  const int64_t nchunks = (n + kCompactChunk - 1) / kCompactChunk;
+  return arena_round(sizeof(int32_t) * std::max<int64_t>(nchunks, 1)) +
+         arena_round(sizeof(int64_t) * (nchunks + 1)) + scan_scratch_bytes(nchunks);
 }
 
 int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d_count,

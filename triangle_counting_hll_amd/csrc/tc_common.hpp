@@ -244,4 +244,11 @@ int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
 int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d_count,
                     hipStream_t s);
 
+// Scratch bytes each helper above carves from an arena for a problem of the
+// given size (the sum of its Scratch pieces, rounded as arena_round does).
+size_t scan_scratch_bytes(int64_t n);
+size_t bucket_sort_scratch_bytes(int64_t n, int nbuckets);
+size_t radix_sort_scratch_bytes(int64_t n, int bit_lo, int bit_hi);
+size_t compact_scratch_bytes(int64_t n);
+
 }  // namespace tc

@@ -25,6 +25,7 @@
 // Streaming the shorter row into a hash of the longer one costs
 // sum_e min(d+u, d+v) probes, against sum_e (d+u + d+v) for a merge.
 #include <algorithm>
+#include <mutex>
 
 #include "tc_common.hpp"
 #include "tc_exact.hpp"
@@ -1109,6 +1110,43 @@ __global__ void k_run_item_counts(const int64_t *__restrict__ run_ptr,
   }
 }
 
+// Others per work item: at most 65536 keeps each lane's 32-bit hit count exact.
+static int effective_chunk(const tc_exact_params *params) {
+  return (params && params->chunk > 0) ? std::min(params->chunk, 65536) : 512;
+}
+
+// Scratch bytes exact_sorted_plan carves for n vertices and E edges: its own
+// pieces in allocation order, then those of the radix sort, the run
+// compaction, the bucket sort and the item scan (each rounded by arena_round).
+static size_t exact_sorted_scratch_bytes(int64_t n, int64_t E, int chunk, bool csr_mode) {
+  int key_bits = 1;
+  while ((int64_t(1) << key_bits) <= n) ++key_bits;
+  const int64_t max_items = n + E / chunk + 1;
+  const int64_t pieces[] = {
+      csr_mode ? (int64_t)sizeof(int32_t) * E : 0,          // eu_buf
+      (int64_t)sizeof(uint64_t) * E,                         // packed
+      (int64_t)sizeof(uint64_t) * E,                         // tmp
+      E,                                                     // flags
+      (int64_t)sizeof(int32_t) * n,                          // run_idx
+      (int64_t)sizeof(int64_t),                              // n_runs
+      (int64_t)sizeof(int64_t) * (n + 1),                    // run_ptr
+      n,                                                     // rkey
+      (int64_t)sizeof(int32_t) * n,                          // perm
+      (int64_t)sizeof(int64_t) * (kCenterBuckets + 1),       // starts
+      (int64_t)sizeof(int32_t) * n,                          // nitems
+      (int64_t)sizeof(int64_t) * (n + 1),                    // iptr
+      (int64_t)sizeof(int2) * max_items,                     // items
+      (int64_t)(7 * sizeof(unsigned long long)),             // ctr
+  };
+  size_t b = 0;
+  for (const int64_t x : pieces) b += x > 0 ? arena_round((size_t)x) : 0;
+  b += radix_sort_scratch_bytes(E, 64 - key_bits, 64);
+  b += compact_scratch_bytes(E);
+  b += bucket_sort_scratch_bytes(n, kCenterBuckets);
+  b += scan_scratch_bytes(n);
+  return b;
+}
+
 static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t *col, int64_t n,
                              const int32_t *eu, const int32_t *ev, int64_t E,
                              const uint8_t *skip, const int32_t *graph_of,
@@ -1258,8 +1296,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                      const int32_t *graph_of, unsigned long long *count,
                      const tc_exact_params *params, hipStream_t s, const HybridRoute *hr) {
   const int light_max = (params && params->light_max > 0) ? params->light_max : 32;
-  // at most 65536 others per work item keeps each lane's 32-bit hit count exact
-  const int chunk = (params && params->chunk > 0) ? std::min(params->chunk, 65536) : 512;
+  const int chunk = effective_chunk(params);
   const int wpc = (params && params->waves_per_cu > 0) ? params->waves_per_cu : 16;
   const int unroll = (params && params->unroll > 0) ? params->unroll : kDefaultUnroll;
   TC_CHECK(unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8 || unroll == 16,
@@ -1389,6 +1426,78 @@ extern "C" int tc_exact_count(const void *row_ptr, int row_ptr_64, const int32_t
   return exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges,
                           nullptr, nullptr, (unsigned long long *)count, params,
                           (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// The exact count without allocation: the pipeline's scratch is carved from
+// one file-scope device array (per device, held until the process ends)
+// instead of the stream workspace or the memory pool.
+// ---------------------------------------------------------------------------
+namespace tc {
+
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;  // 128 MiB
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static char *g_scratch_ptr[64] = {};
+
+// Device address of this device's copy of `scratch`, looked up once per
+// device (the array's name is not a device address in host code).
+static int scratch_base(char **out) {
+  int dev = 0;
+  TC_HIP(hipGetDevice(&dev));
+  TC_CHECK(dev >= 0 && dev < 64, TC_ERR_UNSUPPORTED, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    TC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TC_CHECK(p != nullptr, TC_ERR_HIP, "no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<char *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return TC_OK;
+}
+
+}  // namespace tc
+
+extern "C" int tc_exact_count_noalloc(const void *row_ptr, int row_ptr_64,
+                                      const int32_t *col_idx, int64_t n_nodes,
+                                      const int32_t *edge_u, const int32_t *edge_v,
+                                      int64_t n_edges, uint64_t *count,
+                                      const tc_exact_params *params, void *stream) {
+  using namespace tc;
+  TC_CHECK(row_ptr && count, TC_ERR_INVALID, "row_ptr and count are required");
+  TC_CHECK(n_nodes >= 0 && n_edges >= 0, TC_ERR_INVALID, "negative size");
+  TC_CHECK((edge_u == nullptr) == (edge_v == nullptr), TC_ERR_INVALID,
+           "edge_u and edge_v must both be given or both be NULL");
+  TC_CHECK(n_edges == 0 || col_idx, TC_ERR_INVALID, "col_idx is required");
+  TC_CHECK(n_nodes < INT32_MAX, TC_ERR_INVALID, "node ids must fit int32");
+  TC_CHECK(n_edges < INT32_MAX, TC_ERR_INVALID, "edge positions must fit int32");
+  TC_CHECK(!params || params->plan == TC_PLAN_SORTED, TC_ERR_INVALID,
+           "tc_exact_count_noalloc runs the sorted plan only");
+  TC_TRY(ensure_init());
+  if (n_edges == 0 || n_nodes == 0) return TC_OK;
+  const size_t need =
+      exact_sorted_scratch_bytes(n_nodes, n_edges, effective_chunk(params), edge_u == nullptr);
+  TC_CHECK(need <= kNoAllocScratchBytes, TC_ERR_INVALID,
+           "%lld nodes and %lld edges need %zu bytes of scratch; tc_exact_count_noalloc holds "
+           "%zu (use tc_exact_count for this graph)",
+           (long long)n_nodes, (long long)n_edges, need, kNoAllocScratchBytes);
+  char *base = nullptr;
+  TC_TRY(scratch_base(&base));
+  Arena ar;
+  ar.carve = true;
+  ar.base = base;
+  ar.cap = kNoAllocScratchBytes;
+  Arena *prev = current_arena();
+  current_arena() = &ar;
+  const int rc = exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges,
+                                  nullptr, nullptr, (unsigned long long *)count, params,
+                                  (hipStream_t)stream);
+  current_arena() = prev;
+  TC_CHECK(rc != TC_OK || ar.total <= need, TC_ERR_HIP,
+           "scratch use %zu exceeded its bound %zu", ar.total, need);
+  return rc;
 }
 
 extern "C" int tc_edge_supports(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,

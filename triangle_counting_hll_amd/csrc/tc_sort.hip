@@ -155,6 +155,15 @@ k_radix_scatter(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int
   }
 }
 
+size_t radix_sort_scratch_bytes(int64_t n, int bit_lo, int bit_hi) {
+  if (n <= 1 || bit_hi <= bit_lo) return 0;
+  const int64_t ntiles = (n + kRadixTile - 1) / kRadixTile;
+  const int passes = (bit_hi - bit_lo + kRadixBits - 1) / kRadixBits;
+  return arena_round(sizeof(int32_t) * ntiles * kRadixDigits) +
+         arena_round(sizeof(int64_t) * (ntiles * kRadixDigits + 1)) +
+         (size_t)passes * scan_scratch_bytes(ntiles * kRadixDigits);
+}
+
 int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
                    uint64_t **result, hipStream_t s, const uint64_t *drop,
                    const unsigned long long *n_dev) {
