@@ -333,6 +333,45 @@ __device__ __forceinline__ void table_build(int *__restrict__ tbl, const int32_t
   if constexpr (kBlockWide) __syncthreads(); else wave_lds_sync();
 }
 
+// How a work item's streamed ids are tested against its center row.
+constexpr int kProbeGlobal = 0;  // binary search of the row in global memory
+constexpr int kProbeHash = 1;    // LDS hash set (table_build)
+constexpr int kProbeBitmap = 2;  // LDS bitmap over the row's id range (bitmap_build)
+
+// Bitmap of one center row over its id range: bit (x - lo) is set for every
+// id x of the row (lo = the row's first, smallest id).  Bit `sentinel` (the
+// one after the row's largest id) stays clear: a probe clamps every offset
+// to it, so ids outside the range read a zero bit with no branch.  Only the
+// words up to the sentinel are cleared.  Needs sentinel < kInts * 32.
+template <int kInts, bool kBlockWide>
+__device__ __forceinline__ void bitmap_build(int *__restrict__ tbl, const int32_t *__restrict__ row,
+                                             int dc, int32_t lo, uint32_t sentinel) {
+  const int t0 = kBlockWide ? (int)threadIdx.x : lane_id();
+  const int stride = kBlockWide ? (int)blockDim.x : kWave;
+  const int n4 = (int)((sentinel >> 7) + 1);  // int4 groups covering words 0 .. sentinel / 32
+  int4 *tbl4 = reinterpret_cast<int4 *>(tbl);
+  for (int i = t0; i < n4; i += stride) tbl4[i] = make_int4(0, 0, 0, 0);
+  if constexpr (kBlockWide) __syncthreads(); else wave_lds_sync();
+  uint32_t *words = reinterpret_cast<uint32_t *>(tbl);
+  for (int i = t0; i < dc; i += stride) {
+    const uint32_t off = (uint32_t)(row[i] - lo);
+    atomicOr(&words[off >> 5], 1u << (off & 31));
+  }
+  if constexpr (kBlockWide) __syncthreads(); else wave_lds_sync();
+}
+
+// The bitmap form fits a table of kInts ints when the row's id range plus the
+// sentinel bit fits its bits; *lo and *sentinel receive the bitmap's origin
+// and sentinel index.  Wave-uniform (every lane reads the same two ids).
+template <int kInts>
+__device__ __forceinline__ bool bitmap_fits(const int32_t *__restrict__ row, int dc, int32_t *lo,
+                                            uint32_t *sentinel) {
+  const int32_t a = row[0], b = row[dc - 1];
+  *lo = a;
+  *sentinel = (uint32_t)(b - a) + 1u;
+  return (uint64_t)(uint32_t)(b - a) + 2u <= (uint64_t)kInts * 32u;
+}
+
 // 0 iff x is one of the bucket's four slots (distance by xor, straight-line
 // VALU work with no per-slot branches).
 __device__ __forceinline__ uint32_t bucket_miss(int4 v, int x) {
@@ -463,15 +502,31 @@ struct IdOffset<true> {
   }
 };
 
+// The center row's set for one work item: an LDS hash table (tbl, shift,
+// mask of table_build), an LDS bitmap (tbl, lo, sentinel of bitmap_build) or
+// the row itself in global memory (cb, dc).
+struct CenterSet {
+  const int4 *tbl;
+  int shift;
+  uint32_t mask;
+  int32_t lo;
+  uint32_t sentinel;
+  int64_t cb;
+  int dc;
+};
+
 // Streams the rows of others[ob:oe] as one flattened sequence, U ids
-// per lane in flight, and counts the ids found in the center's set.
-template <class I, bool kLds, int U, class O, bool kNarrow>
+// per lane in flight, and counts the ids found in the center's set
+// (kMode: kProbeGlobal, kProbeHash or kProbeBitmap).
+template <class I, int kMode, int U, class O, bool kNarrow>
 __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__restrict__ col,
                                                   O others, int64_t ob,
                                                   int64_t oe, int *__restrict__ seg_end,
                                                   int64_t *__restrict__ seg_base,
-                                                  const int4 *__restrict__ tbl, int shift,
-                                                  uint32_t mask, int64_t cb, int dc) {
+                                                  const CenterSet &C) {
+  const int4 *__restrict__ tbl = C.tbl;
+  const int shift = C.shift;
+  const uint32_t mask = C.mask;
   const int lane = lane_id();
   uint32_t acc = 0;  // at most chunk * kBigMaxDeg hits per call
   for (int64_t base = ob; base < oe; base += kWave) {
@@ -520,7 +575,17 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
           }
         }
       }
-      if constexpr (kLds) {
+      if constexpr (kMode == kProbeBitmap) {
+        // offsets outside the row's range (and the -2 padding) clamp to the
+        // clear sentinel bit: one LDS word read and one bit extract per id
+        const uint32_t *__restrict__ words = reinterpret_cast<const uint32_t *>(tbl);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t off = min((uint32_t)(xs[u] - C.lo), C.sentinel);
+          // v_bfe_u32 takes the bit position from the offset's low 5 bits
+          acc += __builtin_amdgcn_ubfe(words[off >> 5], off, 1u);
+        }
+      } else if constexpr (kMode == kProbeHash) {
         // all U home buckets are read before the first compare (one LDS
         // wait), then the few lanes whose home bucket is full without the
         // id continue along the probe sequence
@@ -551,7 +616,7 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
       } else {
 #pragma unroll
         for (int u = 0; u < U; ++u)
-          if (xs[u] >= 0) acc += global_probe(col, cb, dc, xs[u]);
+          if (xs[u] >= 0) acc += global_probe(col, C.cb, C.dc, xs[u]);
       }
     }
     wave_lds_sync();
@@ -561,18 +626,42 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
 
 // stream_others with the id addressing chosen at run time (uniform per
 // kernel): 32-bit offsets when the rows lie within 2^30 ids, else 64-bit.
-template <class I, bool kLds, int U, class O>
+template <class I, int kMode, int U, class O>
 __device__ __forceinline__ uint64_t stream_others_rt(bool narrow, Rows<I> R,
                                                      const int32_t *__restrict__ col, O others,
                                                      int64_t ob, int64_t oe,
                                                      int *__restrict__ seg_end,
                                                      int64_t *__restrict__ seg_base,
-                                                     const int4 *__restrict__ tbl, int shift,
-                                                     uint32_t mask, int64_t cb, int dc) {
-  return narrow ? stream_others<I, kLds, U, O, true>(R, col, others, ob, oe, seg_end, seg_base,
-                                                     tbl, shift, mask, cb, dc)
-                : stream_others<I, kLds, U, O, false>(R, col, others, ob, oe, seg_end, seg_base,
-                                                      tbl, shift, mask, cb, dc);
+                                                     const CenterSet &C) {
+  return narrow ? stream_others<I, kMode, U, O, true>(R, col, others, ob, oe, seg_end, seg_base, C)
+                : stream_others<I, kMode, U, O, false>(R, col, others, ob, oe, seg_end, seg_base,
+                                                       C);
+}
+
+// Builds the LDS set of a center row with the lanes of one wavefront
+// (kBlockWide false) or the whole workgroup, choosing the bitmap when the
+// row's id range fits the table's bits, else the hash table when the row
+// fits it (kMaxHash ids), else the global row; then streams others[ob:oe].
+// Every lane (every thread, kBlockWide) calls this with the same item.
+template <class I, int U, class O, int kInts, int kMaxHash, bool kBlockWide>
+__device__ __forceinline__ uint64_t count_item(bool narrow, Rows<I> R,
+                                               const int32_t *__restrict__ col, O others,
+                                               int64_t ob, int64_t oe, int *__restrict__ seg_end,
+                                               int64_t *__restrict__ seg_base, int *tbl,
+                                               int64_t cb, int dc) {
+  CenterSet C{reinterpret_cast<const int4 *>(tbl), 0, 0, 0, 0, cb, dc};
+  if (bitmap_fits<kInts>(col + cb, dc, &C.lo, &C.sentinel)) {
+    bitmap_build<kInts, kBlockWide>(tbl, col + cb, dc, C.lo, C.sentinel);
+    return stream_others_rt<I, kProbeBitmap, U, O>(narrow, R, col, others, ob, oe, seg_end,
+                                                   seg_base, C);
+  }
+  if (dc <= kMaxHash) {
+    table_build<kInts, kBlockWide>(tbl, col + cb, dc, C.shift, C.mask);
+    return stream_others_rt<I, kProbeHash, U, O>(narrow, R, col, others, ob, oe, seg_end,
+                                                 seg_base, C);
+  }
+  return stream_others_rt<I, kProbeGlobal, U, O>(narrow, R, col, others, ob, oe, seg_end,
+                                                 seg_base, C);
 }
 
 // Per-graph counts of a batch: a wavefront keeps per-lane partial sums for
@@ -598,8 +687,10 @@ struct GraphAcc {
   }
 };
 
+// Four wavefronts per SIMD (at most 128 VGPRs): the LDS tables allow four
+// workgroups per CU, and the loads in flight need the occupancy.
 template <class I, int U, class O>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, 4)
 k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__restrict__ row_stats,
         const int64_t *__restrict__ cptr, O others, const int2 *__restrict__ items,
         const int64_t *__restrict__ n_items_p, const int64_t *__restrict__ iptr,
@@ -649,11 +740,8 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
       const int dc = (int)(R.end(c) - cb);
       const int64_t part = (oe - ob + kWavesPerBlock - 1) / kWavesPerBlock;
       const int64_t wb = min(oe, ob + w * part), we = min(oe, wb + part);
-      int shift;
-      uint32_t mask;
-      table_build<kBlkTblInts, true>(tbl_blk, col + cb, dc, shift, mask);
-      uint64_t acc = stream_others_rt<I, true, U, O>(narrow, R, col, others, wb, we, seg_end,
-                                                     seg_base, &tbl_all[0][0], shift, mask, cb, dc);
+      const uint64_t acc = count_item<I, U, O, kBlkTblInts, kBlkTblInts / 2 - 1, true>(
+          narrow, R, col, others, wb, we, seg_end, seg_base, tbl_blk, cb, dc);
       if (graph_of)
         gacc.add(graph_of[c], acc, count);
       else
@@ -684,16 +772,8 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
     const int64_t cb = R.begin(c);
     const int dc = (int)(R.end(c) - cb);
     uint64_t acc;
-    if (dc <= kTblMaxDeg) {
-      int shift;
-      uint32_t mask;
-      table_build<kTblInts, false>(reinterpret_cast<int *>(tbl), col + cb, dc, shift, mask);
-      acc = stream_others_rt<I, true, U, O>(narrow, R, col, others, ob, oe, seg_end, seg_base,
-                                            tbl, shift, mask, cb, dc);
-    } else {
-      acc = stream_others_rt<I, false, U, O>(narrow, R, col, others, ob, oe, seg_end, seg_base,
-                                             tbl, 0, 0, cb, dc);
-    }
+    acc = count_item<I, U, O, kTblInts, kTblMaxDeg, false>(
+        narrow, R, col, others, ob, oe, seg_end, seg_base, reinterpret_cast<int *>(tbl), cb, dc);
     if (graph_of)
       gacc.add(graph_of[c], acc, count);
     else
@@ -774,17 +854,9 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col,
     const int64_t part = (oe - ob + kWavesPerBlock - 1) / kWavesPerBlock;
     const int64_t wb = min(oe, ob + w * part), we = min(oe, wb + part);
     uint64_t acc;
-    if (dc <= kBigMaxDeg) {
-      int shift;
-      uint32_t mask;
-      table_build<kBigTblInts, true>(tbl_i, col + cb, dc, shift, mask);
-      acc = stream_others_rt<I, true, U, O>(narrow, R, col, others, wb, we, seg_end_all[w],
-                                            seg_base_all[w], tbl, shift, mask, cb, dc);
-      __syncthreads();  // the table is rebuilt for the next item
-    } else {
-      acc = stream_others_rt<I, false, U, O>(narrow, R, col, others, wb, we, seg_end_all[w],
-                                             seg_base_all[w], tbl, 0, 0, cb, dc);
-    }
+    acc = count_item<I, U, O, kBigTblInts, kBigMaxDeg, true>(
+        narrow, R, col, others, wb, we, seg_end_all[w], seg_base_all[w], tbl_i, cb, dc);
+    __syncthreads();  // the table is rebuilt for the next item
     if (graph_of)
       gacc.add(graph_of[c], acc, count);
     else
