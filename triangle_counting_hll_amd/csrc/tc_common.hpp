@@ -235,9 +235,13 @@ int bucket_sort_u8(const uint8_t *key, int64_t n, int nbuckets, int32_t *perm, i
 // (host pointer to a key value), the first pass leaves out every word equal to
 // *drop and the sorted output holds only the others; then n_dev (device) must
 // hold their number, which the later passes read (n sizes the launches).
+// With run_flags (uint8, n entries), the last pass also writes
+// run_flags[i] = 1 iff sorted word i starts a run of equal bits [bit_lo,
+// bit_hi) (i == 0 or the bits differ from word i - 1), 0 otherwise, for
+// every i below the number of sorted words.
 int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
                    uint64_t **result, hipStream_t s, const uint64_t *drop = nullptr,
-                   const unsigned long long *n_dev = nullptr);
+                   const unsigned long long *n_dev = nullptr, uint8_t *run_flags = nullptr);
 
 // Stream compaction by flag: idx_out receives the indices i with flag[i] != 0 in
 // increasing order; *d_count (device) receives their number.
@@ -248,7 +252,7 @@ int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d
 // given size (the sum of its Scratch pieces, rounded as arena_round does).
 size_t scan_scratch_bytes(int64_t n);
 size_t bucket_sort_scratch_bytes(int64_t n, int nbuckets);
-size_t radix_sort_scratch_bytes(int64_t n, int bit_lo, int bit_hi);
+size_t radix_sort_scratch_bytes(int64_t n, int bit_lo, int bit_hi, bool run_flags = false);
 size_t compact_scratch_bytes(int64_t n);
 
 }  // namespace tc

@@ -1137,14 +1137,6 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
   }
 }
 
-__global__ void k_run_flags(const uint64_t *__restrict__ s, const unsigned long long *n_heavy_p,
-                            int64_t E, int shift, uint8_t *__restrict__ flag) {
-  const int64_t H = (int64_t)*n_heavy_p;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
-       i += (int64_t)gridDim.x * blockDim.x)
-    flag[i] = (i < H && (i == 0 || (s[i] >> shift) != (s[i - 1] >> shift))) ? 1 : 0;
-}
-
 // run offsets (int64, R + 1 entries) and the bucket key of each run's center
 template <class I>
 __global__ void k_run_offsets(Rows<I> R, const uint64_t *__restrict__ s,
@@ -1212,7 +1204,7 @@ static size_t exact_sorted_scratch_bytes(int64_t n, int64_t E, int chunk, bool c
   };
   size_t b = 0;
   for (const int64_t x : pieces) b += x > 0 ? arena_round((size_t)x) : 0;
-  b += radix_sort_scratch_bytes(E, 64 - key_bits, 64);
+  b += radix_sort_scratch_bytes(E, 64 - key_bits, 64, true);
   b += compact_scratch_bytes(E);
   b += bucket_sort_scratch_bytes(n, kCenterBuckets);
   b += scan_scratch_bytes(n);
@@ -1307,13 +1299,11 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     }
     uint64_t *sorted = nullptr;
     // the first pass leaves out the light edges' sentinel words; the later
-    // passes sort the n_heavy heavy words only
+    // passes sort the n_heavy heavy words only; the last pass flags the first
+    // word of each center's run
     const uint64_t sentinel = (uint64_t)n << shift;
     TC_TRY(radix_sort_u64(packed.as<uint64_t>(), tmp.as<uint64_t>(), E, shift, 64, &sorted, s,
-                          &sentinel, n_heavy));
-    hipLaunchKernelGGL(k_run_flags, dim3(ge), dim3(kBlock), 0, s, (const uint64_t *)sorted,
-                       (const unsigned long long *)n_heavy, E, shift, flags.as<uint8_t>());
-    TC_HIP(hipGetLastError());
+                          &sentinel, n_heavy, flags.as<uint8_t>()));
     TC_TRY(compact_indices(flags.as<uint8_t>(), E, run_idx.as<int32_t>(), n_runs.as<int64_t>(),
                            s));
     hipLaunchKernelGGL(k_run_offsets<I>, dim3(gr), dim3(kBlock), 0, s, R,
