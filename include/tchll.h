@@ -361,7 +361,11 @@ int tc_vertex_graph_ids(const int64_t *node_ids, int64_t n_nodes, const int64_t 
                         int n_graphs, int32_t *graph_of, void *stream);
 
 /* counts[g] += triangles of graph g (uint64 [n_graphs]); same pipeline as
- * tc_exact_count, each triangle credited to graph_of[center]. */
+ * tc_exact_count, each triangle credited to graph_of[center].  When graph_of
+ * is nondecreasing, no edge joins two graphs and the edges are grouped by
+ * graph in that order (the packed CSR's order), the radix sort of the heavy
+ * edges covers only the bit width of the largest graph; a device check of
+ * those conditions synchronises `stream` once per call. */
 int tc_batch_exact_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
                          int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
                          int64_t n_edges, const int32_t *graph_of, int n_graphs,

@@ -1,11 +1,13 @@
 """Batched many-graph counting on the GPU: per-graph counts bit-exact against the oracle."""
 
+import ctypes
+
 import numpy as np
 import pytest
 
 from oracle import cbind
 from oracle import tc_oracle as O
-from triangle_counting_hll_amd import batch
+from triangle_counting_hll_amd import _lib, batch, gpu_hybrid
 from triangle_counting_hll_amd.graphgen import rmat
 
 pytestmark = pytest.mark.gpu
@@ -72,6 +74,51 @@ def test_batch_with_workgroup_centers(device, plan):
     b = batch.build_batch(spec, pairs=torch.from_numpy(pairs).cuda())
     got = batch.batch_exact_count(b, {"plan": plan}).cpu().numpy()
     assert np.array_equal(got, truth)
+
+
+def _clique_and_rmat_batch(parts, n_ids):
+    import torch
+
+    n_ids = np.asarray(n_ids, dtype=np.int64)
+    G = len(parts)
+    spec = batch.BatchSpec(kind=np.zeros(G, np.uint8), param=np.zeros(G, np.int32), n_ids=n_ids,
+                           n_pairs=np.array([len(p) for p in parts], dtype=np.int64),
+                           keys=np.zeros(G, np.uint64), seed=0)
+    pairs = np.concatenate([p.astype(np.int64) + off
+                            for p, off in zip(parts, spec.id_off[:-1])]).astype(np.int32)
+    truth = np.array([cbind.exact_count(*O.csr_from_pairs(p)[:2]) for p in parts], np.int64)
+    return batch.build_batch(spec, pairs=torch.from_numpy(pairs).cuda()), truth
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_batch_sort_width_follows_the_layout(device, shuffle):
+    # Edges grouped by graph in CSR order let the planner sort the heavy
+    # edges by the low bits of their center only (here 7 bits, one radix
+    # pass: the largest graph has 65..128 vertices, the batch about 11,900,
+    # 14 bits); with the edge order shuffled the layout check fails and every
+    # center bit is sorted (two passes).  Graph sizes sit on both sides of
+    # powers of two (64, 65, 128), where centers of neighbouring graphs share
+    # their low bits.
+    import torch
+
+    parts = [np.stack(np.triu_indices(64, 1), axis=1), np.stack(np.triu_indices(65, 1), axis=1),
+             rmat(7, 16, seed=4), np.stack(np.triu_indices(40, 1), axis=1)]
+    parts = parts * 40
+    b, truth = _clique_and_rmat_batch(parts, [64, 65, 128, 40] * 40)
+    eu, ev = b.csr.edge_u, b.csr.edge_v
+    if shuffle:
+        perm = torch.from_numpy(np.random.default_rng(1).permutation(b.csr.n_oriented)).cuda()
+        eu, ev = eu[perm].contiguous(), ev[perm].contiguous()
+    counts = torch.zeros(b.n_graphs, dtype=torch.int64, device="cuda")
+    rp, is64 = _lib.row_ptr_arg(b.csr.row_ptr)
+    for params in ({}, {"pack_ids": True}):
+        counts.zero_()
+        p = gpu_hybrid._exact_params(params)
+        _lib.check(_lib.load().tc_batch_exact_count(
+            _lib.ptr(rp), is64, _lib.ptr(b.csr.col_idx), b.csr.n_nodes, _lib.ptr(eu),
+            _lib.ptr(ev), b.csr.n_oriented, _lib.ptr(b.graph_of), b.n_graphs, _lib.ptr(counts),
+            ctypes.byref(p) if p is not None else None, _lib.stream_ptr()))
+        assert np.array_equal(counts.cpu().numpy(), truth), params
 
 
 def test_graph_replays_match_eager(device):

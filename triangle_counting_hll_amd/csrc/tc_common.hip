@@ -401,7 +401,7 @@ k_compact_counts(const uint8_t *__restrict__ flag, int64_t n, int64_t nchunks,
 template <bool kVec>
 __global__ void __launch_bounds__(kBlock)
 k_compact_write(const uint8_t *__restrict__ flag, int64_t n, int64_t nchunks,
-                const int64_t *__restrict__ offs, int32_t *__restrict__ idx) {
+                const int64_t *__restrict__ offs, int64_t max_out, int32_t *__restrict__ idx) {
   const int64_t chunk = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   if (chunk >= nchunks) return;
   const int lane = lane_id();
@@ -412,7 +412,15 @@ k_compact_write(const uint8_t *__restrict__ flag, int64_t n, int64_t nchunks,
   const uint32_t w[4] = {f.x, f.y, f.z, f.w};
 #pragma unroll
   for (int k = 0; k < kCompactPerLane; ++k)
-    if ((w[k >> 2] >> (8 * (k & 3))) & 0xFFu) idx[o++] = (int32_t)(first + k);
+    if ((w[k >> 2] >> (8 * (k & 3))) & 0xFFu) {
+      if (o < max_out) idx[o] = (int32_t)(first + k);
+      ++o;
+    }
+}
+
+// *dst = min(*src, cap)
+__global__ void k_copy_min_i64(const int64_t *src, int64_t *dst, int64_t cap) {
+  *dst = *src < cap ? *src : cap;
 }
 
 size_t compact_scratch_bytes(int64_t n) {
@@ -422,7 +430,8 @@ size_t compact_scratch_bytes(int64_t n) {
 }
 
 int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d_count,
-                    hipStream_t s) {
+                    hipStream_t s, int64_t max_out) {
+  if (max_out < 0 || max_out > n) max_out = n;
   const int64_t nchunks = (n + kCompactChunk - 1) / kCompactChunk;
   Scratch cnt, offs;
   TC_TRY(cnt.alloc(sizeof(int32_t) * std::max<int64_t>(nchunks, 1), s));
@@ -443,14 +452,15 @@ int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d
   if (nchunks > 0) {
     if (vec)
       hipLaunchKernelGGL(k_compact_write<true>, dim3(g), dim3(kBlock), 0, s, flag, n, nchunks,
-                         (const int64_t *)offs.as<int64_t>(), idx_out);
+                         (const int64_t *)offs.as<int64_t>(), max_out, idx_out);
     else
       hipLaunchKernelGGL(k_compact_write<false>, dim3(g), dim3(kBlock), 0, s, flag, n, nchunks,
-                         (const int64_t *)offs.as<int64_t>(), idx_out);
+                         (const int64_t *)offs.as<int64_t>(), max_out, idx_out);
     TC_HIP(hipGetLastError());
   }
   if (d_count) {
-    hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(1), 0, s, offs.as<int64_t>() + nchunks, d_count);
+    hipLaunchKernelGGL(k_copy_min_i64, dim3(1), dim3(1), 0, s, offs.as<int64_t>() + nchunks,
+                       d_count, max_out);
     TC_HIP(hipGetLastError());
   }
   return TC_OK;

@@ -236,17 +236,21 @@ int bucket_sort_u8(const uint8_t *key, int64_t n, int nbuckets, int32_t *perm, i
 // *drop and the sorted output holds only the others; then n_dev (device) must
 // hold their number, which the later passes read (n sizes the launches).
 // With run_flags (uint8, n entries), the last pass also writes
-// run_flags[i] = 1 iff sorted word i starts a run of equal bits [bit_lo,
-// bit_hi) (i == 0 or the bits differ from word i - 1), 0 otherwise, for
-// every i below the number of sorted words.
+// run_flags[i] = 1 iff sorted word i starts a run of equal bits [bit_lo, 64)
+// (i == 0 or those bits differ from word i - 1), 0 otherwise, for every i
+// below the number of sorted words.  With bit_hi < 64 the words are ordered by
+// bits [bit_lo, bit_hi) only, so one value of bits [bit_lo, 64) can form
+// several runs unless the caller's input order keeps its words together.
 int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
                    uint64_t **result, hipStream_t s, const uint64_t *drop = nullptr,
                    const unsigned long long *n_dev = nullptr, uint8_t *run_flags = nullptr);
 
 // Stream compaction by flag: idx_out receives the indices i with flag[i] != 0 in
-// increasing order; *d_count (device) receives their number.
+// increasing order; *d_count (device) receives their number.  With max_out
+// >= 0, only the first max_out of them are written and *d_count is at most
+// max_out (idx_out then needs max_out entries, not n).
 int compact_indices(const uint8_t *flag, int64_t n, int32_t *idx_out, int64_t *d_count,
-                    hipStream_t s);
+                    hipStream_t s, int64_t max_out = -1);
 
 // Scratch bytes each helper above carves from an arena for a problem of the
 // given size (the sum of its Scratch pieces, rounded as arena_round does).

@@ -1216,7 +1216,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              const uint8_t *skip, const int32_t *graph_of,
                              unsigned long long *count, int light_max, int chunk, int wpc,
                              int unroll, int blk_bits, bool pack_ids, bool wide,
-                             hipStream_t s, const HybridRoute *hr) {
+                             int sort_bits, hipStream_t s, const HybridRoute *hr) {
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;
   if (!eu) {
@@ -1302,10 +1302,18 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     // passes sort the n_heavy heavy words only; the last pass flags the first
     // word of each center's run
     const uint64_t sentinel = (uint64_t)n << shift;
-    TC_TRY(radix_sort_u64(packed.as<uint64_t>(), tmp.as<uint64_t>(), E, shift, 64, &sorted, s,
-                          &sentinel, n_heavy, flags.as<uint8_t>()));
+    // With sort_bits, only the center's low sort_bits bits are sorted: the
+    // caller guarantees that words of centers with equal low bits never
+    // interleave in packed order (a batch whose edges are grouped by graph,
+    // each graph at most 2^sort_bits vertices), so each center's words still
+    // form one run.  The run flags compare every center bit, so a run never
+    // mixes centers; the compaction keeps at most max_runs runs whatever the
+    // input, so the run arrays cannot overflow.
+    const int sort_hi = (sort_bits > 0 && sort_bits < key_bits) ? shift + sort_bits : 64;
+    TC_TRY(radix_sort_u64(packed.as<uint64_t>(), tmp.as<uint64_t>(), E, shift, sort_hi, &sorted,
+                          s, &sentinel, n_heavy, flags.as<uint8_t>()));
     TC_TRY(compact_indices(flags.as<uint8_t>(), E, run_idx.as<int32_t>(), n_runs.as<int64_t>(),
-                           s));
+                           s, max_runs));
     hipLaunchKernelGGL(k_run_offsets<I>, dim3(gr), dim3(kBlock), 0, s, R,
                        (const uint64_t *)sorted, (const int32_t *)run_idx.as<int32_t>(),
                        (const int64_t *)n_runs.as<int64_t>(), (const unsigned long long *)n_heavy,
@@ -1356,7 +1364,8 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
 int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, int64_t n,
                      const int32_t *eu, const int32_t *ev, int64_t E, const uint8_t *skip,
                      const int32_t *graph_of, unsigned long long *count,
-                     const tc_exact_params *params, hipStream_t s, const HybridRoute *hr) {
+                     const tc_exact_params *params, hipStream_t s, const HybridRoute *hr,
+                     int center_sort_bits) {
   const int light_max = (params && params->light_max > 0) ? params->light_max : 32;
   const int chunk = effective_chunk(params);
   const int wpc = (params && params->waves_per_cu > 0) ? params->waves_per_cu : 16;
@@ -1375,7 +1384,8 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
     return exact_sorted_plan(row_ptr, row_ptr_64, col, n, eu, ev, E, skip, graph_of, count,
                              light_max, chunk, wpc, unroll, blk_bits,
                              params && params->pack_ids != 0,
-                             params && params->wide_offsets != 0, s, hr);
+                             params && params->wide_offsets != 0,
+                             center_sort_bits, s, hr);
 
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;

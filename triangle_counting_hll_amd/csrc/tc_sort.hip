@@ -294,8 +294,9 @@ int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
                    uint64_t **result, hipStream_t s, const uint64_t *drop,
                    const unsigned long long *n_dev, uint8_t *run_flags) {
   *result = a;
+  // runs compare every bit from bit_lo up, also those above bit_hi
   const int run_lo = bit_lo;
-  const uint64_t run_mask = bit_hi - bit_lo >= 64 ? ~0ull : ((1ull << (bit_hi - bit_lo)) - 1);
+  const uint64_t run_mask = ~0ull >> bit_lo;
   if (n <= 1 || bit_hi <= bit_lo) {
     if (run_flags && n > 0) {
       hipLaunchKernelGGL(k_run_flags_plain, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, s,
