@@ -28,22 +28,24 @@ __global__ void k_graph_of(const int64_t *__restrict__ node_ids, int64_t n,
   }
 }
 
-// Layout facts that let the planner sort packed words by the low bits of
-// their center only.  out[0] = max over graphs of their vertex count;
-// out[1] != 0 when the layout does not qualify: graph ids decrease along the
-// vertices, an edge joins two graphs, or the edges are not grouped by graph
-// in nondecreasing order (graph_of[edge_u]).
-__global__ void k_batch_layout(const int32_t *__restrict__ graph_of, int64_t n,
+// Layout facts of a batch that let the planner narrow its packed words.
+// out[0] = max over graphs of their vertex count, out[1] = max over graphs of
+// the ids their rows span (row_ptr[last + 1] - row_ptr[first]), out[2] != 0
+// when the layout does not qualify: graph ids decrease along the vertices,
+// row_ptr decreases, an edge joins two graphs, or the edges are not grouped
+// by graph in nondecreasing order (graph_of[edge_u]).
+template <class I>
+__global__ void k_batch_layout(Rows<I> R, const int32_t *__restrict__ graph_of, int64_t n,
                                const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
                                int64_t E, unsigned long long *__restrict__ out) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long big = 0;
+  unsigned long long verts = 0, span = 0;
   bool bad = false;
   for (int64_t v = t0; v < n; v += stride) {
     const int32_t g = graph_of[v];
     const int32_t gn = v + 1 < n ? graph_of[v + 1] : g;
-    bad |= gn < g;
+    bad |= gn < g || R.end(v) < R.begin(v);
     if (v + 1 == n || gn != g) {
       // last vertex of graph g: its first vertex by binary search (exact
       // when graph_of is nondecreasing, else `bad` discards the result)
@@ -52,40 +54,65 @@ __global__ void k_batch_layout(const int32_t *__restrict__ graph_of, int64_t n,
         const int64_t mid = (lo + hi) >> 1;
         if (graph_of[mid] < g) lo = mid + 1; else hi = mid;
       }
-      big = max(big, (unsigned long long)(v - lo + 1));
+      verts = max(verts, (unsigned long long)(v - lo + 1));
+      const int64_t sp = R.end(v) - R.begin(lo);
+      span = max(span, (unsigned long long)max<int64_t>(sp, 0));
     }
   }
   for (int64_t e = t0; e < E; e += stride) {
     const int32_t g = graph_of[eu[e]];
     bad |= graph_of[ev[e]] != g || (e + 1 < E && graph_of[eu[e + 1]] < g);
   }
-  if (big) atomicMax(&out[0], big);
-  if (bad) atomicOr(&out[1], 1ull);
+  if (verts) atomicMax(&out[0], verts);
+  if (span) atomicMax(&out[1], span);
+  if (bad) atomicOr(&out[2], 1ull);
 }
 
-// Low center bits the batched planner sorts by: the bit width of the largest
-// graph when k_batch_layout's conditions hold (every center's packed words
-// are then consecutive among the words whose centers share those bits), else
-// 0 (all center bits).  Synchronises s.
-static int batch_center_bits(const int32_t *graph_of, int64_t n, const int32_t *eu,
-                             const int32_t *ev, int64_t E, hipStream_t s, int *bits) {
-  *bits = 0;
+static int bit_width(unsigned long long x) {
+  int b = 0;
+  while (b < 63 && (1ull << b) <= x) ++b;
+  return b;
+}
+
+// Narrowed planner fields for a batch when k_batch_layout's conditions hold,
+// else 0 (the single-graph layout):
+//   *sort_bits: low center bits the radix sort orders by, the bit width of
+//     the largest graph's vertex count minus one (every center's packed
+//     words are then consecutive among those whose centers share the bits);
+//   *rel_bits: width of a row start stored relative to the center's row
+//     start (PackLayout), one more than the bit width of the largest span of
+//     ids a graph's rows cover; 0 when it is not narrower than row_ptr[n].
+// Synchronises s (nothing is checked, and both stay 0, while s is being
+// captured into a graph).
+static int batch_layout_bits(const void *row_ptr, int row_ptr_64, const int32_t *graph_of,
+                             int64_t n, const int32_t *eu, const int32_t *ev, int64_t E,
+                             hipStream_t s, int *sort_bits, int *rel_bits) {
+  *sort_bits = 0;
+  *rel_bits = 0;
   if (n <= 0 || E <= 0) return TC_OK;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  TC_HIP(hipStreamIsCapturing(s, &cap));
+  if (cap != hipStreamCaptureStatusNone) return TC_OK;
   Scratch out;
-  TC_TRY(out.alloc(2 * sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kWave), 0, s, out.as<unsigned long long>(), 2,
-                     0ull);
-  hipLaunchKernelGGL(k_batch_layout, dim3(grid_for(std::max(n, E), kBlock, 4096)), dim3(kBlock),
-                     0, s, graph_of, n, eu, ev, E, out.as<unsigned long long>());
+  TC_TRY(out.alloc(4 * sizeof(unsigned long long), s));
+  unsigned long long *o = out.as<unsigned long long>();
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kWave), 0, s, o, 3, 0ull);
+  TC_DISPATCH_ROWS(row_ptr_64, I, {
+    hipLaunchKernelGGL(k_batch_layout<I>, dim3(grid_for(std::max(n, E), kBlock, 4096)),
+                       dim3(kBlock), 0, s, Rows<I>{(const I *)row_ptr}, graph_of, n, eu, ev, E,
+                       o);
+    // out[3] = row_ptr[n]
+    TC_HIP(hipMemcpyAsync(o + 3, (const I *)row_ptr + n, sizeof(I), hipMemcpyDeviceToDevice, s));
+  });
   TC_HIP(hipGetLastError());
-  unsigned long long h[2] = {0, 0};
-  TC_HIP(hipMemcpyAsync(h, out.ptr, sizeof(h), hipMemcpyDeviceToHost, s));
+  unsigned long long h[4] = {0, 0, 0, 0};
+  TC_HIP(hipMemcpyAsync(h, o, sizeof(h), hipMemcpyDeviceToHost, s));
   TC_HIP(hipStreamSynchronize(s));
-  if (h[1] == 0 && h[0] > 0) {
-    int b = 1;
-    while (b < 62 && (1ull << b) < h[0]) ++b;
-    *bits = b;
-  }
+  if (h[2] != 0 || h[0] == 0) return TC_OK;
+  *sort_bits = std::max(1, bit_width(h[0] - 1));
+  const int rb = bit_width(h[1]) + 1;
+  const unsigned long long total = row_ptr_64 ? h[3] : (unsigned long long)(uint32_t)h[3];
+  if (rb < std::max(1, bit_width(total))) *rel_bits = rb;
   return TC_OK;
 }
 
@@ -140,12 +167,12 @@ extern "C" int tc_batch_exact_count(const void *row_ptr, int row_ptr_64, const i
   TC_TRY(check_batch_args(row_ptr, col_idx, n_nodes, edge_u, edge_v, n_edges, graph_of, n_graphs,
                           counts));
   TC_TRY(ensure_init());
-  int bits = 0;
-  TC_TRY(batch_center_bits(graph_of, n_nodes, edge_u, edge_v, n_edges, (hipStream_t)stream,
-                           &bits));
+  int bits = 0, rel = 0;
+  TC_TRY(batch_layout_bits(row_ptr, row_ptr_64, graph_of, n_nodes, edge_u, edge_v, n_edges,
+                           (hipStream_t)stream, &bits, &rel));
   return exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges, nullptr,
                           graph_of, (unsigned long long *)counts, params, (hipStream_t)stream,
-                          nullptr, bits);
+                          nullptr, bits, rel);
 }
 
 extern "C" int tc_batch_graph_create(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
@@ -170,16 +197,17 @@ extern "C" int tc_batch_graph_create(const void *row_ptr, int row_ptr_64, const 
 
   // 1. eager run that measures the workspace (its counts are overwritten by
   //    the first replay, which starts by clearing them)
-  int bits = 0;
+  int bits = 0, rel = 0;
   {
-    const int rcb = batch_center_bits(graph_of, n_nodes, edge_u, edge_v, n_edges, s, &bits);
+    const int rcb = batch_layout_bits(row_ptr, row_ptr_64, graph_of, n_nodes, edge_u, edge_v,
+                                      n_edges, s, &bits, &rel);
     if (rcb) return fail(rcb);
   }
   Arena measure;
   current_arena() = &measure;
   int rc = exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges,
                             nullptr, graph_of, (unsigned long long *)counts, params, s, nullptr,
-                            bits);
+                            bits, rel);
   current_arena() = nullptr;
   if (rc) return fail(rc);
   if (hipStreamSynchronize(s) != hipSuccess) {
@@ -211,7 +239,7 @@ extern "C" int tc_batch_graph_create(const void *row_ptr, int row_ptr_64, const 
   hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n_graphs, kBlock, 1024)), dim3(kBlock), 0, cs,
                      (unsigned long long *)counts, (int64_t)n_graphs, 0ull);
   rc = exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges, nullptr,
-                        graph_of, (unsigned long long *)counts, params, cs, nullptr, bits);
+                        graph_of, (unsigned long long *)counts, params, cs, nullptr, bits, rel);
   current_arena() = nullptr;
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(cs, &g);

@@ -424,15 +424,27 @@ __device__ __forceinline__ bool rows_narrow(const unsigned long long *stats) {
 //               in bits [0, lbits), so the intersection kernel reads no row
 //               offsets for the streamed rows, or
 //   id layout:  the other's id in the low 32 bits.
-// The row layout is used when the start field (lbits = shift - bit_length of
-// row_ptr[n]) leaves at least one bit for lengths and every forward degree is
-// below 2^lbits, unless force_ids.
+// The row layout is used when the start field (lbits = shift - its width)
+// leaves at least one bit for lengths and every forward degree is below
+// 2^lbits, unless force_ids.  The start field holds row_ptr[other] in
+// bit_length(row_ptr[n]) bits, or, with rel_bits > 0, row_ptr[other] -
+// row_ptr[center] + 2^(rel_bits - 1) in rel_bits bits: the caller guarantees
+// |row_ptr[other] - row_ptr[center]| < 2^(rel_bits - 1) for every edge (a
+// batch of graphs whose rows each span fewer ids than that).
 struct PackLayout {
   int shift;
   int force_ids;
   const unsigned long long *stats;
+  int rel_bits = 0;
   __device__ __forceinline__ int lbits() const {
-    return shift - max(1, bit_length_u64(stats[1]));
+    return shift - (rel_bits > 0 ? rel_bits : max(1, bit_length_u64(stats[1])));
+  }
+  // start field of an other's row (row layout)
+  __device__ __forceinline__ uint64_t start_field(int64_t other_start,
+                                                  int64_t center_start) const {
+    return rel_bits > 0
+               ? (uint64_t)(other_start - center_start + (int64_t(1) << (rel_bits - 1)))
+               : (uint64_t)other_start;
   }
   __device__ __forceinline__ bool rows() const {
     const int lb = lbits();
@@ -448,7 +460,8 @@ struct OthersIds {
   const int32_t *p;
   __device__ __forceinline__ void resolve() {}
   template <class I>
-  __device__ __forceinline__ void row(Rows<I> R, int64_t k, int64_t &st, int &len) const {
+  __device__ __forceinline__ void row(Rows<I> R, int64_t k, int64_t, int64_t &st,
+                                      int &len) const {
     const int32_t o = p[k];
     st = R.begin(o);
     len = (int)(R.end(o) - st);
@@ -464,11 +477,14 @@ struct OthersPacked {
     rows_layout = L.rows();
     lb = L.lbits();
   }
+  // cb: row_ptr[center] of the work item (the base of relative starts)
   template <class I>
-  __device__ __forceinline__ void row(Rows<I> R, int64_t k, int64_t &st, int &len) const {
+  __device__ __forceinline__ void row(Rows<I> R, int64_t k, int64_t cb, int64_t &st,
+                                      int &len) const {
     const uint64_t w = p[k];
     if (rows_layout) {
       st = (int64_t)((w >> lb) & ((1ull << (L.shift - lb)) - 1));
+      if (L.rel_bits > 0) st += cb - (int64_t(1) << (L.rel_bits - 1));
       len = (int)(w & ((1ull << lb) - 1));
     } else {
       const int32_t o = (int32_t)(uint32_t)w;
@@ -533,7 +549,7 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
     const int64_t k = base + lane;
     int len = 0;
     int64_t st = 0;
-    if (k < oe) others.row(R, k, st, len);
+    if (k < oe) others.row(R, k, C.cb, st, len);
     const int inc = wave_inclusive_scan(len);
     const int total = __shfl(inc, kWave - 1);
     seg_end[lane] = inc;
@@ -1079,7 +1095,8 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
           const uint32_t c = (uint32_t)(cv ? v : u), o = (uint32_t)(cv ? u : v);
           word = (uint64_t)c << L.shift;
           if (rows_layout)
-            word |= ((uint64_t)(cv ? ub : vb) << lbits) | (uint64_t)(cv ? du : dv);
+            word |= (L.start_field(cv ? ub : vb, cv ? vb : ub) << lbits) |
+                    (uint64_t)(cv ? du : dv);
           else
             word |= o;
           ++nh;
@@ -1216,7 +1233,8 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              const uint8_t *skip, const int32_t *graph_of,
                              unsigned long long *count, int light_max, int chunk, int wpc,
                              int unroll, int blk_bits, bool pack_ids, bool wide,
-                             int sort_bits, hipStream_t s, const HybridRoute *hr) {
+                             int sort_bits, int rel_bits, hipStream_t s,
+                             const HybridRoute *hr) {
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;
   if (!eu) {
@@ -1253,7 +1271,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   unsigned long long *next_item = ctr.as<unsigned long long>();
   unsigned long long *n_heavy = next_item + 1;
   unsigned long long *row_stats = next_item + 4;  // 3 entries
-  const PackLayout layout{shift, pack_ids ? 1 : 0, row_stats};
+  const PackLayout layout{shift, pack_ids ? 1 : 0, row_stats, rel_bits};
   const unsigned long long *rows_total = csr_mode ? row_stats + 1 : nullptr;
 
   const unsigned gr = grid_for(max_runs + 1, kBlock, 16384);
@@ -1365,7 +1383,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                      const int32_t *eu, const int32_t *ev, int64_t E, const uint8_t *skip,
                      const int32_t *graph_of, unsigned long long *count,
                      const tc_exact_params *params, hipStream_t s, const HybridRoute *hr,
-                     int center_sort_bits) {
+                     int center_sort_bits, int rel_start_bits) {
   const int light_max = (params && params->light_max > 0) ? params->light_max : 32;
   const int chunk = effective_chunk(params);
   const int wpc = (params && params->waves_per_cu > 0) ? params->waves_per_cu : 16;
@@ -1385,7 +1403,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                              light_max, chunk, wpc, unroll, blk_bits,
                              params && params->pack_ids != 0,
                              params && params->wide_offsets != 0,
-                             center_sort_bits, s, hr);
+                             center_sort_bits, rel_start_bits, s, hr);
 
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;

@@ -48,7 +48,8 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                      const int32_t *eu, const int32_t *ev, int64_t E, const uint8_t *skip,
                      const int32_t *graph_of, unsigned long long *count,
                      const tc_exact_params *params, hipStream_t s,
-                     const HybridRoute *hr = nullptr, int center_sort_bits = 0);
+                     const HybridRoute *hr = nullptr, int center_sort_bits = 0,
+                     int rel_start_bits = 0);
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v);
 __global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v);
