@@ -28,24 +28,27 @@ __global__ void k_graph_of(const int64_t *__restrict__ node_ids, int64_t n,
   }
 }
 
-// Layout facts of a batch that let the planner narrow its packed words.
-// out[0] = max over graphs of their vertex count, out[1] = max over graphs of
-// the ids their rows span (row_ptr[last + 1] - row_ptr[first]), out[2] != 0
-// when the layout does not qualify: graph ids decrease along the vertices,
-// row_ptr decreases, an edge joins two graphs, or the edges are not grouped
-// by graph in nondecreasing order (graph_of[edge_u]).
+// Layout facts of a batch that let the planner narrow its packed words and
+// schedule by graph.  out[0] = max over graphs of their vertex count,
+// out[1] = max over graphs of the ids their rows span (row_ptr[last + 1] -
+// row_ptr[first]), out[2] != 0 when the layout does not qualify (graph ids
+// decrease along the vertices, row_ptr decreases, an edge joins two graphs,
+// or the edges are not grouped by graph in nondecreasing order of
+// graph_of[edge_u]), out[3] = the largest forward degree.
 template <class I>
 __global__ void k_batch_layout(Rows<I> R, const int32_t *__restrict__ graph_of, int64_t n,
                                const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
                                int64_t E, unsigned long long *__restrict__ out) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long verts = 0, span = 0;
+  unsigned long long verts = 0, span = 0, dmax = 0;
   bool bad = false;
   for (int64_t v = t0; v < n; v += stride) {
     const int32_t g = graph_of[v];
     const int32_t gn = v + 1 < n ? graph_of[v + 1] : g;
-    bad |= gn < g || R.end(v) < R.begin(v);
+    const int64_t d = R.end(v) - R.begin(v);
+    bad |= gn < g || d < 0;
+    dmax = max(dmax, (unsigned long long)max<int64_t>(d, 0));
     if (v + 1 == n || gn != g) {
       // last vertex of graph g: its first vertex by binary search (exact
       // when graph_of is nondecreasing, else `bad` discards the result)
@@ -66,6 +69,7 @@ __global__ void k_batch_layout(Rows<I> R, const int32_t *__restrict__ graph_of, 
   if (verts) atomicMax(&out[0], verts);
   if (span) atomicMax(&out[1], span);
   if (bad) atomicOr(&out[2], 1ull);
+  if (dmax) atomicMax(&out[3], dmax);
 }
 
 static int bit_width(unsigned long long x) {
@@ -74,45 +78,49 @@ static int bit_width(unsigned long long x) {
   return b;
 }
 
-// Narrowed planner fields for a batch when k_batch_layout's conditions hold,
-// else 0 (the single-graph layout):
-//   *sort_bits: low center bits the radix sort orders by, the bit width of
-//     the largest graph's vertex count minus one (every center's packed
-//     words are then consecutive among those whose centers share the bits);
-//   *rel_bits: width of a row start stored relative to the center's row
-//     start (PackLayout), one more than the bit width of the largest span of
-//     ids a graph's rows cover; 0 when it is not narrower than row_ptr[n].
-// Synchronises s (nothing is checked, and both stay 0, while s is being
-// captured into a graph).
-static int batch_layout_bits(const void *row_ptr, int row_ptr_64, const int32_t *graph_of,
-                             int64_t n, const int32_t *eu, const int32_t *ev, int64_t E,
-                             hipStream_t s, int *sort_bits, int *rel_bits) {
-  *sort_bits = 0;
-  *rel_bits = 0;
+// The BatchPlan of a batch (all 0 unless k_batch_layout's conditions hold):
+//   sort_bits: the bit width of the largest graph's vertex count minus one
+//     (every center's packed words are then consecutive among those whose
+//     centers share those low bits);
+//   rel_bits: one more than the bit width of the largest span of ids a
+//     graph's rows cover (row starts relative to the center's), 0 when that
+//     is not narrower than row_ptr[n];
+//   graph_bits: the bit width of the largest graph id, when no forward
+//     degree exceeds a wavefront table (kWaveTableMaxDeg).
+// Synchronises s (nothing is checked, and the plan stays all 0, while s is
+// being captured into a graph).
+static int batch_plan(const void *row_ptr, int row_ptr_64, const int32_t *graph_of, int64_t n,
+                      const int32_t *eu, const int32_t *ev, int64_t E, hipStream_t s,
+                      BatchPlan *bp) {
+  *bp = BatchPlan{};
   if (n <= 0 || E <= 0) return TC_OK;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   TC_HIP(hipStreamIsCapturing(s, &cap));
   if (cap != hipStreamCaptureStatusNone) return TC_OK;
   Scratch out;
-  TC_TRY(out.alloc(4 * sizeof(unsigned long long), s));
+  TC_TRY(out.alloc(6 * sizeof(unsigned long long), s));
   unsigned long long *o = out.as<unsigned long long>();
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kWave), 0, s, o, 3, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kWave), 0, s, o, 6, 0ull);
   TC_DISPATCH_ROWS(row_ptr_64, I, {
     hipLaunchKernelGGL(k_batch_layout<I>, dim3(grid_for(std::max(n, E), kBlock, 4096)),
                        dim3(kBlock), 0, s, Rows<I>{(const I *)row_ptr}, graph_of, n, eu, ev, E,
                        o);
-    // out[3] = row_ptr[n]
-    TC_HIP(hipMemcpyAsync(o + 3, (const I *)row_ptr + n, sizeof(I), hipMemcpyDeviceToDevice, s));
+    // out[4] = row_ptr[n], out[5] = graph_of[n - 1] (low bytes)
+    TC_HIP(hipMemcpyAsync(o + 4, (const I *)row_ptr + n, sizeof(I), hipMemcpyDeviceToDevice, s));
   });
+  TC_HIP(hipMemcpyAsync(o + 5, graph_of + (n - 1), sizeof(int32_t), hipMemcpyDeviceToDevice, s));
   TC_HIP(hipGetLastError());
-  unsigned long long h[4] = {0, 0, 0, 0};
+  unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
   TC_HIP(hipMemcpyAsync(h, o, sizeof(h), hipMemcpyDeviceToHost, s));
   TC_HIP(hipStreamSynchronize(s));
   if (h[2] != 0 || h[0] == 0) return TC_OK;
-  *sort_bits = std::max(1, bit_width(h[0] - 1));
+  bp->sort_bits = std::max(1, bit_width(h[0] - 1));
   const int rb = bit_width(h[1]) + 1;
-  const unsigned long long total = row_ptr_64 ? h[3] : (unsigned long long)(uint32_t)h[3];
-  if (rb < std::max(1, bit_width(total))) *rel_bits = rb;
+  const unsigned long long total = row_ptr_64 ? h[4] : (unsigned long long)(uint32_t)h[4];
+  if (rb < std::max(1, bit_width(total))) bp->rel_bits = rb;
+  const int32_t last_graph = (int32_t)(uint32_t)h[5];
+  if (last_graph >= 0 && h[3] <= (unsigned long long)kWaveTableMaxDeg)
+    bp->graph_bits = std::max(1, bit_width((unsigned long long)last_graph));
   return TC_OK;
 }
 
@@ -167,12 +175,12 @@ extern "C" int tc_batch_exact_count(const void *row_ptr, int row_ptr_64, const i
   TC_TRY(check_batch_args(row_ptr, col_idx, n_nodes, edge_u, edge_v, n_edges, graph_of, n_graphs,
                           counts));
   TC_TRY(ensure_init());
-  int bits = 0, rel = 0;
-  TC_TRY(batch_layout_bits(row_ptr, row_ptr_64, graph_of, n_nodes, edge_u, edge_v, n_edges,
-                           (hipStream_t)stream, &bits, &rel));
+  BatchPlan bp;
+  TC_TRY(batch_plan(row_ptr, row_ptr_64, graph_of, n_nodes, edge_u, edge_v, n_edges,
+                    (hipStream_t)stream, &bp));
   return exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges, nullptr,
                           graph_of, (unsigned long long *)counts, params, (hipStream_t)stream,
-                          nullptr, bits, rel);
+                          nullptr, &bp);
 }
 
 extern "C" int tc_batch_graph_create(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
@@ -197,17 +205,17 @@ extern "C" int tc_batch_graph_create(const void *row_ptr, int row_ptr_64, const 
 
   // 1. eager run that measures the workspace (its counts are overwritten by
   //    the first replay, which starts by clearing them)
-  int bits = 0, rel = 0;
+  BatchPlan bp;
   {
-    const int rcb = batch_layout_bits(row_ptr, row_ptr_64, graph_of, n_nodes, edge_u, edge_v,
-                                      n_edges, s, &bits, &rel);
+    const int rcb = batch_plan(row_ptr, row_ptr_64, graph_of, n_nodes, edge_u, edge_v, n_edges,
+                               s, &bp);
     if (rcb) return fail(rcb);
   }
   Arena measure;
   current_arena() = &measure;
   int rc = exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges,
                             nullptr, graph_of, (unsigned long long *)counts, params, s, nullptr,
-                            bits, rel);
+                            &bp);
   current_arena() = nullptr;
   if (rc) return fail(rc);
   if (hipStreamSynchronize(s) != hipSuccess) {
@@ -239,7 +247,7 @@ extern "C" int tc_batch_graph_create(const void *row_ptr, int row_ptr_64, const 
   hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n_graphs, kBlock, 1024)), dim3(kBlock), 0, cs,
                      (unsigned long long *)counts, (int64_t)n_graphs, 0ull);
   rc = exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges, nullptr,
-                        graph_of, (unsigned long long *)counts, params, cs, nullptr, bits, rel);
+                        graph_of, (unsigned long long *)counts, params, cs, nullptr, &bp);
   current_arena() = nullptr;
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(cs, &g);

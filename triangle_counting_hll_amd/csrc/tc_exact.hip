@@ -35,7 +35,7 @@
 namespace tc {
 
 constexpr int kTblInts = 2048;    // per-wave LDS hash table: 512 buckets x 4 slots (8 KiB)
-constexpr int kTblMaxDeg = 1023;  // longer center rows: one table per workgroup
+constexpr int kTblMaxDeg = kWaveTableMaxDeg;  // longer center rows: one table per workgroup
 constexpr int kBlkMinBits = 10;   // default: d+ in [512, 4095] take the workgroup's four wave tables
 constexpr int kBlkTblInts = 4 * kTblInts;  // joined into one (2048 buckets, 32 KiB)
 constexpr int kBigMinBits = 13;   // d+ >= 4096: the 128 KiB workgroup kernel
@@ -259,6 +259,8 @@ k_scatter_others(Rows<I> R, const int32_t *__restrict__ eu, const int32_t *__res
 constexpr int kCenterBuckets = 32;
 constexpr int kBigCenterBits = 8;  // centers with d+ >= 128 get one item per queue ticket
 constexpr int kSmallBatch = 16;    // items per ticket for the shorter centers
+constexpr int kXcds = 8;           // XCDs of an MI355X (one L2 each)
+constexpr int kXcdStride = 16;     // u64 counters between two XCD queues (128 B)
 
 template <class I>
 __global__ void k_center_keys(Rows<I> R, const int32_t *__restrict__ center_cnt, int64_t n,
@@ -705,7 +707,7 @@ struct GraphAcc {
 
 // Four wavefronts per SIMD (at most 128 VGPRs): the LDS tables allow four
 // workgroups per CU, and the loads in flight need the occupancy.
-template <class I, int U, class O>
+template <class I, int U, class O, bool kXcdLocal>
 __global__ void __launch_bounds__(kBlock, 4)
 k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__restrict__ row_stats,
         const int64_t *__restrict__ cptr, O others, const int2 *__restrict__ items,
@@ -713,7 +715,8 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
         const int64_t *__restrict__ n_huge_runs_p, const int64_t *__restrict__ n_first_runs_p,
         const int64_t *__restrict__ n_big_centers_p, int chunk,
         unsigned long long *__restrict__ next_item, unsigned long long *__restrict__ next_block,
-        const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count) {
+        unsigned long long *__restrict__ xcd_next, const int32_t *__restrict__ graph_of,
+        unsigned long long *__restrict__ count) {
   __shared__ int4 tbl_all[kWavesPerBlock][kTblInts / 4];
   __shared__ int seg_end_all[kWavesPerBlock][kWave];
   __shared__ int64_t seg_base_all[kWavesPerBlock][kWave];
@@ -739,8 +742,9 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
 
   // Workgroup phase.  Every thread sees the same ticket, so the loop exits
   // together; the barrier at its top keeps the table and the ticket in place
-  // until all wavefronts are done with the previous item.
-  if (first > huge) {
+  // until all wavefronts are done with the previous item.  (The XCD-local
+  // form has no workgroup items.)
+  if (!kXcdLocal && first > huge) {
     int *tbl_blk = reinterpret_cast<int *>(&tbl_all[0][0]);
     while (true) {
       __syncthreads();
@@ -765,35 +769,55 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
     }
   }
 
-  int64_t it = 0, it_end = 0;
-
+  // Wavefront phase: each ticket gives a range of items [it0, it1).  By
+  // default one counter hands out the long centers' items one per ticket,
+  // then kSmallBatch per ticket.  In the XCD-local form (batches) the items,
+  // ordered by graph, are split into kXcds consecutive ranges with one
+  // counter each in xcd_next (kXcdStride apart): workgroup b runs on XCD
+  // b % kXcds (round-robin dispatch), so its wavefronts start on that range
+  // and the graphs of one range share one L2; a wavefront whose range is used
+  // up moves on to the next and leaves after trying all of them.
+  [[maybe_unused]] int q = (int)(blockIdx.x % kXcds), tried = 0;
   while (true) {
-    if (it >= it_end) {
-      unsigned long long ticket = 0;
+    int64_t it0, it1;
+    unsigned long long ticket = 0;
+    if constexpr (kXcdLocal) {
+      const int64_t lo = n_items * q / kXcds, hi = n_items * (q + 1) / kXcds;
+      if (lane == 0) ticket = atomicAdd(&xcd_next[q * kXcdStride], 1ull);
+      it0 = lo + (int64_t)__shfl(ticket, 0) * kSmallBatch;
+      it1 = min(it0 + (int64_t)kSmallBatch, hi);
+      if (it0 >= hi) {
+        q = (q + 1) % kXcds;
+        if (++tried == kXcds) break;
+        continue;
+      }
+    } else {
       if (lane == 0) ticket = atomicAdd(next_item, 1ull);
       const int64_t tk = (int64_t)__shfl(ticket, 0);
       if (tk < n_big - first) {
-        it = first + tk;
-        it_end = it + 1;
+        it0 = first + tk;
+        it1 = it0 + 1;
       } else {
-        it = n_big + (tk - (n_big - first)) * kSmallBatch;
-        it_end = min(it + (int64_t)kSmallBatch, n_items);
+        it0 = n_big + (tk - (n_big - first)) * kSmallBatch;
+        it1 = min(it0 + (int64_t)kSmallBatch, n_items);
       }
-      if (it >= n_items) break;
+      if (it0 >= n_items) break;
     }
-    const int2 item = items[it++];
-    const int64_t ob = cptr[item.x] + (int64_t)item.y * chunk;
-    const int64_t oe = min(cptr[item.x + 1], ob + (int64_t)chunk);
-    const int32_t c = others.center(item.x, ob);
-    const int64_t cb = R.begin(c);
-    const int dc = (int)(R.end(c) - cb);
-    uint64_t acc;
-    acc = count_item<I, U, O, kTblInts, kTblMaxDeg, false>(
-        narrow, R, col, others, ob, oe, seg_end, seg_base, reinterpret_cast<int *>(tbl), cb, dc);
-    if (graph_of)
-      gacc.add(graph_of[c], acc, count);
-    else
-      acc_total += acc;
+    for (int64_t it = it0; it < it1; ++it) {
+      const int2 item = items[it];
+      const int64_t ob = cptr[item.x] + (int64_t)item.y * chunk;
+      const int64_t oe = min(cptr[item.x + 1], ob + (int64_t)chunk);
+      const int32_t c = others.center(item.x, ob);
+      const int64_t cb = R.begin(c);
+      const int dc = (int)(R.end(c) - cb);
+      const uint64_t acc = count_item<I, U, O, kTblInts, kTblMaxDeg, false>(
+          narrow, R, col, others, ob, oe, seg_end, seg_base, reinterpret_cast<int *>(tbl), cb,
+          dc);
+      if (graph_of)
+        gacc.add(graph_of[c], acc, count);
+      else
+        acc_total += acc;
+    }
   }
   if (graph_of) {
     gacc.flush(count);
@@ -897,14 +921,14 @@ static auto heavy_big_kernel(int unroll) -> decltype(&k_heavy_big<I, 4, O>) {
   }
 }
 
-template <class I, class O>
-static auto heavy_kernel(int unroll) -> decltype(&k_heavy<I, 4, O>) {
+template <class I, class O, bool kXcdLocal = false>
+static auto heavy_kernel(int unroll) -> decltype(&k_heavy<I, 4, O, kXcdLocal>) {
   switch (unroll) {
-    case 1: return &k_heavy<I, 1, O>;
-    case 2: return &k_heavy<I, 2, O>;
-    case 4: return &k_heavy<I, 4, O>;
-    case 8: return &k_heavy<I, 8, O>;
-    default: return &k_heavy<I, 16, O>;
+    case 1: return &k_heavy<I, 1, O, kXcdLocal>;
+    case 2: return &k_heavy<I, 2, O, kXcdLocal>;
+    case 4: return &k_heavy<I, 4, O, kXcdLocal>;
+    case 8: return &k_heavy<I, 8, O, kXcdLocal>;
+    default: return &k_heavy<I, 16, O, kXcdLocal>;
   }
 }
 
@@ -1175,6 +1199,33 @@ __global__ void k_run_offsets(Rows<I> R, const uint64_t *__restrict__ s,
   }
 }
 
+// Sort keys that order runs by graph: (graph of the run's center << 32 | run)
+// for r < n_runs, ~0 (left out by the sort) for the other slots.
+__global__ void k_run_graph_keys(const uint64_t *__restrict__ s,
+                                 const int64_t *__restrict__ run_ptr,
+                                 const int64_t *__restrict__ n_runs_p, int64_t max_runs, int shift,
+                                 const int32_t *__restrict__ graph_of,
+                                 uint64_t *__restrict__ keys) {
+  const int64_t nr = *n_runs_p;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < max_runs;
+       r += (int64_t)gridDim.x * blockDim.x)
+    keys[r] = r < nr ? ((uint64_t)(uint32_t)graph_of[s[run_ptr[r]] >> shift] << 32) | (uint64_t)r
+                     : ~0ull;
+}
+
+// perm[i] = run of the i-th sorted key; the bucket boundaries put every run
+// in the wavefront phase (starts[0 .. kCenterBuckets - 1] = 0,
+// starts[kCenterBuckets] = n_runs).
+__global__ void k_graph_perm(const uint64_t *__restrict__ sorted_keys,
+                             const int64_t *__restrict__ n_runs_p, int64_t max_runs,
+                             int32_t *__restrict__ perm, int64_t *__restrict__ starts) {
+  const int64_t nr = *n_runs_p;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = t0; i < max_runs; i += (int64_t)gridDim.x * blockDim.x)
+    perm[i] = i < nr ? (int32_t)(uint32_t)sorted_keys[i] : 0;
+  if (t0 <= kCenterBuckets) starts[t0] = t0 == kCenterBuckets ? nr : 0;
+}
+
 __global__ void k_run_item_counts(const int64_t *__restrict__ run_ptr,
                                   const int32_t *__restrict__ perm,
                                   const int64_t *__restrict__ n_runs_p, int64_t max_runs, int chunk,
@@ -1233,8 +1284,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              const uint8_t *skip, const int32_t *graph_of,
                              unsigned long long *count, int light_max, int chunk, int wpc,
                              int unroll, int blk_bits, bool pack_ids, bool wide,
-                             int sort_bits, int rel_bits, hipStream_t s,
-                             const HybridRoute *hr) {
+                             const BatchPlan &bp, hipStream_t s, const HybridRoute *hr) {
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;
   if (!eu) {
@@ -1268,10 +1318,18 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   const int64_t max_items = max_runs + E / chunk + 1;
   TC_TRY(items.alloc(sizeof(int2) * max_items, s));
   TC_TRY(ctr.alloc(7 * sizeof(unsigned long long), s));
+  // batches: runs ordered by graph, one item queue per XCD
+  const bool xcd_local = bp.graph_bits > 0 && graph_of != nullptr;
+  Scratch xcd_ctr;
+  if (xcd_local) {
+    TC_TRY(xcd_ctr.alloc(sizeof(unsigned long long) * kXcds * kXcdStride, s));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kBlock), 0, s, xcd_ctr.as<unsigned long long>(),
+                       (int64_t)kXcds * kXcdStride, 0ull);
+  }
   unsigned long long *next_item = ctr.as<unsigned long long>();
   unsigned long long *n_heavy = next_item + 1;
   unsigned long long *row_stats = next_item + 4;  // 3 entries
-  const PackLayout layout{shift, pack_ids ? 1 : 0, row_stats, rel_bits};
+  const PackLayout layout{shift, pack_ids ? 1 : 0, row_stats, bp.rel_bits};
   const unsigned long long *rows_total = csr_mode ? row_stats + 1 : nullptr;
 
   const unsigned gr = grid_for(max_runs + 1, kBlock, 16384);
@@ -1327,7 +1385,8 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     // form one run.  The run flags compare every center bit, so a run never
     // mixes centers; the compaction keeps at most max_runs runs whatever the
     // input, so the run arrays cannot overflow.
-    const int sort_hi = (sort_bits > 0 && sort_bits < key_bits) ? shift + sort_bits : 64;
+    const int sort_hi = (bp.sort_bits > 0 && bp.sort_bits < key_bits) ? shift + bp.sort_bits
+                                                                       : 64;
     TC_TRY(radix_sort_u64(packed.as<uint64_t>(), tmp.as<uint64_t>(), E, shift, sort_hi, &sorted,
                           s, &sentinel, n_heavy, flags.as<uint8_t>()));
     TC_TRY(compact_indices(flags.as<uint8_t>(), E, run_idx.as<int32_t>(), n_runs.as<int64_t>(),
@@ -1337,9 +1396,30 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                        (const int64_t *)n_runs.as<int64_t>(), (const unsigned long long *)n_heavy,
                        max_runs, shift, run_ptr.as<int64_t>(), rkey.as<uint8_t>());
     TC_HIP(hipGetLastError());
-    // runs ordered longest center row first
-    TC_TRY(bucket_sort_u8(rkey.as<uint8_t>(), max_runs, kCenterBuckets, perm.as<int32_t>(),
-                          starts.as<int64_t>(), s));
+    if (xcd_local) {
+      // runs ordered by graph (stable: by center inside a graph)
+      Scratch gk, gk2;
+      TC_TRY(gk.alloc(sizeof(uint64_t) * max_runs, s));
+      TC_TRY(gk2.alloc(sizeof(uint64_t) * max_runs, s));
+      hipLaunchKernelGGL(k_run_graph_keys, dim3(gr), dim3(kBlock), 0, s, (const uint64_t *)sorted,
+                         (const int64_t *)run_ptr.as<int64_t>(),
+                         (const int64_t *)n_runs.as<int64_t>(), max_runs, shift, graph_of,
+                         gk.as<uint64_t>());
+      TC_HIP(hipGetLastError());
+      uint64_t *gsorted = nullptr;
+      const uint64_t no_run = ~0ull;
+      TC_TRY(radix_sort_u64(gk.as<uint64_t>(), gk2.as<uint64_t>(), max_runs, 32,
+                            32 + bp.graph_bits, &gsorted, s, &no_run,
+                            (const unsigned long long *)n_runs.as<int64_t>()));
+      hipLaunchKernelGGL(k_graph_perm, dim3(gr), dim3(kBlock), 0, s, (const uint64_t *)gsorted,
+                         (const int64_t *)n_runs.as<int64_t>(), max_runs, perm.as<int32_t>(),
+                         starts.as<int64_t>());
+      TC_HIP(hipGetLastError());
+    } else {
+      // runs ordered longest center row first
+      TC_TRY(bucket_sort_u8(rkey.as<uint8_t>(), max_runs, kCenterBuckets, perm.as<int32_t>(),
+                            starts.as<int64_t>(), s));
+    }
     const int64_t *n_sorted_runs = starts.as<int64_t>() + kCenterBuckets;
     const int64_t *n_big_runs = starts.as<int64_t>() + (kCenterBuckets - kBigCenterBits + 1);
     hipLaunchKernelGGL(k_run_item_counts, dim3(gr), dim3(kBlock), 0, s,
@@ -1364,7 +1444,8 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                          (const int2 *)items.as<int2>(),
                          (const int64_t *)iptr.as<int64_t>(), n_huge, chunk, next_big, graph_of,
                          count);
-      const auto kfn = heavy_kernel<I, OthersPacked>(unroll);
+      const auto kfn = xcd_local ? heavy_kernel<I, OthersPacked, true>(unroll)
+                                 : heavy_kernel<I, OthersPacked>(unroll);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R,
                          col, (const unsigned long long *)row_stats,
                          (const int64_t *)run_ptr.as<int64_t>(),
@@ -1372,7 +1453,8 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                          (const int2 *)items.as<int2>(),
                          (const int64_t *)(iptr.as<int64_t>() + max_runs),
                          (const int64_t *)iptr.as<int64_t>(), n_huge, n_first, n_big_runs, chunk,
-                         next_item, next_blk, graph_of, count);
+                         next_item, next_blk, xcd_local ? xcd_ctr.as<unsigned long long>() : nullptr,
+                         graph_of, count);
     }
     TC_HIP(hipGetLastError());
   });
@@ -1383,7 +1465,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                      const int32_t *eu, const int32_t *ev, int64_t E, const uint8_t *skip,
                      const int32_t *graph_of, unsigned long long *count,
                      const tc_exact_params *params, hipStream_t s, const HybridRoute *hr,
-                     int center_sort_bits, int rel_start_bits) {
+                     const BatchPlan *bp) {
   const int light_max = (params && params->light_max > 0) ? params->light_max : 32;
   const int chunk = effective_chunk(params);
   const int wpc = (params && params->waves_per_cu > 0) ? params->waves_per_cu : 16;
@@ -1403,7 +1485,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                              light_max, chunk, wpc, unroll, blk_bits,
                              params && params->pack_ids != 0,
                              params && params->wide_offsets != 0,
-                             center_sort_bits, rel_start_bits, s, hr);
+                             bp ? *bp : BatchPlan{}, s, hr);
 
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;
@@ -1489,8 +1571,8 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                          OthersIds{(const int32_t *)others.as<int32_t>()},
                          (const int2 *)items.as<int2>(), (const int64_t *)(iptr.as<int64_t>() + n),
                          (const int64_t *)iptr.as<int64_t>(), n_huge, n_first, n_big_cent, chunk,
-                         ctr.as<unsigned long long>(), ctr.as<unsigned long long>() + 2, graph_of,
-                         count);
+                         ctr.as<unsigned long long>(), ctr.as<unsigned long long>() + 2, nullptr,
+                         graph_of, count);
     }
     TC_HIP(hipGetLastError());
   });

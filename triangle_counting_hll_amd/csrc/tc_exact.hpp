@@ -39,6 +39,24 @@ struct HybridRoute {
   RouterNet net;
 };
 
+// Planner choices for a batch of graphs (sorted plan; all 0 = the
+// single-graph form).  The batched entry points derive them from a device
+// check of the batch's layout (tc_batch.hip); each is valid only under the
+// conditions stated.
+struct BatchPlan {
+  // radix-sort only the low sort_bits bits of the center: the input's
+  // packed words never interleave two centers whose low bits agree (edges
+  // grouped by graph, every graph at most 2^sort_bits vertices)
+  int sort_bits = 0;
+  // row starts relative to the center's row start (PackLayout::rel_bits)
+  int rel_bits = 0;
+  // > 0: graph ids are below 2^graph_bits and no center row is longer than a
+  // wavefront table holds; the runs are then ordered by graph and the
+  // intersection kernel takes them from one queue per XCD (consecutive
+  // graphs stay in one L2)
+  int graph_bits = 0;
+};
+
 // count[0] (or count[graph_of[center]] when graph_of != nullptr) +=
 // sum over edges e with !(skip && skip[e]) of |N+(eu[e]) ∩ N+(ev[e])|.
 // eu == nullptr means the CSR's own COO (ev is then ignored).  With hr (sorted
@@ -48,8 +66,10 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                      const int32_t *eu, const int32_t *ev, int64_t E, const uint8_t *skip,
                      const int32_t *graph_of, unsigned long long *count,
                      const tc_exact_params *params, hipStream_t s,
-                     const HybridRoute *hr = nullptr, int center_sort_bits = 0,
-                     int rel_start_bits = 0);
+                     const HybridRoute *hr = nullptr, const BatchPlan *bp = nullptr);
+
+// Longest center row a wavefront's hash table holds.
+constexpr int kWaveTableMaxDeg = 1023;
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v);
 __global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v);
