@@ -196,7 +196,9 @@ __device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t *tota
 
 template <class T>
 __global__ void __launch_bounds__(kScanBlock)
-k_tile_sums(const T *__restrict__ in, int64_t n, int64_t *__restrict__ sums) {
+k_tile_sums(const T *__restrict__ in, int64_t n, const int64_t *__restrict__ n_dev,
+            int64_t *__restrict__ sums) {
+  if (n_dev) n = min(n, *n_dev);
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   int64_t s = 0;
 #pragma unroll
@@ -209,21 +211,24 @@ k_tile_sums(const T *__restrict__ in, int64_t n, int64_t *__restrict__ sums) {
 
 template <class T>
 __global__ void __launch_bounds__(kScanBlock)
-k_tile_scan(const T *__restrict__ in, int64_t n, const int64_t *__restrict__ offs,
-            int64_t *__restrict__ out) {
+k_tile_scan(const T *__restrict__ in, int64_t n, const int64_t *__restrict__ n_dev,
+            const int64_t *__restrict__ offs, int64_t *__restrict__ out) {
+  // with n_dev only the first m = min(n, *n_dev) inputs count; out[0..m] is
+  // written (out[m] = the total) and out[n] = the total
+  const int64_t m = n_dev ? min(n, *n_dev) : n;
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   int64_t vals[kScanItems];
   int64_t s = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    vals[i] = (base + i < n) ? (int64_t)in[base + i] : 0;
+    vals[i] = (base + i < m) ? (int64_t)in[base + i] : 0;
     s += vals[i];
   }
   int64_t tot;
   int64_t run = block_exclusive_scan(s, &tot) + (offs ? offs[blockIdx.x] : 0);
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    if (base + i < n) out[base + i] = run;
+    if (base + i <= m && base + i < n) out[base + i] = run;
     run += vals[i];
   }
   // out has n + 1 entries: the grand total goes to out[n].
@@ -243,7 +248,8 @@ size_t scan_scratch_bytes(int64_t n) {
 }
 
 template <class T>
-int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t * /*unused*/, hipStream_t s) {
+int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t * /*unused*/, hipStream_t s,
+                   const int64_t *n_dev) {
   if (n <= 0) {
     // a kernel, not a memset node, so that captured graphs replay it correctly
     hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(1), 0, s, out);
@@ -252,7 +258,7 @@ int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t * /*unused*/, h
   }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
   if (tiles == 1) {
-    hipLaunchKernelGGL(k_tile_scan<T>, dim3(1), dim3(kScanBlock), 0, s, in, n,
+    hipLaunchKernelGGL(k_tile_scan<T>, dim3(1), dim3(kScanBlock), 0, s, in, n, n_dev,
                        (const int64_t *)nullptr, out);
     TC_HIP(hipGetLastError());
     return TC_OK;
@@ -260,20 +266,24 @@ int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t * /*unused*/, h
   Scratch sums, offs;
   TC_TRY(sums.alloc(sizeof(int64_t) * tiles, s));
   TC_TRY(offs.alloc(sizeof(int64_t) * (tiles + 1), s));
-  hipLaunchKernelGGL(k_tile_sums<T>, dim3((unsigned)tiles), dim3(kScanBlock), 0, s, in, n,
+  hipLaunchKernelGGL(k_tile_sums<T>, dim3((unsigned)tiles), dim3(kScanBlock), 0, s, in, n, n_dev,
                      sums.as<int64_t>());
   TC_HIP(hipGetLastError());
   TC_TRY(exclusive_scan<int64_t>(sums.as<int64_t>(), offs.as<int64_t>(), tiles, nullptr, s));
-  hipLaunchKernelGGL(k_tile_scan<T>, dim3((unsigned)tiles), dim3(kScanBlock), 0, s, in, n,
+  hipLaunchKernelGGL(k_tile_scan<T>, dim3((unsigned)tiles), dim3(kScanBlock), 0, s, in, n, n_dev,
                      (const int64_t *)offs.as<int64_t>(), out);
   TC_HIP(hipGetLastError());
   return TC_OK;
 }
 
-template int exclusive_scan<int32_t>(const int32_t *, int64_t *, int64_t, int64_t *, hipStream_t);
-template int exclusive_scan<int64_t>(const int64_t *, int64_t *, int64_t, int64_t *, hipStream_t);
-template int exclusive_scan<uint8_t>(const uint8_t *, int64_t *, int64_t, int64_t *, hipStream_t);
-template int exclusive_scan<uint32_t>(const uint32_t *, int64_t *, int64_t, int64_t *, hipStream_t);
+template int exclusive_scan<int32_t>(const int32_t *, int64_t *, int64_t, int64_t *, hipStream_t,
+                                   const int64_t *);
+template int exclusive_scan<int64_t>(const int64_t *, int64_t *, int64_t, int64_t *, hipStream_t,
+                                   const int64_t *);
+template int exclusive_scan<uint8_t>(const uint8_t *, int64_t *, int64_t, int64_t *, hipStream_t,
+                                   const int64_t *);
+template int exclusive_scan<uint32_t>(const uint32_t *, int64_t *, int64_t, int64_t *, hipStream_t,
+                                   const int64_t *);
 
 __global__ void k_copy_i64(const int64_t *src, int64_t *dst) { *dst = *src; }
 

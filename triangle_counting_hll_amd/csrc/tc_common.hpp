@@ -217,10 +217,13 @@ template <class I> struct Rows {
 
 // ---------------------------------------------------------------------------
 // Device-wide exclusive scan (reduce-then-scan), int64 output.
-// out[i] = sum_{j<i} in[i]; *total (device, optional) = sum of all.
+// out[i] = sum_{j<i} in[i] for i <= n (out has n + 1 entries).  With n_dev
+// (device), only the first m = min(n, *n_dev) inputs are read: out[0..m] and
+// out[n] are written (both ends hold the total), the rest is left alone.
 // ---------------------------------------------------------------------------
 template <class T>
-int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t *d_total, hipStream_t s);
+int exclusive_scan(const T *in, int64_t *out, int64_t n, int64_t *d_total, hipStream_t s,
+                   const int64_t *n_dev = nullptr);
 
 // Stable bucket sort of 0..n-1 by a small key (key[i] < nbuckets <= 64, keys
 // equal to nbuckets or more are dropped).  perm receives the indices grouped
@@ -235,6 +238,7 @@ int bucket_sort_u8(const uint8_t *key, int64_t n, int nbuckets, int32_t *perm, i
 // (host pointer to a key value), the first pass leaves out every word equal to
 // *drop and the sorted output holds only the others; then n_dev (device) must
 // hold their number, which the later passes read (n sizes the launches).
+// With n_dev and no drop, every pass reads only the first *n_dev words.
 // With run_flags (uint8, n entries), the last pass also writes
 // run_flags[i] = 1 iff sorted word i starts a run of equal bits [bit_lo, 64)
 // (i == 0 or those bits differ from word i - 1), 0 otherwise, for every i

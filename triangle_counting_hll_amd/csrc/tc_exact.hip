@@ -1200,29 +1200,28 @@ __global__ void k_run_offsets(Rows<I> R, const uint64_t *__restrict__ s,
 }
 
 // Sort keys that order runs by graph: (graph of the run's center << 32 | run)
-// for r < n_runs, ~0 (left out by the sort) for the other slots.
+// for r < n_runs.
 __global__ void k_run_graph_keys(const uint64_t *__restrict__ s,
                                  const int64_t *__restrict__ run_ptr,
-                                 const int64_t *__restrict__ n_runs_p, int64_t max_runs, int shift,
+                                 const int64_t *__restrict__ n_runs_p, int shift,
                                  const int32_t *__restrict__ graph_of,
                                  uint64_t *__restrict__ keys) {
   const int64_t nr = *n_runs_p;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < max_runs;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nr;
        r += (int64_t)gridDim.x * blockDim.x)
-    keys[r] = r < nr ? ((uint64_t)(uint32_t)graph_of[s[run_ptr[r]] >> shift] << 32) | (uint64_t)r
-                     : ~0ull;
+    keys[r] = ((uint64_t)(uint32_t)graph_of[s[run_ptr[r]] >> shift] << 32) | (uint64_t)r;
 }
 
-// perm[i] = run of the i-th sorted key; the bucket boundaries put every run
-// in the wavefront phase (starts[0 .. kCenterBuckets - 1] = 0,
+// perm[i] = run of the i-th sorted key (i < n_runs); the bucket boundaries
+// put every run in the wavefront phase (starts[0 .. kCenterBuckets - 1] = 0,
 // starts[kCenterBuckets] = n_runs).
 __global__ void k_graph_perm(const uint64_t *__restrict__ sorted_keys,
-                             const int64_t *__restrict__ n_runs_p, int64_t max_runs,
-                             int32_t *__restrict__ perm, int64_t *__restrict__ starts) {
+                             const int64_t *__restrict__ n_runs_p, int32_t *__restrict__ perm,
+                             int64_t *__restrict__ starts) {
   const int64_t nr = *n_runs_p;
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (int64_t i = t0; i < max_runs; i += (int64_t)gridDim.x * blockDim.x)
-    perm[i] = i < nr ? (int32_t)(uint32_t)sorted_keys[i] : 0;
+  for (int64_t i = t0; i < nr; i += (int64_t)gridDim.x * blockDim.x)
+    perm[i] = (int32_t)(uint32_t)sorted_keys[i];
   if (t0 <= kCenterBuckets) starts[t0] = t0 == kCenterBuckets ? nr : 0;
 }
 
@@ -1230,15 +1229,12 @@ __global__ void k_run_item_counts(const int64_t *__restrict__ run_ptr,
                                   const int32_t *__restrict__ perm,
                                   const int64_t *__restrict__ n_runs_p, int64_t max_runs, int chunk,
                                   int32_t *__restrict__ nitems) {
-  const int64_t nr = *n_runs_p;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < max_runs;
+  // entries from n_runs on are not read (the scan is bounded by n_runs)
+  const int64_t nr = min(*n_runs_p, max_runs);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nr;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int k = 0;
-    if (i < nr) {
-      const int32_t r = perm[i];
-      k = (int)((run_ptr[r + 1] - run_ptr[r] + chunk - 1) / chunk);
-    }
-    nitems[i] = k;
+    const int32_t r = perm[i];
+    nitems[i] = (int)((run_ptr[r + 1] - run_ptr[r] + chunk - 1) / chunk);
   }
 }
 
@@ -1403,16 +1399,15 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
       TC_TRY(gk2.alloc(sizeof(uint64_t) * max_runs, s));
       hipLaunchKernelGGL(k_run_graph_keys, dim3(gr), dim3(kBlock), 0, s, (const uint64_t *)sorted,
                          (const int64_t *)run_ptr.as<int64_t>(),
-                         (const int64_t *)n_runs.as<int64_t>(), max_runs, shift, graph_of,
+                         (const int64_t *)n_runs.as<int64_t>(), shift, graph_of,
                          gk.as<uint64_t>());
       TC_HIP(hipGetLastError());
       uint64_t *gsorted = nullptr;
-      const uint64_t no_run = ~0ull;
       TC_TRY(radix_sort_u64(gk.as<uint64_t>(), gk2.as<uint64_t>(), max_runs, 32,
-                            32 + bp.graph_bits, &gsorted, s, &no_run,
+                            32 + bp.graph_bits, &gsorted, s, nullptr,
                             (const unsigned long long *)n_runs.as<int64_t>()));
       hipLaunchKernelGGL(k_graph_perm, dim3(gr), dim3(kBlock), 0, s, (const uint64_t *)gsorted,
-                         (const int64_t *)n_runs.as<int64_t>(), max_runs, perm.as<int32_t>(),
+                         (const int64_t *)n_runs.as<int64_t>(), perm.as<int32_t>(),
                          starts.as<int64_t>());
       TC_HIP(hipGetLastError());
     } else {
@@ -1426,7 +1421,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                        (const int64_t *)run_ptr.as<int64_t>(), (const int32_t *)perm.as<int32_t>(),
                        n_sorted_runs, max_runs, chunk, nitems.as<int32_t>());
     TC_TRY(exclusive_scan<int32_t>(nitems.as<int32_t>(), iptr.as<int64_t>(), max_runs, nullptr,
-                                   s));
+                                   s, n_sorted_runs));
     hipLaunchKernelGGL(k_item_fill, dim3(gr), dim3(kBlock), 0, s,
                        (const int32_t *)nitems.as<int32_t>(), (const int32_t *)perm.as<int32_t>(),
                        (const int64_t *)iptr.as<int64_t>(), n_sorted_runs, items.as<int2>());

@@ -46,8 +46,10 @@ constexpr int kMaxPasses = 8;
 constexpr int kHistCopies = 4;
 
 __global__ void __launch_bounds__(kRadixBlock)
-k_onesweep_hist(const uint64_t *__restrict__ keys, int64_t n, int drop, uint64_t drop_key,
+k_onesweep_hist(const uint64_t *__restrict__ keys, int64_t n,
+                const unsigned long long *__restrict__ n_dev, int drop, uint64_t drop_key,
                 int bit_lo, int bit_hi, unsigned long long *__restrict__ ghist) {
+  if (n_dev) n = min(n, (int64_t)*n_dev);
   __shared__ unsigned int hist[kMaxPasses * kRadixDigits * kHistCopies];
   const int passes = (bit_hi - bit_lo + kRadixBits - 1) / kRadixBits;
   for (int i = threadIdx.x; i < passes * kRadixDigits * kHistCopies; i += kRadixBlock)
@@ -327,8 +329,8 @@ int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
                      n_ctr, 0ull);
   const uint64_t drop_key = drop ? *drop : 0;
   hipLaunchKernelGGL(k_onesweep_hist, dim3((unsigned)std::min<int64_t>(ntiles, num_cus() * 4)),
-                     dim3(kRadixBlock), 0, s, (const uint64_t *)a, n, drop ? 1 : 0, drop_key,
-                     bit_lo, bit_hi, ghist);
+                     dim3(kRadixBlock), 0, s, (const uint64_t *)a, n, drop ? nullptr : n_dev,
+                     drop ? 1 : 0, drop_key, bit_lo, bit_hi, ghist);
   hipLaunchKernelGGL(k_onesweep_bases, dim3(1), dim3(kWave), 0, s,
                      (const unsigned long long *)ghist, passes, bases.as<int64_t>());
   TC_HIP(hipGetLastError());
@@ -337,10 +339,11 @@ int radix_sort_u64(uint64_t *a, uint64_t *b, int64_t n, int bit_lo, int bit_hi,
     const int shift = bit_lo + p * kRadixBits;
     // digits never reach past bit_hi, so higher bits do not perturb the order
     const int dmask = (1 << std::min(kRadixBits, bit_hi - shift)) - 1;
-    // the first pass reads all n words (dropping *drop); later ones the kept count
+    // with drop, the first pass reads all n words (dropping *drop) and later
+    // ones the kept count; without, every pass reads *n_dev words
     const bool first = p == 0;
     const int drop_now = (first && drop) ? 1 : 0;
-    const unsigned long long *nd = first ? nullptr : n_dev;
+    const unsigned long long *nd = (first && drop) ? nullptr : n_dev;
     const bool last = p == passes - 1;
     hipLaunchKernelGGL(k_set_u64, dim3(grid_for(ntiles * kRadixDigits, kBlock, 4096)),
                        dim3(kBlock), 0, s, status.as<unsigned long long>(),
