@@ -190,7 +190,8 @@ def copy_bandwidth_gbs(gib: int = 4, reps: int = 5):
 
 
 TRAFFIC_PROFILES = {"chunglu": "profiles/exact_chunglu117M_hbm_traffic.md",
-                    "rmat23": "profiles/exact_rmat23_hbm_traffic.md"}
+                    "rmat23": "profiles/exact_rmat23_hbm_traffic.md",
+                    "batch4096": "profiles/batch4096_hbm_traffic.md"}
 
 
 def measured_traffic(workload: str):
@@ -511,8 +512,9 @@ def run_batch_extra(args, reps=10):
     g = batch.BatchCountGraph(b, params)
     _, ms = _timed(g.launch, reps)
     counts = g.counts.cpu().numpy()
-    plain, plain_s = batch.plain_per_graph_counts(b, params)
-    hyb, hyb_s, hyb_build_s = batch.plain_per_graph_hybrid(b, gamma=1024, p=12)
+    graphs = batch.per_graph_csrs(b)
+    plain, plain_s, plain_dev_ms = batch.plain_per_graph_counts(b, params, graphs)
+    hyb, hyb_s, hyb_build_s = batch.plain_per_graph_hybrid(b, gamma=1024, p=12, graphs=graphs)
     # CPU oracle per graph on the same packed arrays
     rp = b.csr.row_ptr.cpu().numpy().astype(np.int64)
     col = b.csr.col_idx.cpu().numpy()
@@ -528,8 +530,14 @@ def run_batch_extra(args, reps=10):
         "raw_pairs": int(spec.pair_off[-1]), "build_s": build_s,
         "ms_per_launch": ms, "ms_per_graph": ms / b.n_graphs,
         "oriented_edges_per_s": E / (ms * 1e-3),
+        "plain": "one gpu_exact_triangle_count call per graph on that graph's own CSR "
+                 "(ids from 0), synchronised per call",
         "plain_ms_total": plain_s * 1e3, "plain_ms_per_graph": plain_s * 1e3 / b.n_graphs,
         "speedup_vs_plain": plain_s * 1e3 / ms,
+        "plain_device_ms_total": plain_dev_ms,
+        "plain_device_note": "sum over graphs of the device time between HIP events around "
+                             "each per-graph call (its kernels and the gaps between them)",
+        "speedup_vs_plain_device_time": plain_dev_ms / ms,
         "cpu_oracle_s": cpu_s, "cpu_threads": len(os.sched_getaffinity(0)),
         "plain_hybrid": {
             "rule": "per-graph gpu_hybrid_triangle_count, p = 12, gamma = theta = 1024, "
@@ -719,6 +727,7 @@ def run_batch_workload(args, info):
                         for i in range(b.n_graphs)], dtype=np.int64)
         parity = bool(np.array_equal(counts, cpu))
     achieved = floor["bytes"] / (heavy_avg * 1e-3) / 1e9 if heavy_n else None
+    traffic, l2_hit = measured_traffic("batch4096")
     line = {
         "metric": "oriented edges/sec (batched exact count, 4,096 graphs in one captured launch)",
         "value": E / (ms * 1e-3), "unit": "oriented edges/s", "n_gpus": 1,
@@ -734,7 +743,13 @@ def run_batch_workload(args, info):
         "eager_equals_captured": bool(np.array_equal(eager.cpu().numpy(), counts)),
         "roofline": {"bound": "hbm", "kernel": "k_heavy + k_heavy_big (eager run)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                     "traffic_source": TRAFFIC_PROFILES["batch4096"] + " (rocprofv3 --pmc "
+                                       "FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy's "
+                                       "counts)" if traffic else None,
+                     "traffic_gbs": traffic / (heavy_avg * 1e-3) / 1e9
+                                    if traffic and heavy_avg else None,
+                     "l2_hit_rate": l2_hit,
                      "algorithmic_bytes_per_launch": floor["bytes"],
                      "algorithmic_bytes_terms": floor, "avg_launch_ms": heavy_avg},
         "stage_ms_eager": {k: v[0] / max(v[1], 1) for k, v in stages.items() if v[1]},

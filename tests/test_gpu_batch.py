@@ -47,8 +47,15 @@ def test_batch_counts_bit_exact(device, plan):
     b = batch.build_batch(spec)
     got = batch.batch_exact_count(b, {"plan": plan}).cpu().numpy()
     assert np.array_equal(got, truth)
-    plain, secs = batch.plain_per_graph_counts(b)
-    assert np.array_equal(plain, truth) and secs > 0
+    plain, secs, dev_ms = batch.plain_per_graph_counts(b)
+    assert np.array_equal(plain, truth) and secs > 0 and dev_ms > 0
+    # the per-graph CSRs hold each graph rebased to ids from 0
+    graphs = batch.per_graph_csrs(b)
+    assert len(graphs) == b.n_graphs
+    for g in (0, b.n_graphs // 2, b.n_graphs - 1):
+        rp, col, *_ = O.csr_from_pairs(pairs[spec.pair_off[g]:spec.pair_off[g + 1]])
+        assert np.array_equal(graphs[g].row_ptr.cpu().numpy(), rp)
+        assert np.array_equal(graphs[g].col_idx.cpu().numpy(), col)
 
 
 @pytest.mark.parametrize("plan", ["sorted", "atomic"])

@@ -223,37 +223,74 @@ class BatchCountGraph:
             pass
 
 
-def plain_per_graph_counts(b: BatchedCSR, params: Optional[dict] = None):
-    """The plain form: one ``gpu_exact_triangle_count`` call per graph.
+@dataclass
+class GraphCSR:
+    """One graph of a batch as its own forward CSR (dense ids from 0)."""
 
-    Returns ``(counts int64[G], seconds)``; each call synchronises, as the
-    reference launcher does (src/gpu_hybrid.py:346-354).
+    row_ptr: object   # int32 [n_g + 1] device
+    col_idx: object   # int32 [E_g] device
+    edge_u: object
+    edge_v: object
+    v0: int           # first vertex of the graph in the packed CSR
+
+
+def per_graph_csrs(b: BatchedCSR) -> list:
+    """Every graph of the batch as the CSR a one-graph-per-run caller holds
+    (the reference counts one graph per process, experiment/exp_gpu.py:632-935):
+    its rows and ids rebased to start at 0.  Device slices, built once."""
+    t = _lib.require_device()
+    rp = b.csr.row_ptr.to(t.int64)
+    out = []
+    for g in range(b.n_graphs):
+        v0, v1 = int(b.vert_off[g]), int(b.vert_off[g + 1])
+        e0, e1 = int(b.edge_off[g]), int(b.edge_off[g + 1])
+        out.append(GraphCSR((rp[v0:v1 + 1] - e0).to(t.int32), b.csr.col_idx[e0:e1] - v0,
+                            b.csr.edge_u[e0:e1] - v0, b.csr.edge_v[e0:e1] - v0, v0))
+    return out
+
+
+def plain_per_graph_counts(b: BatchedCSR, params: Optional[dict] = None, graphs=None):
+    """The plain form: one ``gpu_exact_triangle_count`` call per graph, each on
+    that graph's own CSR (``per_graph_csrs``, built before timing).
+
+    Returns ``(counts int64[G], seconds, device_ms)``: host wall time of the
+    loop (each call synchronises, as the reference launcher does,
+    src/gpu_hybrid.py:346-354) and the sum over graphs of the device time
+    between HIP events recorded around each call on the stream.
     """
     t = _lib.require_device()
-    eu, ev = b.csr.edge_u, b.csr.edge_v
+    graphs = graphs if graphs is not None else per_graph_csrs(b)
     out = np.zeros(b.n_graphs, dtype=np.int64)
+    ev = [(t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True))
+          for _ in range(b.n_graphs)]
     t.cuda.synchronize()
     t0 = time.perf_counter()
-    for g in range(b.n_graphs):
-        lo, hi = int(b.edge_off[g]), int(b.edge_off[g + 1])
-        if hi > lo:
-            out[g], _ = gpu_hybrid.gpu_exact_triangle_count(b.csr.row_ptr, b.csr.col_idx,
-                                                            eu[lo:hi], ev[lo:hi], params=params)
-    return out, time.perf_counter() - t0
+    for g, c in enumerate(graphs):
+        if c.col_idx.numel():
+            ev[g][0].record()
+            out[g], _ = gpu_hybrid.gpu_exact_triangle_count(c.row_ptr, c.col_idx, c.edge_u,
+                                                            c.edge_v, params=params)
+            ev[g][1].record()
+    secs = time.perf_counter() - t0
+    t.cuda.synchronize()
+    dev_ms = sum(a.elapsed_time(z) for (a, z), c in zip(ev, graphs) if c.col_idx.numel())
+    return out, secs, dev_ms
 
 
-def plain_per_graph_hybrid(b: BatchedCSR, gamma: int = 1024, p: int = 12):
+def plain_per_graph_hybrid(b: BatchedCSR, gamma: int = 1024, p: int = 12, graphs=None):
     """The plain fixed-threshold hybrid, one graph at a time.
 
     One sketch build over the batch (vertices with forward degree >= gamma,
     compact register slots; the reference experiment passes theta = gamma,
     experiment/exp_gpu.py:556-557), then one ``gpu_hybrid_triangle_count`` call
-    per graph (src/gpu_hybrid.py:357-431: split exact and HLL kernels,
-    synchronised).  Returns ``(estimates float64[G], seconds, build_seconds)``.
+    per graph on that graph's own CSR (src/gpu_hybrid.py:357-431: split exact
+    and HLL kernels, synchronised).  Returns ``(estimates float64[G], seconds,
+    build_seconds)``.
     """
     from . import gpu_hll
 
     t = _lib.require_device()
+    graphs = graphs if graphs is not None else per_graph_csrs(b)
     deg = b.csr.degrees
     t.cuda.synchronize()
     t0 = time.perf_counter()
@@ -267,13 +304,12 @@ def plain_per_graph_hybrid(b: BatchedCSR, gamma: int = 1024, p: int = 12):
                            slot=slot, accumulate=False)
     t.cuda.synchronize()
     build_s = time.perf_counter() - t0
-    eu, ev = b.csr.edge_u, b.csr.edge_v
     out = np.zeros(b.n_graphs, dtype=np.float64)
     t0 = time.perf_counter()
-    for g in range(b.n_graphs):
-        lo, hi = int(b.edge_off[g]), int(b.edge_off[g + 1])
-        if hi > lo:
+    for g, c in enumerate(graphs):
+        if c.col_idx.numel():
+            v0, v1 = int(b.vert_off[g]), int(b.vert_off[g + 1])
             out[g], _, _ = gpu_hybrid.gpu_hybrid_triangle_count(
-                b.csr.row_ptr, b.csr.col_idx, eu[lo:hi], ev[lo:hi], deg, regs, has, gamma, p,
-                slot=slot)
+                c.row_ptr, c.col_idx, c.edge_u, c.edge_v, deg[v0:v1], regs, has[v0:v1], gamma,
+                p, slot=slot[v0:v1])
     return out, build_s + time.perf_counter() - t0, build_s
