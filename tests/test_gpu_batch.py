@@ -218,3 +218,49 @@ def test_batch4096_full_size(device):
         assert np.array_equal(got.cpu().numpy(), truth), f"replay {i}"
         assert int(x.item()) == 499500
     g.close()
+
+
+def test_batch_plan_falls_back_on_other_layouts(device):
+    # The planner narrows its sort and packed words and orders work by graph
+    # only when the batch's layout allows it (graph ids ascending along the
+    # vertices, no edge joining two graphs, edges grouped by graph); any other
+    # layout takes the single-graph form, and the counts stay exact.
+    import torch
+
+    spec = batch.many_graphs_spec(64, seed=7, scale_down=4)
+    pairs = batch.generate_pairs_numpy(spec)
+    truth = _per_graph_truth(spec, pairs)
+    b = batch.build_batch(spec)
+    rp, is64 = _lib.row_ptr_arg(b.csr.row_ptr)
+
+    def run(bb, graph_of, G):
+        counts = torch.zeros(G, dtype=torch.int64, device="cuda")
+        _lib.check(_lib.load().tc_batch_exact_count(
+            _lib.ptr(_lib.row_ptr_arg(bb.csr.row_ptr)[0]), is64, _lib.ptr(bb.csr.col_idx),
+            bb.csr.n_nodes, _lib.ptr(bb.csr.edge_u), _lib.ptr(bb.csr.edge_v), bb.csr.n_oriented,
+            _lib.ptr(graph_of), G, _lib.ptr(counts), None, _lib.stream_ptr()))
+        return counts.cpu().numpy()
+
+    assert np.array_equal(run(b, b.graph_of, b.n_graphs), truth)
+    # graph ids descending along the vertices
+    rev = (b.n_graphs - 1 - b.graph_of).to(torch.int32).contiguous()
+    assert np.array_equal(run(b, rev, b.n_graphs)[::-1], truth)
+    # the whole batch as one graph
+    one = torch.zeros_like(b.graph_of)
+    assert run(b, one, 1)[0] == truth.sum()
+    # edges that join graphs: the total is the whole graph's count, and the
+    # graphs no such edge touches keep their own counts
+    rng = np.random.default_rng(3)
+    g1 = rng.choice(b.n_graphs, 6, replace=False)
+    g2 = (g1 + 1) % b.n_graphs
+    off = spec.id_off
+    cross = np.stack([off[g1] + rng.integers(0, spec.n_ids[g1]),
+                      off[g2] + rng.integers(0, spec.n_ids[g2])], axis=1)
+    both = np.concatenate([pairs, cross.astype(np.int32)])
+    bx = batch.build_batch(spec, pairs=torch.from_numpy(both).cuda())
+    got = run(bx, bx.graph_of, bx.n_graphs)
+    rpx = bx.csr.row_ptr.cpu().numpy().astype(np.int64)
+    colx = bx.csr.col_idx.cpu().numpy()
+    assert got.sum() == cbind.exact_count(rpx, colx)
+    untouched = np.setdiff1d(np.arange(b.n_graphs), np.concatenate([g1, g2]))
+    assert np.array_equal(got[untouched], truth[untouched])
