@@ -596,12 +596,17 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
       if constexpr (kMode == kProbeBitmap) {
         // offsets outside the row's range (and the -2 padding) clamp to the
         // clear sentinel bit: one LDS word read and one bit extract per id
-        const uint32_t *__restrict__ words = reinterpret_cast<const uint32_t *>(tbl);
+        // the word's LDS byte address is base + (off >> 5) * 4, one shift and
+        // one shift-add (the compiler's own form takes a shift, a mask and an
+        // add); v_bfe_u32 takes the bit position from off's low 5 bits
+        typedef __attribute__((address_space(3))) const uint32_t lds_u32;
+        const uint32_t base = (uint32_t)(uintptr_t)(lds_u32 *)reinterpret_cast<const uint32_t *>(tbl);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const uint32_t off = min((uint32_t)(xs[u] - C.lo), C.sentinel);
-          // v_bfe_u32 takes the bit position from the offset's low 5 bits
-          acc += __builtin_amdgcn_ubfe(words[off >> 5], off, 1u);
+          uint32_t addr;
+          asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(addr) : "v"(off >> 5), "v"(base));
+          acc += __builtin_amdgcn_ubfe(*(lds_u32 *)(uintptr_t)addr, off, 1u);
         }
       } else if constexpr (kMode == kProbeHash) {
         // all U home buckets are read before the first compare (one LDS
