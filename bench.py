@@ -686,7 +686,7 @@ def run_router_extra(reps=3, p=12, theta=256, rel_tol=0.02, budget=0.005, gates=
 
 def run_batch_workload(args, info):
     """The many-graph workload as the measured step: one replay of the captured
-    launch that counts all 4,096 graphs (library-default knobs); prints one
+    launch that counts all 4,096 graphs (the command line's knobs); prints one
     JSON line.  The roofline is that of the hash-probe kernels inside it, timed
     with HIP events in an eager run of the same pipeline (events cannot sit
     inside a captured graph)."""
@@ -699,7 +699,9 @@ def run_batch_workload(args, info):
     b = batch.build_batch(spec)
     torch.cuda.synchronize()
     E = b.csr.n_oriented
-    g = batch.BatchCountGraph(b)
+    params = {"light_max": args.light_max, "chunk": args.chunk,
+              "waves_per_cu": args.waves_per_cu, "unroll": args.unroll}
+    g = batch.BatchCountGraph(b, params)
     for _ in range(args.warmup):
         g.launch()
     torch.cuda.synchronize()
@@ -711,13 +713,13 @@ def run_batch_workload(args, info):
     counts = g.counts.cpu().numpy()
     _lib.timing_enable(True)
     _lib.timing_read()
-    eager = batch.batch_exact_count(b)
+    eager = batch.batch_exact_count(b, params)
     stages = _lib.timing_read()
     _lib.timing_enable(False)
     rp = b.csr.row_ptr.cpu().numpy().astype(np.int64)
     col = b.csr.col_idx.cpu().numpy()
     eu = b.csr.edge_u.cpu().numpy()
-    floor = heavy_floor_bytes(rp, eu, col, 32, 512)
+    floor = heavy_floor_bytes(rp, eu, col, args.light_max, args.chunk)
     heavy_ms, heavy_n = stages["heavy"]
     heavy_avg = heavy_ms / max(heavy_n, 1)
     parity = None
@@ -738,7 +740,7 @@ def run_batch_workload(args, info):
         "config": {"workload": "batch4096 (4,096 graphs, seed 3: half ER n in [200, 50000] "
                                "p = 8/n, half R-MAT scale 8..16 edge factor 16)",
                    "n_nodes": b.csr.n_nodes, "oriented_edges": E,
-                   "triangles": int(counts.sum())},
+                   "triangles": int(counts.sum()), **params},
         "parity_vs_cpu_oracle": parity,
         "eager_equals_captured": bool(np.array_equal(eager.cpu().numpy(), counts)),
         "roofline": {"bound": "hbm", "kernel": "k_heavy + k_heavy_big (eager run)",
@@ -778,6 +780,8 @@ def main():
                          "degree_stats) or 'all'")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the hybrid and many-graph measurements added to the line")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="count with the knob flags as given instead of tuning them first")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (reference point for "
                          "HBM byte counters under a profiler)")
@@ -826,6 +830,28 @@ def main():
     params = {"light_max": args.light_max, "chunk": args.chunk,
               "waves_per_cu": args.waves_per_cu, "unroll": args.unroll}
     count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    tune_evals = None
+    if not args.no_autotune:
+        # the merge-versus-probe crossover, the work-item tile and the
+        # workgroup-phase bound chosen by the project's autotuner on this
+        # graph (coordinate descent from the flags' values, each candidate
+        # the median of 5 timed counts), before the timed region
+        from triangle_counting_hll_amd import autotune
+
+        def run(p):
+            count.zero_()
+            gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v, count, p)
+
+        space = {"light_max": (24, 32, 36, 40, 48), "chunk": (256, 512, 1024),
+                 "block_min_bits": (9, 10, 11)}
+        t0 = time.perf_counter()
+        tuned, table = autotune.tune(run, f"exact|{args.workload}|n={csr.n_nodes}|E={E}",
+                                     space=space, start=dict(params, block_min_bits=10,
+                                                             plan="sorted"))
+        params = {k: tuned[k] for k in ("light_max", "chunk", "waves_per_cu", "unroll",
+                                        "block_min_bits")}
+        tune_evals = len(table)
+        _log(f"autotuned {params} in {time.perf_counter() - t0:.1f}s ({tune_evals} evals)")
     for _ in range(args.warmup):
         count.zero_()
         gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v, count, params)
@@ -850,8 +876,8 @@ def main():
     rp = csr.row_ptr.cpu().numpy().astype(np.int64)
     col = csr.col_idx.cpu().numpy()
     eu = csr.edge_u.cpu().numpy()
-    B_all, B_merge_heavy, _ = merge_model_bytes(rp, col, eu, col, args.light_max)
-    floor = heavy_floor_bytes(rp, eu, col, args.light_max, args.chunk)
+    B_all, B_merge_heavy, _ = merge_model_bytes(rp, col, eu, col, params["light_max"])
+    floor = heavy_floor_bytes(rp, eu, col, params["light_max"], params["chunk"])
     heavy_ms, heavy_n = stages["heavy"]
     heavy_avg_ms = heavy_ms / max(heavy_n, 1)
     achieved = floor["bytes"] / (heavy_avg_ms * 1e-3) / 1e9 if heavy_n else None
@@ -909,8 +935,10 @@ def main():
         "config": {"workload": args.workload if not args.scale_down
                    else f"{args.workload}/2^{args.scale_down}",
                    "n_nodes": csr.n_nodes, "oriented_edges": E, "triangles": gpu_count,
-                   "row_ptr": "int64", "light_max": args.light_max, "chunk": args.chunk,
-                   "waves_per_cu": args.waves_per_cu, "unroll": args.unroll},
+                   "row_ptr": "int64", **params,
+                   "knobs": ("autotuned on this graph before timing "
+                             f"({tune_evals} candidates timed)" if tune_evals is not None
+                             else "as given on the command line")},
         "parity_vs_cpu_oracle": parity,
         "roofline": {
             "bound": "hbm",

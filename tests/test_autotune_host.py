@@ -81,3 +81,21 @@ def test_cache_can_be_bypassed(monkeypatch):
     assert len(t1) > 0 and len(t2) == len(t1)
     autotune.clear_memory()
 
+
+
+def test_tune_starts_from_the_given_knobs():
+    # the first candidate timed is the starting point with the given knobs
+    # over the library defaults; a custom space limits the search to its knobs
+    autotune.clear_memory()
+    seen = []
+
+    def timer(p):
+        seen.append(dict(p))
+        return 10.0 + (0.0 if p["light_max"] == 36 else 1.0)
+
+    space = {"light_max": (24, 32, 36, 40)}
+    got, table = autotune.tune(None, "wl-start", space=space, timer=timer, device="dev0",
+                               use_cache=False, start={"light_max": 40, "chunk": 256})
+    assert seen[0] == dict(autotune.DEFAULTS, light_max=40, chunk=256)
+    assert got["light_max"] == 36 and got["chunk"] == 256
+    assert {p["chunk"] for p, _ in table} == {256}

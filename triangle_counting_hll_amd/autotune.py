@@ -110,11 +110,12 @@ def tune(run: Callable[[dict], None], key: str, space: Optional[dict] = None,
          reps: int = MIN_REPS, use_cache: bool = True,
          log: Optional[Callable[[str], None]] = None,
          timer: Optional[Callable[[dict], float]] = None, device: Optional[str] = None,
-         margin: float = 0.01):
+         margin: float = 0.01, start: Optional[dict] = None):
     """Best knob setting for ``run`` (coordinate descent); returns ``(params, table)``.
 
     ``table`` lists every (params, ms) pair that was timed (empty on a cache
-    hit).  ``timer(params) -> ms`` replaces the device timing of ``run``
+    hit).  ``start`` overrides knobs of the starting point (the library
+    defaults).  ``timer(params) -> ms`` replaces the device timing of ``run``
     (``time_run``) and ``device`` the device name in the cache key; both exist
     so the search and the cache can be exercised without a GPU.
     """
@@ -138,6 +139,6 @@ def tune(run: Callable[[dict], None], key: str, space: Optional[dict] = None,
                 log(f"autotune {p} -> {seen[k]:.3f} ms")
         return seen[k]
 
-    best, _ = descend(measure, DEFAULTS, space, margin=margin)
+    best, _ = descend(measure, dict(DEFAULTS, **(start or {})), space, margin=margin)
     _store(full_key, best)
     return best, table
