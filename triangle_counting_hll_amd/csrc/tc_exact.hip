@@ -510,6 +510,11 @@ struct IdOffset {
   __device__ __forceinline__ static int32_t load(const int32_t *__restrict__ col, T off, int t) {
     return col[off + t];
   }
+  __device__ __forceinline__ static int4 load4(const int32_t *__restrict__ col, T off, int t) {
+    int4 v;
+    __builtin_memcpy(&v, col + off + t, sizeof(v));
+    return v;
+  }
 };
 template <>
 struct IdOffset<true> {
@@ -517,6 +522,14 @@ struct IdOffset<true> {
   __device__ __forceinline__ static int32_t load(const int32_t *__restrict__ col, T off, int t) {
     const uint32_t b = (off + (uint32_t)t) << 2;
     return *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(col) + b);
+  }
+  // ids off + t .. off + t + 3 with one 16-byte load (4-byte aligned)
+  __device__ __forceinline__ static int4 load4(const int32_t *__restrict__ col, T off, int t) {
+    const uint32_t b = (off + (uint32_t)t) << 2;
+    const int32_t *p = reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(col) + b);
+    int4 v;
+    __builtin_memcpy(&v, p, sizeof(v));
+    return v;
   }
 };
 
@@ -533,18 +546,73 @@ struct CenterSet {
   int dc;
 };
 
-// Streams the rows of others[ob:oe] as one flattened sequence, U ids
-// per lane in flight, and counts the ids found in the center's set
-// (kMode: kProbeGlobal, kProbeHash or kProbeBitmap).
+// Counts the ids of xs[0:K] found in the center's set (kMode: kProbeGlobal,
+// kProbeHash or kProbeBitmap).  Padding slots hold -2, which matches no id,
+// no empty hash slot and no bit of a bitmap.
+template <int kMode, int K>
+__device__ __forceinline__ uint32_t probe_ids(const int (&xs)[K], const int32_t *__restrict__ col,
+                                              const CenterSet &C) {
+  uint32_t acc = 0;
+  if constexpr (kMode == kProbeBitmap) {
+    // offsets outside the row's range (and the -2 padding) clamp to the
+    // clear sentinel bit: one LDS word read and one bit extract per id;
+    // the word's LDS byte address is base + (off >> 5) * 4, one shift and
+    // one shift-add (the compiler's own form takes a shift, a mask and an
+    // add); v_bfe_u32 takes the bit position from off's low 5 bits
+    typedef __attribute__((address_space(3))) const uint32_t lds_u32;
+    const uint32_t base =
+        (uint32_t)(uintptr_t)(lds_u32 *)reinterpret_cast<const uint32_t *>(C.tbl);
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      const uint32_t off = min((uint32_t)(xs[u] - C.lo), C.sentinel);
+      uint32_t addr;
+      asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(addr) : "v"(off >> 5), "v"(base));
+      acc += __builtin_amdgcn_ubfe(*(lds_u32 *)(uintptr_t)addr, off, 1u);
+    }
+  } else if constexpr (kMode == kProbeHash) {
+    // all K home buckets are read before the first compare (one LDS
+    // wait), then the few lanes whose home bucket is full without the
+    // id continue along the probe sequence; groups of up to four reads
+    // keep the register footprint small
+    const int4 *__restrict__ tbl = C.tbl;
+    constexpr int G = K < 4 ? K : 4;
+    uint32_t more = 0;  // bit u: the home bucket of xs[u] is full and lacks it
+#pragma unroll
+    for (int g = 0; g < K; g += G) {
+      int4 vv[G];
+#pragma unroll
+      for (int j = 0; j < G; ++j) vv[j] = tbl[bucket_of((uint32_t)xs[g + j], C.shift)];
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const uint32_t miss = bucket_miss(vv[j], xs[g + j]);
+        acc += miss == 0 ? 1u : 0u;
+        more |= ((miss != 0) & (vv[j].w != -1)) ? (1u << (g + j)) : 0u;
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < K; ++u)
+        if (more & (1u << u))
+          acc += table_probe_from(tbl, xs[u], (bucket_of((uint32_t)xs[u], C.shift) + 1) & C.mask,
+                                  C.mask);
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (xs[u] >= 0) acc += global_probe(col, C.cb, C.dc, xs[u]);
+  }
+  return acc;
+}
+
+// The rows of others[ob:oe] for unroll 1 and 2: one flattened sequence of
+// ids, U per lane in flight, each lane following the row boundaries of its
+// positions through seg_end / seg_base.
 template <class I, int kMode, int U, class O, bool kNarrow>
-__device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__restrict__ col,
-                                                  O others, int64_t ob,
-                                                  int64_t oe, int *__restrict__ seg_end,
-                                                  int64_t *__restrict__ seg_base,
-                                                  const CenterSet &C) {
-  const int4 *__restrict__ tbl = C.tbl;
-  const int shift = C.shift;
-  const uint32_t mask = C.mask;
+__device__ __forceinline__ uint64_t stream_others_ids(Rows<I> R, const int32_t *__restrict__ col,
+                                                      O others, int64_t ob, int64_t oe,
+                                                      int *__restrict__ seg_end,
+                                                      int64_t *__restrict__ seg_base,
+                                                      const CenterSet &C) {
   const int lane = lane_id();
   uint32_t acc = 0;  // at most chunk * kBigMaxDeg hits per call
   for (int64_t base = ob; base < oe; base += kWave) {
@@ -557,9 +625,6 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
     seg_end[lane] = inc;
     seg_base[lane] = st - (int64_t)(inc - len);
     wave_lds_sync();
-    // current row of this lane: its end in the flattened sequence and the
-    // offset that maps a flattened position to col[]; LDS is read only when
-    // the lane crosses into the next row
     using Off = IdOffset<kNarrow>;
     int seg = 0;
     int seg_hi = seg_end[0];
@@ -567,10 +632,11 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
     for (int w0 = 0; w0 < total; w0 += kWave * U) {
       const int t0 = w0 + lane;
       int xs[U];
-      if (w0 + kWave * U <= total) {  // uniform: every position of the step exists
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int t = t0 + u * kWave;
+      for (int u = 0; u < U; ++u) {
+        const int t = t0 + u * kWave;
+        xs[u] = -2;  // past the end
+        if (t < total) {
           while (seg_hi <= t) {
             ++seg;
             seg_hi = seg_end[seg];
@@ -578,73 +644,115 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
           }
           xs[u] = Off::load(col, seg_off, t);
         }
-      } else {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int t = t0 + u * kWave;
-          xs[u] = -2;  // past the end: matches no id and no empty slot
-          if (t < total) {
-            while (seg_hi <= t) {
-              ++seg;
-              seg_hi = seg_end[seg];
-              seg_off = (typename Off::T)seg_base[seg];
-            }
-            xs[u] = Off::load(col, seg_off, t);
-          }
-        }
       }
-      if constexpr (kMode == kProbeBitmap) {
-        // offsets outside the row's range (and the -2 padding) clamp to the
-        // clear sentinel bit: one LDS word read and one bit extract per id
-        // the word's LDS byte address is base + (off >> 5) * 4, one shift and
-        // one shift-add (the compiler's own form takes a shift, a mask and an
-        // add); v_bfe_u32 takes the bit position from off's low 5 bits
-        typedef __attribute__((address_space(3))) const uint32_t lds_u32;
-        const uint32_t base = (uint32_t)(uintptr_t)(lds_u32 *)reinterpret_cast<const uint32_t *>(tbl);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const uint32_t off = min((uint32_t)(xs[u] - C.lo), C.sentinel);
-          uint32_t addr;
-          asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(addr) : "v"(off >> 5), "v"(base));
-          acc += __builtin_amdgcn_ubfe(*(lds_u32 *)(uintptr_t)addr, off, 1u);
-        }
-      } else if constexpr (kMode == kProbeHash) {
-        // all U home buckets are read before the first compare (one LDS
-        // wait), then the few lanes whose home bucket is full without the
-        // id continue along the probe sequence
-        // groups of up to four reads keep the register footprint small
-        constexpr int G = U < 4 ? U : 4;
-        uint32_t more = 0;  // bit u: the home bucket of xs[u] is full and lacks it
-#pragma unroll
-        for (int g = 0; g < U; g += G) {
-          int4 vv[G];
-#pragma unroll
-          for (int j = 0; j < G; ++j) {
-            vv[j] = tbl[bucket_of((uint32_t)xs[g + j], shift)];
-          }
-#pragma unroll
-          for (int j = 0; j < G; ++j) {
-            const uint32_t miss = bucket_miss(vv[j], xs[g + j]);
-            acc += miss == 0 ? 1u : 0u;
-            more |= ((miss != 0) & (vv[j].w != -1)) ? (1u << (g + j)) : 0u;
-          }
-        }
-        if (more) {
-#pragma unroll
-          for (int u = 0; u < U; ++u)
-            if (more & (1u << u))
-              acc += table_probe_from(tbl, xs[u], (bucket_of((uint32_t)xs[u], shift) + 1) & mask,
-                                      mask);
-        }
-      } else {
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (xs[u] >= 0) acc += global_probe(col, C.cb, C.dc, xs[u]);
-      }
+      acc += probe_ids<kMode, U>(xs, col, C);
     }
     wave_lds_sync();
   }
   return acc;
+}
+
+// The rows of others[ob:oe] for unroll 4, 8 and 16.  Each lane takes one
+// row of a batch of 64.  A row's whole 4-id units are numbered across the
+// batch (rows without one left out), and step by step the wavefront takes
+// the next 64 * U / 4 units: lane l loads units W + 64 i + l (i < U / 4),
+// each with one 16-byte load, so every load instruction reads 1 KiB of
+// consecutive units.  A lane finds the row of its unit without walking the
+// row ends: at each step every row whose last unit falls in the step's
+// window sets its bit in U / 4 LDS words; the row of unit W + 64 i + l is
+// the number of rows that ended before W, plus the bits of the words before
+// word i, plus the bits of word i below lane l (mbcnt).  The last len % 4
+// ids of each row are read by that row's own lane with single-id loads, so
+// no load passes the end of a row.
+template <class I, int kMode, int U, class O, bool kNarrow>
+__device__ __forceinline__ uint64_t stream_others_units(Rows<I> R,
+                                                        const int32_t *__restrict__ col,
+                                                        O others, int64_t ob, int64_t oe,
+                                                        int *__restrict__ seg_end,
+                                                        int64_t *__restrict__ seg_base,
+                                                        const CenterSet &C) {
+  static_assert(U == 4 || U == 8 || U == 16, "unit streaming takes 4, 8 or 16 ids per lane");
+  constexpr int IPS = U / 4;        // 16-byte loads per lane per step
+  constexpr int WIN = kWave * IPS;  // units per step
+  using Off = IdOffset<kNarrow>;
+  const int lane = lane_id();
+  // the step's row-end bits: 64-bit word i covers units W + 64 i .. + 63
+  const uint64_t *ends = reinterpret_cast<const uint64_t *>(seg_end);
+  uint32_t acc = 0;  // at most chunk * kBigMaxDeg hits per call
+  for (int64_t base = ob; base < oe; base += kWave) {
+    const int64_t k = base + lane;
+    int len = 0;
+    int64_t st = 0;
+    if (k < oe) others.row(R, k, C.cb, st, len);
+    const int nu = len >> 2;  // whole units of this row
+    // the row's last len % 4 ids
+    const int tail = len & 3;
+    if (__ballot(tail > 0) != 0) {
+      const typename Off::T p = (typename Off::T)(st + 4 * (int64_t)nu);
+      int xs[3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) xs[m] = m < tail ? Off::load(col, p, m) : -2;
+      acc += probe_ids<kMode, 3>(xs, col, C);
+    }
+    // rows with whole units, numbered in lane order
+    const uint64_t has = __ballot(nu > 0);
+    const int ri = (int)__builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(has >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
+    const int inc = wave_inclusive_scan(nu);
+    const int T = __shfl(inc, kWave - 1);  // units of the batch
+    // unit u of row ri sits at col[seg_base[ri] + 4 u]
+    if (nu > 0) seg_base[ri] = st - 4 * (int64_t)(inc - nu);
+    const int last = inc - 1;  // this row's last unit
+    int r0 = 0;                // rows that end before the step's window
+    for (int W = 0; W < T; W += WIN) {
+      wave_lds_sync();
+      if (lane < 2 * IPS) seg_end[lane] = 0;
+      wave_lds_sync();
+      const int e = last - W;
+      if (nu > 0 && (unsigned)e < (unsigned)WIN)
+        atomicOr(reinterpret_cast<unsigned *>(&seg_end[e >> 5]), 1u << (e & 31));
+      wave_lds_sync();
+      int xs[U];
+      int rs = r0;
+#pragma unroll
+      for (int i = 0; i < IPS; ++i) {
+        const uint64_t ei = ends[i];  // the same word for every lane
+        const int u = W + i * kWave + lane;
+        int4 v = make_int4(-2, -2, -2, -2);
+        if (u < T) {
+          const int r = rs + (int)__builtin_amdgcn_mbcnt_hi(
+                                 (uint32_t)(ei >> 32),
+                                 __builtin_amdgcn_mbcnt_lo((uint32_t)ei, 0u));
+          v = Off::load4(col, (typename Off::T)seg_base[r], 4 * u);
+        }
+        rs += __popcll(ei);
+        xs[4 * i] = v.x;
+        xs[4 * i + 1] = v.y;
+        xs[4 * i + 2] = v.z;
+        xs[4 * i + 3] = v.w;
+      }
+      r0 = rs;
+      acc += probe_ids<kMode, U>(xs, col, C);
+    }
+    wave_lds_sync();
+  }
+  return acc;
+}
+
+// Streams the rows of others[ob:oe] and counts the ids found in the
+// center's set: in 4-id units for unroll 4, 8 and 16, id by id below.
+template <class I, int kMode, int U, class O, bool kNarrow>
+__device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__restrict__ col,
+                                                  O others, int64_t ob,
+                                                  int64_t oe, int *__restrict__ seg_end,
+                                                  int64_t *__restrict__ seg_base,
+                                                  const CenterSet &C) {
+  if constexpr (U >= 4)
+    return stream_others_units<I, kMode, U, O, kNarrow>(R, col, others, ob, oe, seg_end,
+                                                         seg_base, C);
+  else
+    return stream_others_ids<I, kMode, U, O, kNarrow>(R, col, others, ob, oe, seg_end, seg_base,
+                                                       C);
 }
 
 // stream_others with the id addressing chosen at run time (uniform per
