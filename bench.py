@@ -50,32 +50,52 @@ def merge_model_bytes(row_ptr: np.ndarray, col: np.ndarray, eu: np.ndarray, ev: 
 
 
 def heavy_floor_bytes(row_ptr: np.ndarray, eu: np.ndarray, ev: np.ndarray, light_max: int,
-                      chunk: int) -> dict:
+                      chunk: int, col: np.ndarray = None, trim: bool = False) -> dict:
     """Algorithmic bytes of one launch of the hash-probe kernels (k_heavy, k_heavy_big).
 
     Heavy edges are those with d+u, d+v > 0 and d+u + d+v > light_max.  Each
     is assigned to its center (the endpoint with the longer row; v on ties)
     and the other row is streamed against an LDS table of the center's row:
-      streamed   = sum over heavy edges of 4 * min(d+u, d+v)
+      streamed   = sum over heavy edges of 4 * (ids of the other's row streamed)
       tables     = sum over work items of 4 * d+(center), with
                    ceil(heavy edges of the center / chunk) items per center
       words      = 8 bytes per heavy edge (its packed planner word)
     B = streamed + tables + words: every byte the kernels must read once.
+    The streamed ids are min(d+u, d+v); with trim (the trim_rows knob) an
+    edge whose center is v streams only u's ids <= the largest id of v's
+    row, and an edge left with none has no word.
     """
     deg = np.diff(row_ptr).astype(np.int64)
     du = deg[eu]
     dv = deg[ev]
     heavy = (du > 0) & (dv > 0) & (du + dv > light_max)
+    hu = eu[heavy].astype(np.int64)
+    hv = ev[heavy].astype(np.int64)
     du = du[heavy]
     dv = dv[heavy]
-    center = np.where(dv >= du, ev[heavy], eu[heavy])
-    streamed = 4 * int(np.minimum(du, dv).sum())
+    cv = dv >= du
+    center = np.where(cv, hv, hu)
+    olen = np.minimum(du, dv)
+    if trim:
+        # ids of u's row <= last(v): a global search over (row, id) keys
+        owner = np.repeat(np.arange(len(deg), dtype=np.int64), deg)
+        keys = (owner << 32) | col.astype(np.int64)
+        del owner
+        hi = col[row_ptr[hv[cv] + 1] - 1].astype(np.int64)
+        cnt = np.searchsorted(keys, (hu[cv] << 32) | hi, side="right") - row_ptr[hu[cv]]
+        del keys
+        olen[cv] = cnt
+        keep = olen > 0
+        center = center[keep]
+        olen = olen[keep]
+    streamed = 4 * int(olen.sum())
     per_center = np.bincount(center, minlength=len(deg)).astype(np.int64)
     items = (per_center + chunk - 1) // chunk
     tables = 4 * int((items * deg).sum())
-    words = 8 * int(heavy.sum())
+    words = 8 * int(len(olen))
     return {"bytes": streamed + tables + words, "streamed_rows": streamed, "center_tables": tables,
-            "packed_words": words, "heavy_edges": int(heavy.sum()), "work_items": int(items.sum())}
+            "packed_words": words, "heavy_edges": int(len(olen)), "work_items": int(items.sum()),
+            "trimmed_rows": bool(trim)}
 
 
 def cpu_model() -> str:
@@ -700,7 +720,8 @@ def run_batch_workload(args, info):
     torch.cuda.synchronize()
     E = b.csr.n_oriented
     params = {"light_max": args.light_max, "chunk": args.chunk,
-              "waves_per_cu": args.waves_per_cu, "unroll": args.unroll}
+              "waves_per_cu": args.waves_per_cu, "unroll": args.unroll,
+              "trim_rows": args.trim_rows}
     g = batch.BatchCountGraph(b, params)
     for _ in range(args.warmup):
         g.launch()
@@ -719,7 +740,8 @@ def run_batch_workload(args, info):
     rp = b.csr.row_ptr.cpu().numpy().astype(np.int64)
     col = b.csr.col_idx.cpu().numpy()
     eu = b.csr.edge_u.cpu().numpy()
-    floor = heavy_floor_bytes(rp, eu, col, args.light_max, args.chunk)
+    floor = heavy_floor_bytes(rp, eu, col, args.light_max, args.chunk, col=col,
+                              trim=bool(args.trim_rows))
     heavy_ms, heavy_n = stages["heavy"]
     heavy_avg = heavy_ms / max(heavy_n, 1)
     parity = None
@@ -774,6 +796,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=512)
     ap.add_argument("--waves-per-cu", type=int, default=16)
     ap.add_argument("--unroll", type=int, default=16)
+    ap.add_argument("--trim-rows", type=int, default=0, choices=(0, 1),
+                    help="1 = the edge pass cuts each streamed row to the center row's "
+                         "largest id")
     ap.add_argument("--extras", default="all",
                     help="comma-separated extra measurements to run (hybrid_fixed_threshold, "
                          "ingest_text, batch4096, learned_router, prototype_table, doulion, "
@@ -828,14 +853,16 @@ def main():
     _log(f"device CSR: n={csr.n_nodes} E={E} in {t_csr * 1e3:.1f} ms")
 
     params = {"light_max": args.light_max, "chunk": args.chunk,
-              "waves_per_cu": args.waves_per_cu, "unroll": args.unroll}
+              "waves_per_cu": args.waves_per_cu, "unroll": args.unroll,
+              "trim_rows": args.trim_rows}
     count = torch.zeros(1, dtype=torch.int64, device="cuda")
     tune_evals = None
     if not args.no_autotune:
-        # the merge-versus-probe crossover, the work-item tile and the
-        # workgroup-phase bound chosen by the project's autotuner on this
-        # graph (coordinate descent from the flags' values, each candidate
-        # the median of 5 timed counts), before the timed region
+        # the merge-versus-probe crossover, the work-item tile, the
+        # workgroup-phase bound and the row trimming chosen by the project's
+        # autotuner on this graph (coordinate descent from the flags' values,
+        # each candidate the median of 5 timed counts), before the timed
+        # region
         from triangle_counting_hll_amd import autotune
 
         def run(p):
@@ -843,13 +870,13 @@ def main():
             gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v, count, p)
 
         space = {"light_max": (24, 32, 36, 40, 48), "chunk": (256, 512, 1024),
-                 "block_min_bits": (9, 10, 11)}
+                 "block_min_bits": (9, 10, 11), "trim_rows": (0, 1)}
         t0 = time.perf_counter()
         tuned, table = autotune.tune(run, f"exact|{args.workload}|n={csr.n_nodes}|E={E}",
                                      space=space, start=dict(params, block_min_bits=10,
                                                              plan="sorted"))
         params = {k: tuned[k] for k in ("light_max", "chunk", "waves_per_cu", "unroll",
-                                        "block_min_bits")}
+                                        "block_min_bits", "trim_rows")}
         tune_evals = len(table)
         _log(f"autotuned {params} in {time.perf_counter() - t0:.1f}s ({tune_evals} evals)")
     for _ in range(args.warmup):
@@ -877,7 +904,8 @@ def main():
     col = csr.col_idx.cpu().numpy()
     eu = csr.edge_u.cpu().numpy()
     B_all, B_merge_heavy, _ = merge_model_bytes(rp, col, eu, col, params["light_max"])
-    floor = heavy_floor_bytes(rp, eu, col, params["light_max"], params["chunk"])
+    floor = heavy_floor_bytes(rp, eu, col, params["light_max"], params["chunk"], col=col,
+                              trim=bool(params.get("trim_rows")))
     heavy_ms, heavy_n = stages["heavy"]
     heavy_avg_ms = heavy_ms / max(heavy_n, 1)
     achieved = floor["bytes"] / (heavy_avg_ms * 1e-3) / 1e9 if heavy_n else None

@@ -151,6 +151,10 @@ typedef struct tc_exact_params {
                          offsets even when every row lies within the first
                          2^30 ids (the form graphs above 2^30 oriented edges
                          take; a test knob); 0 = chosen from row_ptr[n]       */
+  int trim_rows;     /* sorted plan, row layout: 1 = the edge pass cuts each
+                         streamed row to its ids <= the center row's largest
+                         id (an upper-bound search per heavy edge; the ids
+                         above it cannot be common); 0 = whole rows           */
 } tc_exact_params;
 
 /* How heavy edges are grouped by center before the intersection kernel:

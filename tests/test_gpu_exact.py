@@ -55,7 +55,8 @@ def test_clique_beyond_lds_table(device):
 @pytest.mark.parametrize("plan,pack_ids", [("sorted", False), ("sorted", True), ("atomic", False)])
 @pytest.mark.parametrize("wide", [False, True])
 @pytest.mark.parametrize("n", [2100, 4400])
-def test_big_centers_mixed_with_small(device, plan, pack_ids, wide, n):
+@pytest.mark.parametrize("trim", [False, True])
+def test_big_centers_mixed_with_small(device, plan, pack_ids, wide, n, trim):
     # a K_n block plus an R-MAT graph on disjoint ids: forward rows of
     # 1024..4095 ids take the joined workgroup table, rows of 4096 ids and up
     # (n = 4400) the 128 KiB workgroup kernel, the R-MAT rows one wavefront
@@ -65,7 +66,9 @@ def test_big_centers_mixed_with_small(device, plan, pack_ids, wide, n):
     other = rmat(12, 16, seed=8).astype(np.int64) + n
     pairs = np.concatenate([clique, other]).astype(np.int32)
     csr = gpu_graph.build_csr_gpu(pairs)
-    assert (_count(csr, params={"plan": plan, "pack_ids": pack_ids, "wide_offsets": wide})
+    # trim: streamed rows cut to the center row's largest id by the edge pass
+    assert (_count(csr, params={"plan": plan, "pack_ids": pack_ids, "wide_offsets": wide,
+                                "trim_rows": trim})
             == math.comb(n, 3) + _truth(other))
 
 
@@ -88,17 +91,20 @@ def test_rmat14_config(device):
                                                             (32, 256, 16, 16, 10),
                                                             (10**6, 256, 16, 4, 9)])
 @pytest.mark.parametrize("wide", [False, True])
+@pytest.mark.parametrize("trim", [False, True])
 def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc, unroll, blk, plan,
-                                          pack_ids, wide):
+                                          pack_ids, wide, trim):
     # pack_ids: the sorted plan's packed words carry the other's id instead of
     # its row start and length (the layout large graphs fall back to); wide:
     # 64-bit id addressing in the intersection kernels (graphs above 2^30
-    # oriented edges)
+    # oriented edges); trim: the edge pass cuts each streamed row to its ids
+    # at most the center row's largest id
     pairs = rmat(12, 16, seed=7)
     truth = _truth(pairs)
     csr = gpu_graph.build_csr_gpu(pairs)
     params = {"light_max": light_max, "chunk": chunk, "waves_per_cu": wpc, "unroll": unroll,
-              "plan": plan, "block_min_bits": blk, "pack_ids": pack_ids, "wide_offsets": wide}
+              "plan": plan, "block_min_bits": blk, "pack_ids": pack_ids, "wide_offsets": wide,
+              "trim_rows": trim}
     assert _count(csr, params=params) == truth
 
 
@@ -232,7 +238,8 @@ def test_chunglu_full_size_matches_cpu_oracle(device):
     col = csr.col_idx.cpu().numpy()
     truth = cbind.exact_count(rp, col)
     assert truth == FULL["chunglu"]["triangles"]
-    for params in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"wide_offsets": True}):
+    for params in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"wide_offsets": True},
+                   {"trim_rows": True}, {"trim_rows": True, "wide_offsets": True}):
         assert _count(csr, params=params) == truth, params
 
 
@@ -246,7 +253,7 @@ def test_rmat23_full_size_planners_agree(device):
     assert csr.n_oriented == FULL["rmat23"]["oriented_edges"]
     counts = {str(p): _count(csr, params=p)
               for p in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"unroll": 4, "chunk": 64},
-                        {"wide_offsets": True})}
+                        {"wide_offsets": True}, {"trim_rows": True})}
     assert set(counts.values()) == {FULL["rmat23"]["triangles"]}, counts
 
 

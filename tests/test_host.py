@@ -114,3 +114,34 @@ def test_header_constants_match_python_tables():
         _lib.TC_PART_BOTH, _lib.TC_PART_EXACT, _lib.TC_PART_HLL)
     stages = sorted((v, k) for k, v in defs.items() if k.startswith("TC_STAGE_"))
     assert [k[len("TC_STAGE_"):].lower() for _, k in stages] == list(_lib.STAGES)
+
+
+def test_bench_trimmed_floor_matches_brute_force():
+    # the roofline's byte floor with trim_rows: an edge centred on v streams
+    # u's ids up to the largest id of v's row; recomputed edge by edge here
+    import bench
+    from triangle_counting_hll_amd import graphgen
+
+    rp, col, fdeg, _nid, eu, ev, _ = O.csr_from_pairs(graphgen.rmat(10, 8, seed=3))
+    light_max, chunk = 8, 16
+    got = bench.heavy_floor_bytes(rp, eu, ev, light_max, chunk, col=col, trim=True)
+    whole = bench.heavy_floor_bytes(rp, eu, ev, light_max, chunk)
+    streamed, centers = 0, []
+    for u, v in zip(eu.tolist(), ev.tolist()):
+        du, dv = int(fdeg[u]), int(fdeg[v])
+        if du == 0 or dv == 0 or du + dv <= light_max:
+            continue
+        if dv >= du:
+            hi = col[rp[v + 1] - 1]
+            k = int(np.sum(col[rp[u]:rp[u + 1]] <= hi))
+            c = v
+        else:
+            k, c = dv, u
+        if k:
+            streamed += 4 * k
+            centers.append(c)
+    assert got["streamed_rows"] == streamed
+    assert got["heavy_edges"] == len(centers) and got["packed_words"] == 8 * len(centers)
+    per = np.bincount(np.array(centers, dtype=np.int64), minlength=len(fdeg))
+    assert got["center_tables"] == 4 * int((((per + chunk - 1) // chunk) * fdeg).sum())
+    assert got["streamed_rows"] < whole["streamed_rows"]

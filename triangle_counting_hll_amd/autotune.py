@@ -7,7 +7,10 @@ hold at most this many ids are merged in the edge pass, the rest are probed
 by a wavefront), ``waves_per_cu`` (resident wavefronts per compute unit) and
 ``block_min_bits`` (the wavefronts-per-edge choice: centers whose row has at
 least ``2^(block_min_bits - 1)`` ids are taken by a whole workgroup -- four
-wavefronts over one joined LDS table -- and shorter ones by one wavefront).
+wavefronts over one joined LDS table -- and shorter ones by one wavefront) and
+``trim_rows`` (the edge pass cuts each streamed row of u to its ids at most
+the center row's largest id: fewer ids streamed where rows are ordered like
+the degrees, an extra search per heavy edge everywhere).
 
 The search is a coordinate descent from the library defaults, each candidate
 timed as the median of ``reps`` (at least 5) synchronised runs of the
@@ -27,13 +30,14 @@ from typing import Callable, Dict, Optional
 from . import _lib
 
 DEFAULTS = {"chunk": 512, "unroll": 16, "light_max": 32, "waves_per_cu": 16,
-            "block_min_bits": 10, "plan": "sorted"}
+            "block_min_bits": 10, "trim_rows": 0, "plan": "sorted"}
 SPACE = {
     "chunk": (256, 512, 1024),
     "unroll": (8, 16),
     "light_max": (16, 32, 64),
     "waves_per_cu": (8, 16),
     "block_min_bits": (9, 10, 11),
+    "trim_rows": (0, 1),
 }
 MIN_REPS = 5
 

@@ -80,6 +80,46 @@ __device__ __forceinline__ int64_t edge_limit(int64_t E, const unsigned long lon
   return rows_total ? min(E, (int64_t)*rows_total) : E;
 }
 
+// last[v] = the largest id of row v (its last; INT32_MIN for an empty row).
+template <class I>
+__global__ void k_last_ids(Rows<I> R, const int32_t *__restrict__ col, int64_t n,
+                           int32_t *__restrict__ last) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = R.begin(v), e = R.end(v);
+    last[v] = e > b ? col[e - 1] : INT32_MIN;
+  }
+}
+
+// Number of ids <= hi in the sorted row col[st, st + len) whose last id is
+// above hi.  An 8-ary search: each round reads the last ids of the first
+// seven eighths of the range together (one memory round trip instead of
+// three for a binary search) and keeps the eighth that holds the boundary.
+__device__ __forceinline__ int64_t count_at_most(const int32_t *__restrict__ col, int64_t st,
+                                                 int64_t len, int32_t hi) {
+  // the answer lies in [lo, lo + m]: ids before lo are <= hi, ids from
+  // lo + m on are above it (the last one is)
+  int64_t lo = 0, m = len - 1;
+  while (m >= 8) {
+    const int64_t q = m >> 3;
+    int k = 0;  // eighths whose last id is <= hi (a prefix of them)
+#pragma unroll
+    for (int j = 1; j < 8; ++j) k += col[st + lo + j * q - 1] <= hi ? 1 : 0;
+    lo += k * q;
+    m = k == 7 ? m - 7 * q : q - 1;
+  }
+  while (m > 0) {
+    const int64_t half = m >> 1;
+    if (col[st + lo + half] <= hi) {
+      lo += half + 1;
+      m -= half + 1;
+    } else {
+      m = half;
+    }
+  }
+  return lo;
+}
+
 __device__ __forceinline__ uint32_t merge_count(const int32_t *__restrict__ col, int64_t a0,
                                                 int64_t a1, int64_t b0, int64_t b1) {
   uint32_t c = 0;
@@ -438,6 +478,11 @@ struct PackLayout {
   int force_ids;
   const unsigned long long *stats;
   int rel_bits = 0;
+  // row layout, other = u: with last_id (the largest id of each row), the
+  // length field holds the number of u's ids that are <= the center row's
+  // largest id (the rest cannot be in the center row), found by the edge
+  // pass; else the other's whole length
+  const int32_t *last_id = nullptr;
   __device__ __forceinline__ int lbits() const {
     return shift - (rel_bits > 0 ? rel_bits : max(1, bit_length_u64(stats[1])));
   }
@@ -1161,6 +1206,7 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
     snorm = s_norm;
     s_src = s_list[threadIdx.x / kWave];
   }
+  const bool trim = rows_layout && L.last_id != nullptr;
   // the loop bound is uniform per wavefront, so every lane stays active for
   // the cross-lane work below
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < E;
@@ -1230,13 +1276,21 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
         } else {
           const bool cv = dv >= du;
           const uint32_t c = (uint32_t)(cv ? v : u), o = (uint32_t)(cv ? u : v);
-          word = (uint64_t)c << L.shift;
-          if (rows_layout)
-            word |= (L.start_field(cv ? ub : vb, cv ? vb : ub) << lbits) |
-                    (uint64_t)(cv ? du : dv);
-          else
-            word |= o;
-          ++nh;
+          int64_t olen = cv ? du : dv;
+          // the streamed row is u's own (this pass walks the rows of u, so
+          // the search mostly hits cached lines); rows of v are not cut
+          if (trim && cv) {
+            const int32_t hi = L.last_id[v];
+            if (L.last_id[u] > hi) olen = count_at_most(col, ub, du, hi);
+          }
+          if (olen > 0) {
+            word = (uint64_t)c << L.shift;
+            if (rows_layout)
+              word |= (L.start_field(cv ? ub : vb, cv ? vb : ub) << lbits) | (uint64_t)olen;
+            else
+              word |= o;
+            ++nh;
+          }
         }
       }
       packed[e] = word;
@@ -1392,7 +1446,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              const int32_t *eu, const int32_t *ev, int64_t E,
                              const uint8_t *skip, const int32_t *graph_of,
                              unsigned long long *count, int light_max, int chunk, int wpc,
-                             int unroll, int blk_bits, bool pack_ids, bool wide,
+                             int unroll, int blk_bits, bool pack_ids, bool wide, bool trim,
                              const BatchPlan &bp, hipStream_t s, const HybridRoute *hr) {
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;
@@ -1438,7 +1492,10 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   unsigned long long *next_item = ctr.as<unsigned long long>();
   unsigned long long *n_heavy = next_item + 1;
   unsigned long long *row_stats = next_item + 4;  // 3 entries
-  const PackLayout layout{shift, pack_ids ? 1 : 0, row_stats, bp.rel_bits};
+  // with trim_rows the rows' last ids go to nitems (int32[n], unused until
+  // the planner after the sort)
+  const PackLayout layout{shift, pack_ids ? 1 : 0, row_stats, bp.rel_bits,
+                          trim && !pack_ids ? nitems.as<int32_t>() : nullptr};
   const unsigned long long *rows_total = csr_mode ? row_stats + 1 : nullptr;
 
   const unsigned gr = grid_for(max_runs + 1, kBlock, 16384);
@@ -1451,6 +1508,9 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     Rows<I> R{(const I *)row_ptr};
     {
       StageTimer _timer(TC_STAGE_CLASSIFY, s);
+      if (layout.last_id)
+        hipLaunchKernelGGL(k_last_ids<I>, dim3(grid_for(n, kBlock, 16384)), dim3(kBlock), 0, s,
+                           R, col, n, nitems.as<int32_t>());
       if (hr && hr->net.params) {
         TC_DISPATCH_GROUP(hr->net.sample_bits, G, {
           hipLaunchKernelGGL((k_classify_pack<I, 2, G>), dim3(ge), dim3(kBlock), 0, s, R, col,
@@ -1593,7 +1653,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                              light_max, chunk, wpc, unroll, blk_bits,
                              params && params->pack_ids != 0,
                              params && params->wide_offsets != 0,
-                             bp ? *bp : BatchPlan{}, s, hr);
+                             params && params->trim_rows != 0, bp ? *bp : BatchPlan{}, s, hr);
 
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;

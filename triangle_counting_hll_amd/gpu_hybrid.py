@@ -81,7 +81,8 @@ def _exact_params(params: Optional[dict]):
                             _lib.PLANS[plan] if isinstance(plan, str) else int(plan),
                             int(params.get("block_min_bits", 0)),
                             int(bool(params.get("pack_ids", False))),
-                            int(bool(params.get("wide_offsets", False))))
+                            int(bool(params.get("wide_offsets", False))),
+                            int(bool(params.get("trim_rows", False))))
 
 
 @_lib.on_stream
