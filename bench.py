@@ -721,7 +721,7 @@ def run_batch_workload(args, info):
     E = b.csr.n_oriented
     params = {"light_max": args.light_max, "chunk": args.chunk,
               "waves_per_cu": args.waves_per_cu, "unroll": args.unroll,
-              "trim_rows": args.trim_rows}
+              "trim_rows": args.trim_rows, "tail_units": args.tail_units}
     g = batch.BatchCountGraph(b, params)
     for _ in range(args.warmup):
         g.launch()
@@ -796,6 +796,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=512)
     ap.add_argument("--waves-per-cu", type=int, default=16)
     ap.add_argument("--unroll", type=int, default=16)
+    ap.add_argument("--tail-units", type=int, default=0, choices=(0, 1),
+                    help="1 = each streamed row's last len %% 4 ids are read in one masked "
+                         "16-byte load with its last whole ids")
     ap.add_argument("--trim-rows", type=int, default=0, choices=(0, 1),
                     help="1 = the edge pass cuts each streamed row to the center row's "
                          "largest id")
@@ -854,12 +857,13 @@ def main():
 
     params = {"light_max": args.light_max, "chunk": args.chunk,
               "waves_per_cu": args.waves_per_cu, "unroll": args.unroll,
-              "trim_rows": args.trim_rows}
+              "trim_rows": args.trim_rows, "tail_units": args.tail_units}
     count = torch.zeros(1, dtype=torch.int64, device="cuda")
     tune_evals = None
     if not args.no_autotune:
         # the merge-versus-probe crossover, the work-item tile, the
-        # workgroup-phase bound and the row trimming chosen by the project's
+        # workgroup-phase bound, the row trimming and the reading of the
+        # rows' last ids chosen by the project's
         # autotuner on this graph (coordinate descent from the flags' values,
         # each candidate the median of 5 timed counts), before the timed
         # region
@@ -870,13 +874,13 @@ def main():
             gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v, count, p)
 
         space = {"light_max": (24, 32, 36, 40, 48), "chunk": (256, 512, 1024),
-                 "block_min_bits": (9, 10, 11), "trim_rows": (0, 1)}
+                 "block_min_bits": (9, 10, 11), "trim_rows": (0, 1), "tail_units": (0, 1)}
         t0 = time.perf_counter()
         tuned, table = autotune.tune(run, f"exact|{args.workload}|n={csr.n_nodes}|E={E}",
                                      space=space, start=dict(params, block_min_bits=10,
                                                              plan="sorted"))
         params = {k: tuned[k] for k in ("light_max", "chunk", "waves_per_cu", "unroll",
-                                        "block_min_bits", "trim_rows")}
+                                        "block_min_bits", "trim_rows", "tail_units")}
         tune_evals = len(table)
         _log(f"autotuned {params} in {time.perf_counter() - t0:.1f}s ({tune_evals} evals)")
     for _ in range(args.warmup):

@@ -155,6 +155,11 @@ typedef struct tc_exact_params {
                          streamed row to its ids <= the center row's largest
                          id (an upper-bound search per heavy edge; the ids
                          above it cannot be common); 0 = whole rows           */
+  int tail_units;    /* intersection kernels, unroll 4..16: 1 = each streamed
+                         row's last len % 4 ids are read in one 16-byte load
+                         with its last whole ids, the ids past the row
+                         masked (one memory line read once; masking work per
+                         load); 0 = read one by one before the row's units   */
 } tc_exact_params;
 
 /* How heavy edges are grouped by center before the intersection kernel:

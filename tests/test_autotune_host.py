@@ -18,7 +18,7 @@ def _cost(best):
 
 def test_descent_finds_the_separable_minimum():
     best = {"chunk": 1024, "unroll": 8, "light_max": 16, "waves_per_cu": 8, "block_min_bits": 11,
-            "trim_rows": 1}
+            "trim_rows": 1, "tail_units": 1}
     calls = []
 
     def measure(p):
@@ -52,7 +52,7 @@ def test_cache_round_trip(tmp_path, monkeypatch):
     monkeypatch.setenv("TCHLL_TUNE_CACHE", str(path))
     autotune.clear_memory()
     best = {"chunk": 256, "unroll": 16, "light_max": 64, "waves_per_cu": 16, "block_min_bits": 9,
-            "trim_rows": 0}
+            "trim_rows": 0, "tail_units": 1}
     n = []
 
     def timer(p):

@@ -118,7 +118,7 @@ def test_batch_sort_width_follows_the_layout(device, shuffle):
         eu, ev = eu[perm].contiguous(), ev[perm].contiguous()
     counts = torch.zeros(b.n_graphs, dtype=torch.int64, device="cuda")
     rp, is64 = _lib.row_ptr_arg(b.csr.row_ptr)
-    for params in ({}, {"pack_ids": True}, {"trim_rows": True}):
+    for params in ({}, {"pack_ids": True}, {"trim_rows": True}, {"tail_units": True}):
         counts.zero_()
         p = gpu_hybrid._exact_params(params)
         _lib.check(_lib.load().tc_batch_exact_count(

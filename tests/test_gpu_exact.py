@@ -43,6 +43,20 @@ def test_complete_graph_closed_form(device, n):
     assert _count(csr) == math.comb(n, 3)
 
 
+@pytest.mark.parametrize("n", [5, 6, 7, 9, 17, 40])
+def test_tail_units_rows_at_the_end_of_col(device, n):
+    # K_n oriented by id: the last rows of col are 3, 2 and 1 ids long and,
+    # with light_max 1, are streamed as the shorter row of heavy edges; a
+    # masked last unit there would read past col's last id, so those rows
+    # take the single-id path
+    pairs = np.stack(np.triu_indices(n, 1), axis=1).astype(np.int32)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    for params in ({"light_max": 1, "tail_units": True},
+                   {"light_max": 1, "tail_units": True, "trim_rows": True},
+                   {"light_max": 1, "tail_units": True, "plan": "atomic"}):
+        assert _count(csr, params=params) == math.comb(n, 3), params
+
+
 def test_clique_beyond_lds_table(device):
     # K_1400: forward rows of 1024..1399 ids exceed the per-wave LDS table and
     # are taken by whole workgroups over the joined table; C(1400, 3) triangles.
@@ -55,8 +69,8 @@ def test_clique_beyond_lds_table(device):
 @pytest.mark.parametrize("plan,pack_ids", [("sorted", False), ("sorted", True), ("atomic", False)])
 @pytest.mark.parametrize("wide", [False, True])
 @pytest.mark.parametrize("n", [2100, 4400])
-@pytest.mark.parametrize("trim", [False, True])
-def test_big_centers_mixed_with_small(device, plan, pack_ids, wide, n, trim):
+@pytest.mark.parametrize("trim,tail", [(False, False), (True, True), (False, True)])
+def test_big_centers_mixed_with_small(device, plan, pack_ids, wide, n, trim, tail):
     # a K_n block plus an R-MAT graph on disjoint ids: forward rows of
     # 1024..4095 ids take the joined workgroup table, rows of 4096 ids and up
     # (n = 4400) the 128 KiB workgroup kernel, the R-MAT rows one wavefront
@@ -66,9 +80,10 @@ def test_big_centers_mixed_with_small(device, plan, pack_ids, wide, n, trim):
     other = rmat(12, 16, seed=8).astype(np.int64) + n
     pairs = np.concatenate([clique, other]).astype(np.int32)
     csr = gpu_graph.build_csr_gpu(pairs)
-    # trim: streamed rows cut to the center row's largest id by the edge pass
+    # trim: streamed rows cut to the center row's largest id by the edge pass;
+    # tail: rows' last len % 4 ids in a masked 16-byte load
     assert (_count(csr, params={"plan": plan, "pack_ids": pack_ids, "wide_offsets": wide,
-                                "trim_rows": trim})
+                                "trim_rows": trim, "tail_units": tail})
             == math.comb(n, 3) + _truth(other))
 
 
@@ -91,20 +106,21 @@ def test_rmat14_config(device):
                                                             (32, 256, 16, 16, 10),
                                                             (10**6, 256, 16, 4, 9)])
 @pytest.mark.parametrize("wide", [False, True])
-@pytest.mark.parametrize("trim", [False, True])
+@pytest.mark.parametrize("trim,tail", [(False, False), (True, True), (False, True)])
 def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc, unroll, blk, plan,
-                                          pack_ids, wide, trim):
+                                          pack_ids, wide, trim, tail):
     # pack_ids: the sorted plan's packed words carry the other's id instead of
     # its row start and length (the layout large graphs fall back to); wide:
     # 64-bit id addressing in the intersection kernels (graphs above 2^30
     # oriented edges); trim: the edge pass cuts each streamed row to its ids
-    # at most the center row's largest id
+    # at most the center row's largest id; tail: each streamed row's last
+    # len % 4 ids read in a masked 16-byte load with its last whole ids
     pairs = rmat(12, 16, seed=7)
     truth = _truth(pairs)
     csr = gpu_graph.build_csr_gpu(pairs)
     params = {"light_max": light_max, "chunk": chunk, "waves_per_cu": wpc, "unroll": unroll,
               "plan": plan, "block_min_bits": blk, "pack_ids": pack_ids, "wide_offsets": wide,
-              "trim_rows": trim}
+              "trim_rows": trim, "tail_units": tail}
     assert _count(csr, params=params) == truth
 
 
@@ -239,7 +255,9 @@ def test_chunglu_full_size_matches_cpu_oracle(device):
     truth = cbind.exact_count(rp, col)
     assert truth == FULL["chunglu"]["triangles"]
     for params in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"wide_offsets": True},
-                   {"trim_rows": True}, {"trim_rows": True, "wide_offsets": True}):
+                   {"trim_rows": True}, {"trim_rows": True, "wide_offsets": True},
+                   {"trim_rows": True, "tail_units": True},
+                   {"tail_units": True, "wide_offsets": True, "plan": "atomic"}):
         assert _count(csr, params=params) == truth, params
 
 
@@ -253,7 +271,7 @@ def test_rmat23_full_size_planners_agree(device):
     assert csr.n_oriented == FULL["rmat23"]["oriented_edges"]
     counts = {str(p): _count(csr, params=p)
               for p in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"unroll": 4, "chunk": 64},
-                        {"wide_offsets": True}, {"trim_rows": True})}
+                        {"wide_offsets": True}, {"trim_rows": True}, {"tail_units": True})}
     assert set(counts.values()) == {FULL["rmat23"]["triangles"]}, counts
 
 

@@ -62,7 +62,7 @@ def test_noalloc_with_knobs_and_repeat(device):
     again = _pair(csr, params=params)
     assert first[0] == first[1] == again[0] == again[1]
     # rows cut by the edge pass (the last ids share scratch with the planner)
-    trimmed = _pair(csr, params=_lib.ExactParams(16, 64, 8, 4, 0, 9, 0, 0, 1))
+    trimmed = _pair(csr, params=_lib.ExactParams(16, 64, 8, 4, 0, 9, 0, 0, 1, 1))
     assert trimmed[0] == trimmed[1] == first[0]
 
 

@@ -10,7 +10,9 @@ least ``2^(block_min_bits - 1)`` ids are taken by a whole workgroup -- four
 wavefronts over one joined LDS table -- and shorter ones by one wavefront) and
 ``trim_rows`` (the edge pass cuts each streamed row of u to its ids at most
 the center row's largest id: fewer ids streamed where rows are ordered like
-the degrees, an extra search per heavy edge everywhere).
+the degrees, an extra search per heavy edge everywhere) and ``tail_units``
+(each streamed row's last ``len % 4`` ids read with its last whole ids in one
+masked 16-byte load, or one by one before its units).
 
 The search is a coordinate descent from the library defaults, each candidate
 timed as the median of ``reps`` (at least 5) synchronised runs of the
@@ -30,7 +32,7 @@ from typing import Callable, Dict, Optional
 from . import _lib
 
 DEFAULTS = {"chunk": 512, "unroll": 16, "light_max": 32, "waves_per_cu": 16,
-            "block_min_bits": 10, "trim_rows": 0, "plan": "sorted"}
+            "block_min_bits": 10, "trim_rows": 0, "tail_units": 0, "plan": "sorted"}
 SPACE = {
     "chunk": (256, 512, 1024),
     "unroll": (8, 16),
@@ -38,6 +40,7 @@ SPACE = {
     "waves_per_cu": (8, 16),
     "block_min_bits": (9, 10, 11),
     "trim_rows": (0, 1),
+    "tail_units": (0, 1),
 }
 MIN_REPS = 5
 

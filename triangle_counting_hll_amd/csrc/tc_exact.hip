@@ -589,6 +589,7 @@ struct CenterSet {
   uint32_t sentinel;
   int64_t cb;
   int dc;
+  int64_t total_ids;  // row_ptr[n]: ids of col[] the rows reach
 };
 
 // Counts the ids of xs[0:K] found in the center's set (kMode: kProbeGlobal,
@@ -698,18 +699,24 @@ __device__ __forceinline__ uint64_t stream_others_ids(Rows<I> R, const int32_t *
 }
 
 // The rows of others[ob:oe] for unroll 4, 8 and 16.  Each lane takes one
-// row of a batch of 64.  A row's whole 4-id units are numbered across the
-// batch (rows without one left out), and step by step the wavefront takes
-// the next 64 * U / 4 units: lane l loads units W + 64 i + l (i < U / 4),
-// each with one 16-byte load, so every load instruction reads 1 KiB of
-// consecutive units.  A lane finds the row of its unit without walking the
-// row ends: at each step every row whose last unit falls in the step's
-// window sets its bit in U / 4 LDS words; the row of unit W + 64 i + l is
-// the number of rows that ended before W, plus the bits of the words before
-// word i, plus the bits of word i below lane l (mbcnt).  The last len % 4
-// ids of each row are read by that row's own lane with single-id loads, so
-// no load passes the end of a row.
-template <class I, int kMode, int U, class O, bool kNarrow>
+// row of a batch of 64.  A row's 4-id units (the last one partial when the
+// row's length is not a multiple of 4) are numbered across the batch, and
+// step by step the wavefront takes the next 64 * U / 4 units: lane l loads
+// units W + 64 i + l (i < U / 4), each with one 16-byte load, so every load
+// instruction reads 1 KiB of consecutive units and a row's last ids are read
+// with the rest of their memory line.  A lane finds the row of its unit
+// without walking the row ends: at each step every row whose last unit falls
+// in the step's window sets its bit in U / 4 LDS words; the row of unit
+// W + 64 i + l is the number of rows that ended before W, plus the bits of
+// the words before word i, plus the bits of word i below lane l (mbcnt).
+// With kTail the ids of a partial unit past its row's end (the next
+// row's, read in the same memory line) are replaced by the padding -2, and
+// a row whose partial unit would reach past the last id of col[]
+// (C.total_ids) reads its last len % 4 ids one by one instead.  Without it
+// every row's last len % 4 ids are read one by one by the row's own lane
+// before the steps (no masking work; the memory line they share with the
+// row's last unit may be fetched twice).
+template <class I, int kMode, int U, class O, bool kNarrow, bool kTail>
 __device__ __forceinline__ uint64_t stream_others_units(Rows<I> R,
                                                         const int32_t *__restrict__ col,
                                                         O others, int64_t ob, int64_t oe,
@@ -723,30 +730,37 @@ __device__ __forceinline__ uint64_t stream_others_units(Rows<I> R,
   const int lane = lane_id();
   // the step's row-end bits: 64-bit word i covers units W + 64 i .. + 63
   const uint64_t *ends = reinterpret_cast<const uint64_t *>(seg_end);
+  // row ri: seg_base[2 ri] = col position of its unit 0 (unit u at
+  // seg_base[2 ri] + 4 u), seg_base[2 ri + 1] = its end in the batch's id
+  // numbering (4 * first unit + length)
   uint32_t acc = 0;  // at most chunk * kBigMaxDeg hits per call
   for (int64_t base = ob; base < oe; base += kWave) {
     const int64_t k = base + lane;
     int len = 0;
     int64_t st = 0;
     if (k < oe) others.row(R, k, C.cb, st, len);
-    const int nu = len >> 2;  // whole units of this row
-    // the row's last len % 4 ids
-    const int tail = len & 3;
-    if (__ballot(tail > 0) != 0) {
+    const int nq = (len + 3) >> 2;
+    const bool apart = !kTail || st + 4 * (int64_t)nq > C.total_ids;
+    const int nu = apart ? (len >> 2) : nq;  // units of this row
+    // the last len % 4 ids of a row read apart from its units, one by one
+    if (__ballot(apart && (len & 3) != 0) != 0) {
+      const int tail = apart ? (len & 3) : 0;
       const typename Off::T p = (typename Off::T)(st + 4 * (int64_t)nu);
       int xs[3];
 #pragma unroll
       for (int m = 0; m < 3; ++m) xs[m] = m < tail ? Off::load(col, p, m) : -2;
       acc += probe_ids<kMode, 3>(xs, col, C);
     }
-    // rows with whole units, numbered in lane order
+    // rows with units, numbered in lane order
     const uint64_t has = __ballot(nu > 0);
     const int ri = (int)__builtin_amdgcn_mbcnt_hi(
         (uint32_t)(has >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
     const int inc = wave_inclusive_scan(nu);
     const int T = __shfl(inc, kWave - 1);  // units of the batch
-    // unit u of row ri sits at col[seg_base[ri] + 4 u]
-    if (nu > 0) seg_base[ri] = st - 4 * (int64_t)(inc - nu);
+    if (nu > 0) {
+      seg_base[2 * ri] = st - 4 * (int64_t)(inc - nu);
+      seg_base[2 * ri + 1] = 4 * (int64_t)(inc - nu) + (apart ? 4 * nu : len);
+    }
     const int last = inc - 1;  // this row's last unit
     int r0 = 0;                // rows that end before the step's window
     for (int W = 0; W < T; W += WIN) {
@@ -758,17 +772,21 @@ __device__ __forceinline__ uint64_t stream_others_units(Rows<I> R,
         atomicOr(reinterpret_cast<unsigned *>(&seg_end[e >> 5]), 1u << (e & 31));
       wave_lds_sync();
       int xs[U];
+      int lim[IPS];  // ids of each unit inside its row (4 when whole)
       int rs = r0;
 #pragma unroll
       for (int i = 0; i < IPS; ++i) {
         const uint64_t ei = ends[i];  // the same word for every lane
         const int u = W + i * kWave + lane;
         int4 v = make_int4(-2, -2, -2, -2);
+        lim[i] = 4;
         if (u < T) {
           const int r = rs + (int)__builtin_amdgcn_mbcnt_hi(
                                  (uint32_t)(ei >> 32),
                                  __builtin_amdgcn_mbcnt_lo((uint32_t)ei, 0u));
-          v = Off::load4(col, (typename Off::T)seg_base[r], 4 * u);
+          const int4 rw = reinterpret_cast<const int4 *>(seg_base)[r];  // base, end
+          v = Off::load4(col, (typename Off::T)(((int64_t)rw.y << 32) | (uint32_t)rw.x), 4 * u);
+          lim[i] = rw.z - 4 * u;
         }
         rs += __popcll(ei);
         xs[4 * i] = v.x;
@@ -777,6 +795,15 @@ __device__ __forceinline__ uint64_t stream_others_units(Rows<I> R,
         xs[4 * i + 3] = v.w;
       }
       r0 = rs;
+      // after every load of the step is issued: the ids past a row's end
+      if constexpr (kTail) {
+#pragma unroll
+        for (int i = 0; i < IPS; ++i) {
+          xs[4 * i + 1] = lim[i] > 1 ? xs[4 * i + 1] : -2;
+          xs[4 * i + 2] = lim[i] > 2 ? xs[4 * i + 2] : -2;
+          xs[4 * i + 3] = lim[i] > 3 ? xs[4 * i + 3] : -2;
+        }
+      }
       acc += probe_ids<kMode, U>(xs, col, C);
     }
     wave_lds_sync();
@@ -785,16 +812,17 @@ __device__ __forceinline__ uint64_t stream_others_units(Rows<I> R,
 }
 
 // Streams the rows of others[ob:oe] and counts the ids found in the
-// center's set: in 4-id units for unroll 4, 8 and 16, id by id below.
-template <class I, int kMode, int U, class O, bool kNarrow>
+// center's set: in 4-id units for unroll 4, 8 and 16 (the rows' last ids
+// with their units with kTail, else apart), id by id below.
+template <class I, int kMode, int U, class O, bool kNarrow, bool kTail>
 __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__restrict__ col,
                                                   O others, int64_t ob,
                                                   int64_t oe, int *__restrict__ seg_end,
                                                   int64_t *__restrict__ seg_base,
                                                   const CenterSet &C) {
   if constexpr (U >= 4)
-    return stream_others_units<I, kMode, U, O, kNarrow>(R, col, others, ob, oe, seg_end,
-                                                         seg_base, C);
+    return stream_others_units<I, kMode, U, O, kNarrow, kTail>(R, col, others, ob, oe, seg_end,
+                                                                seg_base, C);
   else
     return stream_others_ids<I, kMode, U, O, kNarrow>(R, col, others, ob, oe, seg_end, seg_base,
                                                        C);
@@ -802,15 +830,15 @@ __device__ __forceinline__ uint64_t stream_others(Rows<I> R, const int32_t *__re
 
 // stream_others with the id addressing chosen at run time (uniform per
 // kernel): 32-bit offsets when the rows lie within 2^30 ids, else 64-bit.
-template <class I, int kMode, int U, class O>
+template <class I, int kMode, int U, class O, bool kTail>
 __device__ __forceinline__ uint64_t stream_others_rt(bool narrow, Rows<I> R,
                                                      const int32_t *__restrict__ col, O others,
                                                      int64_t ob, int64_t oe,
                                                      int *__restrict__ seg_end,
                                                      int64_t *__restrict__ seg_base,
                                                      const CenterSet &C) {
-  return narrow ? stream_others<I, kMode, U, O, true>(R, col, others, ob, oe, seg_end, seg_base, C)
-                : stream_others<I, kMode, U, O, false>(R, col, others, ob, oe, seg_end, seg_base,
+  return narrow ? stream_others<I, kMode, U, O, true, kTail>(R, col, others, ob, oe, seg_end, seg_base, C)
+                : stream_others<I, kMode, U, O, false, kTail>(R, col, others, ob, oe, seg_end, seg_base,
                                                        C);
 }
 
@@ -819,24 +847,24 @@ __device__ __forceinline__ uint64_t stream_others_rt(bool narrow, Rows<I> R,
 // row's id range fits the table's bits, else the hash table when the row
 // fits it (kMaxHash ids), else the global row; then streams others[ob:oe].
 // Every lane (every thread, kBlockWide) calls this with the same item.
-template <class I, int U, class O, int kInts, int kMaxHash, bool kBlockWide>
+template <class I, int U, class O, int kInts, int kMaxHash, bool kBlockWide, bool kTail>
 __device__ __forceinline__ uint64_t count_item(bool narrow, Rows<I> R,
                                                const int32_t *__restrict__ col, O others,
                                                int64_t ob, int64_t oe, int *__restrict__ seg_end,
                                                int64_t *__restrict__ seg_base, int *tbl,
-                                               int64_t cb, int dc) {
-  CenterSet C{reinterpret_cast<const int4 *>(tbl), 0, 0, 0, 0, cb, dc};
+                                               int64_t cb, int dc, int64_t total_ids) {
+  CenterSet C{reinterpret_cast<const int4 *>(tbl), 0, 0, 0, 0, cb, dc, total_ids};
   if (bitmap_fits<kInts>(col + cb, dc, &C.lo, &C.sentinel)) {
     bitmap_build<kInts, kBlockWide>(tbl, col + cb, dc, C.lo, C.sentinel);
-    return stream_others_rt<I, kProbeBitmap, U, O>(narrow, R, col, others, ob, oe, seg_end,
+    return stream_others_rt<I, kProbeBitmap, U, O, kTail>(narrow, R, col, others, ob, oe, seg_end,
                                                    seg_base, C);
   }
   if (dc <= kMaxHash) {
     table_build<kInts, kBlockWide>(tbl, col + cb, dc, C.shift, C.mask);
-    return stream_others_rt<I, kProbeHash, U, O>(narrow, R, col, others, ob, oe, seg_end,
+    return stream_others_rt<I, kProbeHash, U, O, kTail>(narrow, R, col, others, ob, oe, seg_end,
                                                  seg_base, C);
   }
-  return stream_others_rt<I, kProbeGlobal, U, O>(narrow, R, col, others, ob, oe, seg_end,
+  return stream_others_rt<I, kProbeGlobal, U, O, kTail>(narrow, R, col, others, ob, oe, seg_end,
                                                  seg_base, C);
 }
 
@@ -865,7 +893,7 @@ struct GraphAcc {
 
 // Four wavefronts per SIMD (at most 128 VGPRs): the LDS tables allow four
 // workgroups per CU, and the loads in flight need the occupancy.
-template <class I, int U, class O, bool kXcdLocal>
+template <class I, int U, class O, bool kXcdLocal, bool kTail>
 __global__ void __launch_bounds__(kBlock, 4)
 k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__restrict__ row_stats,
         const int64_t *__restrict__ cptr, O others, const int2 *__restrict__ items,
@@ -877,12 +905,13 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
         unsigned long long *__restrict__ count) {
   __shared__ int4 tbl_all[kWavesPerBlock][kTblInts / 4];
   __shared__ int seg_end_all[kWavesPerBlock][kWave];
-  __shared__ int64_t seg_base_all[kWavesPerBlock][kWave];
+  __shared__ __align__(16) int64_t seg_base_all[kWavesPerBlock][2 * kWave];
   __shared__ int64_t s_ticket;
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   others.resolve();
   const bool narrow = rows_narrow(row_stats);
+  const int64_t total_ids = (int64_t)row_stats[1];
   int4 *tbl = tbl_all[w];
   int *seg_end = seg_end_all[w];
   int64_t *seg_base = seg_base_all[w];
@@ -918,8 +947,8 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
       const int dc = (int)(R.end(c) - cb);
       const int64_t part = (oe - ob + kWavesPerBlock - 1) / kWavesPerBlock;
       const int64_t wb = min(oe, ob + w * part), we = min(oe, wb + part);
-      const uint64_t acc = count_item<I, U, O, kBlkTblInts, kBlkTblInts / 2 - 1, true>(
-          narrow, R, col, others, wb, we, seg_end, seg_base, tbl_blk, cb, dc);
+      const uint64_t acc = count_item<I, U, O, kBlkTblInts, kBlkTblInts / 2 - 1, true, kTail>(
+          narrow, R, col, others, wb, we, seg_end, seg_base, tbl_blk, cb, dc, total_ids);
       if (graph_of)
         gacc.add(graph_of[c], acc, count);
       else
@@ -968,9 +997,9 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
       const int32_t c = others.center(item.x, ob);
       const int64_t cb = R.begin(c);
       const int dc = (int)(R.end(c) - cb);
-      const uint64_t acc = count_item<I, U, O, kTblInts, kTblMaxDeg, false>(
+      const uint64_t acc = count_item<I, U, O, kTblInts, kTblMaxDeg, false, kTail>(
           narrow, R, col, others, ob, oe, seg_end, seg_base, reinterpret_cast<int *>(tbl), cb,
-          dc);
+          dc, total_ids);
       if (graph_of)
         gacc.add(graph_of[c], acc, count);
       else
@@ -1017,7 +1046,7 @@ __global__ void k_edge_supports(Rows<I> R, const int32_t *__restrict__ col,
 // four wavefronts streaming disjoint quarters of the item's others.  Items of
 // these centers come first in the heavy-first order; this kernel takes items
 // [0, n_first) and k_heavy the rest.
-template <class I, int U, class O>
+template <class I, int U, class O, bool kTail>
 __global__ void __launch_bounds__(kBlock)
 k_heavy_big(Rows<I> R, const int32_t *__restrict__ col,
             const unsigned long long *__restrict__ row_stats, const int64_t *__restrict__ cptr,
@@ -1027,12 +1056,13 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col,
             unsigned long long *__restrict__ count) {
   __shared__ int4 tbl[kBigTblInts / 4];
   __shared__ int seg_end_all[kWavesPerBlock][kWave];
-  __shared__ int64_t seg_base_all[kWavesPerBlock][kWave];
+  __shared__ __align__(16) int64_t seg_base_all[kWavesPerBlock][2 * kWave];
   __shared__ int64_t s_item;
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   others.resolve();
   const bool narrow = rows_narrow(row_stats);
+  const int64_t total_ids = (int64_t)row_stats[1];
   const int64_t n_first = iptr[*n_first_runs_p];
   int *tbl_i = reinterpret_cast<int *>(tbl);
   uint64_t acc_total = 0;
@@ -1052,8 +1082,9 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col,
     const int64_t part = (oe - ob + kWavesPerBlock - 1) / kWavesPerBlock;
     const int64_t wb = min(oe, ob + w * part), we = min(oe, wb + part);
     uint64_t acc;
-    acc = count_item<I, U, O, kBigTblInts, kBigMaxDeg, true>(
-        narrow, R, col, others, wb, we, seg_end_all[w], seg_base_all[w], tbl_i, cb, dc);
+    acc = count_item<I, U, O, kBigTblInts, kBigMaxDeg, true, kTail>(
+        narrow, R, col, others, wb, we, seg_end_all[w], seg_base_all[w], tbl_i, cb, dc,
+        total_ids);
     __syncthreads();  // the table is rebuilt for the next item
     if (graph_of)
       gacc.add(graph_of[c], acc, count);
@@ -1069,24 +1100,27 @@ k_heavy_big(Rows<I> R, const int32_t *__restrict__ col,
 }
 
 template <class I, class O>
-static auto heavy_big_kernel(int unroll) -> decltype(&k_heavy_big<I, 4, O>) {
+static auto heavy_big_kernel(int unroll, bool tail) -> decltype(&k_heavy_big<I, 4, O, false>) {
   switch (unroll) {
-    case 1: return &k_heavy_big<I, 1, O>;
-    case 2: return &k_heavy_big<I, 2, O>;
-    case 4: return &k_heavy_big<I, 4, O>;
-    case 8: return &k_heavy_big<I, 8, O>;
-    default: return &k_heavy_big<I, 16, O>;
+    case 1: return &k_heavy_big<I, 1, O, false>;
+    case 2: return &k_heavy_big<I, 2, O, false>;
+    case 4: return tail ? &k_heavy_big<I, 4, O, true> : &k_heavy_big<I, 4, O, false>;
+    case 8: return tail ? &k_heavy_big<I, 8, O, true> : &k_heavy_big<I, 8, O, false>;
+    default: return tail ? &k_heavy_big<I, 16, O, true> : &k_heavy_big<I, 16, O, false>;
   }
 }
 
+// k_heavy for an unroll and a form of reading the rows' last ids (tail: in
+// masked units; only the unit streaming of unroll 4..16 has the choice)
 template <class I, class O, bool kXcdLocal = false>
-static auto heavy_kernel(int unroll) -> decltype(&k_heavy<I, 4, O, kXcdLocal>) {
+static auto heavy_kernel(int unroll, bool tail) -> decltype(&k_heavy<I, 4, O, kXcdLocal, false>) {
   switch (unroll) {
-    case 1: return &k_heavy<I, 1, O, kXcdLocal>;
-    case 2: return &k_heavy<I, 2, O, kXcdLocal>;
-    case 4: return &k_heavy<I, 4, O, kXcdLocal>;
-    case 8: return &k_heavy<I, 8, O, kXcdLocal>;
-    default: return &k_heavy<I, 16, O, kXcdLocal>;
+    case 1: return &k_heavy<I, 1, O, kXcdLocal, false>;
+    case 2: return &k_heavy<I, 2, O, kXcdLocal, false>;
+    case 4: return tail ? &k_heavy<I, 4, O, kXcdLocal, true> : &k_heavy<I, 4, O, kXcdLocal, false>;
+    case 8: return tail ? &k_heavy<I, 8, O, kXcdLocal, true> : &k_heavy<I, 8, O, kXcdLocal, false>;
+    default:
+      return tail ? &k_heavy<I, 16, O, kXcdLocal, true> : &k_heavy<I, 16, O, kXcdLocal, false>;
   }
 }
 
@@ -1447,7 +1481,8 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              const uint8_t *skip, const int32_t *graph_of,
                              unsigned long long *count, int light_max, int chunk, int wpc,
                              int unroll, int blk_bits, bool pack_ids, bool wide, bool trim,
-                             const BatchPlan &bp, hipStream_t s, const HybridRoute *hr) {
+                             bool tail_units, const BatchPlan &bp, hipStream_t s,
+                             const HybridRoute *hr) {
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;
   if (!eu) {
@@ -1604,7 +1639,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
       StageTimer _timer(TC_STAGE_HEAVY, s);
       const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
       const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - blk_bits + 1);
-      const auto kbig = heavy_big_kernel<I, OthersPacked>(unroll);
+      const auto kbig = heavy_big_kernel<I, OthersPacked>(unroll, tail_units);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
                          (const unsigned long long *)row_stats,
                          (const int64_t *)run_ptr.as<int64_t>(),
@@ -1612,8 +1647,8 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                          (const int2 *)items.as<int2>(),
                          (const int64_t *)iptr.as<int64_t>(), n_huge, chunk, next_big, graph_of,
                          count);
-      const auto kfn = xcd_local ? heavy_kernel<I, OthersPacked, true>(unroll)
-                                 : heavy_kernel<I, OthersPacked>(unroll);
+      const auto kfn = xcd_local ? heavy_kernel<I, OthersPacked, true>(unroll, tail_units)
+                                 : heavy_kernel<I, OthersPacked>(unroll, tail_units);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R,
                          col, (const unsigned long long *)row_stats,
                          (const int64_t *)run_ptr.as<int64_t>(),
@@ -1647,13 +1682,15 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
            "block_min_bits must be in [%d, %d], got %d", kBigCenterBits, kBigMinBits, blk_bits);
   TC_CHECK(!hr || (plan == TC_PLAN_SORTED && !skip && eu && ev), TC_ERR_INVALID,
            "hybrid routing in the exact pass needs the sorted plan, an edge list and no skip mask");
+  const bool tail_units = params && params->tail_units != 0;
   if (E <= 0 || n <= 0) return TC_OK;
   if (plan == TC_PLAN_SORTED)
     return exact_sorted_plan(row_ptr, row_ptr_64, col, n, eu, ev, E, skip, graph_of, count,
                              light_max, chunk, wpc, unroll, blk_bits,
                              params && params->pack_ids != 0,
                              params && params->wide_offsets != 0,
-                             params && params->trim_rows != 0, bp ? *bp : BatchPlan{}, s, hr);
+                             params && params->trim_rows != 0, tail_units,
+                             bp ? *bp : BatchPlan{}, s, hr);
 
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;
@@ -1725,14 +1762,14 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
       StageTimer _timer(TC_STAGE_HEAVY, s);
       const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
       const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - blk_bits + 1);
-      const auto kbig = heavy_big_kernel<I, OthersIds>(unroll);
+      const auto kbig = heavy_big_kernel<I, OthersIds>(unroll, tail_units);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
                          (const unsigned long long *)row_stats,
                          (const int64_t *)cptr.as<int64_t>(),
                          OthersIds{(const int32_t *)others.as<int32_t>()},
                          (const int2 *)items.as<int2>(), (const int64_t *)iptr.as<int64_t>(),
                          n_huge, chunk, ctr.as<unsigned long long>() + 1, graph_of, count);
-      const auto kfn = heavy_kernel<I, OthersIds>(unroll);
+      const auto kfn = heavy_kernel<I, OthersIds>(unroll, tail_units);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R, col,
                          (const unsigned long long *)row_stats,
                          (const int64_t *)cptr.as<int64_t>(),

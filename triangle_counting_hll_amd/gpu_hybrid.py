@@ -82,7 +82,8 @@ def _exact_params(params: Optional[dict]):
                             int(params.get("block_min_bits", 0)),
                             int(bool(params.get("pack_ids", False))),
                             int(bool(params.get("wide_offsets", False))),
-                            int(bool(params.get("trim_rows", False))))
+                            int(bool(params.get("trim_rows", False))),
+                            int(bool(params.get("tail_units", False))))
 
 
 @_lib.on_stream
