@@ -57,6 +57,42 @@ def test_tail_units_rows_at_the_end_of_col(device, n):
         assert _count(csr, params=params) == math.comb(n, 3), params
 
 
+@pytest.mark.parametrize("order", ["csr", "csr_mode", "shuffled", "reversed", "subset"])
+def test_trim_search_any_edge_order(device, order):
+    # with trim_rows the edge pass looks for the cut of u's row next to the
+    # edge's own index (v's position in u's row in the CSR's edge order);
+    # other orders give it a useless start and take the fallback searches,
+    # so the count must not depend on the order.  subset: fewer edges than
+    # twice the vertices, where the pass reads the rows' last ids alone and
+    # searches whole rows; the count is the sum of those edges' supports
+    import torch
+
+    pairs = config_pairs("chunglu", 5)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    eu, ev = csr.edge_u, csr.edge_v
+    if order == "subset":
+        keep = np.random.default_rng(9).permutation(eu.numel())[: csr.n_nodes]
+        keep_d = torch.from_numpy(np.sort(keep)).to(eu.device)
+        eu, ev = eu[keep_d], ev[keep_d]
+        assert eu.numel() < 2 * csr.n_nodes
+        truth = int(cbind.edge_supports(csr.row_ptr.cpu().numpy(), csr.col_idx.cpu().numpy(),
+                                        eu.cpu().numpy(), ev.cpu().numpy()).sum())
+    else:
+        truth = _truth(pairs)
+    if order == "shuffled":
+        perm = torch.from_numpy(np.random.default_rng(7).permutation(eu.numel())).to(eu.device)
+        eu, ev = eu[perm], ev[perm]
+    elif order == "reversed":
+        eu, ev = eu.flip(0), ev.flip(0)
+    if order == "csr_mode":
+        eu = ev = None
+    for params in ({"trim_rows": True}, {"trim_rows": True, "light_max": 4, "tail_units": True},
+                   {"trim_rows": True, "wide_offsets": True}):
+        cnt = int(gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, eu, ev,
+                                         params=params).item())
+        assert cnt == truth, (order, params)
+
+
 def test_clique_beyond_lds_table(device):
     # K_1400: forward rows of 1024..1399 ids exceed the per-wave LDS table and
     # are taken by whole workgroups over the joined table; C(1400, 3) triangles.

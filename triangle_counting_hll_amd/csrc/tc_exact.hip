@@ -91,6 +91,20 @@ __global__ void k_last_ids(Rows<I> R, const int32_t *__restrict__ col, int64_t n
   }
 }
 
+// info[v] = {row start (low and high words), length, largest id (INT32_MIN
+// for an empty row)}: what the trimming edge pass needs of an endpoint, in
+// one aligned 16-byte entry.
+template <class I>
+__global__ void k_row_info(Rows<I> R, const int32_t *__restrict__ col, int64_t n,
+                           int4 *__restrict__ info) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = R.begin(v), e = R.end(v);
+    info[v] = make_int4((int32_t)(uint32_t)b, (int32_t)(b >> 32), (int32_t)(e - b),
+                        e > b ? col[e - 1] : INT32_MIN);
+  }
+}
+
 // Number of ids <= hi in the sorted row col[st, st + len) whose last id is
 // above hi.  An 8-ary search: each round reads the last ids of the first
 // seven eighths of the range together (one memory round trip instead of
@@ -118,6 +132,75 @@ __device__ __forceinline__ int64_t count_at_most(const int32_t *__restrict__ col
     }
   }
   return lo;
+}
+
+// count_at_most for a row whose answer is probably near a known position:
+// the answer A (the first position of the row col[ub, ub + du) holding an
+// id above hi; the last one does) is bracketed by [a, b].  The ids of a
+// window at row positions inside [a, b] are sorted, so those <= hi are a
+// prefix of them: c of nv give a = (first position) + c when c > 0 and
+// b = (first position) + c when c < nv.  First the kTrimWin ids of col from
+// qs on are read with 16-byte loads (qs is clamped so that the reads stay
+// inside col[0, total)); when the boundary is not among them, the next
+// window on the side that holds it; then the 8-ary search of what is left of
+// [a, b].  In the CSR's own edge order the edge u -> v at index e has v at
+// col[e], and on graphs whose ids follow the degrees the answer mostly lies
+// within a few ids of it, so the caller centres the first window on e + 1.
+constexpr int kTrimWin = 16;
+
+// The first window's start for an edge at index e (clamped inside col) and
+// its ids, read before the edge's rows are known.
+struct TrimWindow {
+  int64_t qs;
+  int4 w[kTrimWin / 4];
+};
+__device__ __forceinline__ void trim_window_load(const int32_t *__restrict__ col, int64_t e,
+                                                 int64_t total, TrimWindow &T) {
+  T.qs = max((int64_t)0, min(e + 1 - kTrimWin / 2, total - kTrimWin));
+#pragma unroll
+  for (int i = 0; i < kTrimWin / 4; ++i) __builtin_memcpy(&T.w[i], col + T.qs + 4 * i, 16);
+}
+
+// Narrows [a, b] by the window of ids w starting at col position qs (row
+// positions qs - ub on); returns -1, 0 or 1: look left, stop, look right.
+__device__ __forceinline__ int trim_window_step(const int4 (&w)[kTrimWin / 4], int64_t qs,
+                                                int64_t ub, int du, int32_t hi, int &a, int &b) {
+  // row position of the window's first id, clamped to [-kTrimWin, du]: a
+  // window wholly outside the row keeps its side of it
+  const int p0 = (int)max((int64_t)-kTrimWin, min(qs - ub, (int64_t)du));
+  const int pf = max(p0, a), pl = min(p0 + kTrimWin - 1, b);
+  const int nv = pl - pf + 1;  // window positions inside [a, b]
+  if (nv <= 0) return p0 + kTrimWin <= a ? 1 : (p0 > b ? -1 : 0);
+  const uint32_t span = (uint32_t)(b - a);
+  int c = 0;
+#pragma unroll
+  for (int i = 0; i < kTrimWin / 4; ++i) {
+    const int32_t x[4] = {w[i].x, w[i].y, w[i].z, w[i].w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c += ((uint32_t)(p0 + 4 * i + j - a) <= span && x[j] <= hi) ? 1 : 0;
+  }
+  if (c > 0) a = pf + c;
+  if (c < nv) b = pf + c;
+  return c == nv ? 1 : (c == 0 ? -1 : 0);
+}
+
+// count_at_most of the row col[ub, ub + du) given its first window T (read
+// by trim_window_load for the edge's index e): then, while the boundary is
+// not found, the next window on the side that holds it, then the 8-ary
+// search of what is left of [a, b].  total >= kTrimWin.
+__device__ __forceinline__ int64_t count_at_most_near(const int32_t *__restrict__ col, int64_t ub,
+                                                      int du, int32_t hi, const TrimWindow &T,
+                                                      int64_t total) {
+  int a = 0, b = du - 1;
+  int dir = trim_window_step(T.w, T.qs, ub, du, hi, a, b);
+  if (dir != 0 && a < b) {
+    const int64_t qs = max((int64_t)0, min(T.qs + dir * kTrimWin, total - kTrimWin));
+    int4 w[kTrimWin / 4];
+#pragma unroll
+    for (int i = 0; i < kTrimWin / 4; ++i) __builtin_memcpy(&w[i], col + qs + 4 * i, 16);
+    trim_window_step(w, qs, ub, du, hi, a, b);
+  }
+  return a + (a < b ? count_at_most(col, ub + a, b - a + 1, hi) : 0);
 }
 
 __device__ __forceinline__ uint32_t merge_count(const int32_t *__restrict__ col, int64_t a0,
@@ -483,6 +566,9 @@ struct PackLayout {
   // largest id (the rest cannot be in the center row), found by the edge
   // pass; else the other's whole length
   const int32_t *last_id = nullptr;
+  // the same trimming with each endpoint's row start, length and last id in
+  // one entry (k_row_info), for the edge pass instances with kNear
+  const int4 *row_info = nullptr;
   __device__ __forceinline__ int lbits() const {
     return shift - (rel_bits > 0 ? rel_bits : max(1, bit_length_u64(stats[1])));
   }
@@ -1212,8 +1298,12 @@ __device__ __forceinline__ bool net_route_round(const RouterNet &net, const floa
 
 // kRoute: 0 = plain exact pass, 1 = hybrid routing by the degree rule or a
 // route array, 2 = hybrid routing by the learned network (G: lanes per edge of
-// its sampled-register read, group_lanes(net.sample_bits)).
-template <class I, int kRoute, int G>
+// its sampled-register read, group_lanes(net.sample_bits)).  kNear: the row
+// trimming reads each endpoint's row from L.row_info and searches the cut
+// from the edge's own position (count_at_most_near); without it the trimming
+// reads L.last_id and searches the whole row (and untrimmed passes keep fewer
+// registers).
+template <class I, int kRoute, int G, bool kNear = false>
 __global__ void __launch_bounds__(kBlock)
 k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict__ eu,
                 const int32_t *__restrict__ ev, int64_t E, const unsigned long long *rows_total,
@@ -1240,7 +1330,8 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
     snorm = s_norm;
     s_src = s_list[threadIdx.x / kWave];
   }
-  const bool trim = rows_layout && L.last_id != nullptr;
+  const bool trim = rows_layout && (kNear ? L.row_info != nullptr : L.last_id != nullptr);
+  const int64_t total_ids = (int64_t)L.stats[1];  // row_ptr[n]: the ids of col
   // the loop bound is uniform per wavefront, so every lane stays active for
   // the cross-lane work below
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < E;
@@ -1249,14 +1340,26 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
     uint32_t tri = 0;
     int32_t u = 0, v = 0;
     int64_t ub = 0, ue = 0, vb = 0, ve = 0;
+    int32_t last_u = 0, last_v = 0;  // with kNear: the rows' largest ids
     const bool active = e < E_read && !(skip && skip[e]);
     if (active) {
       v = ev[e];
       u = eu[e];
-      ub = R.begin(u);
-      ue = R.end(u);
-      vb = R.begin(v);
-      ve = R.end(v);
+      if constexpr (kNear) {
+        // launched only with L.row_info set (whatever the packed layout)
+        const int4 iu = L.row_info[u], iv = L.row_info[v];
+        ub = ((int64_t)iu.y << 32) | (uint32_t)iu.x;
+        ue = ub + iu.z;
+        last_u = iu.w;
+        vb = ((int64_t)iv.y << 32) | (uint32_t)iv.x;
+        ve = vb + iv.z;
+        last_v = iv.w;
+      } else {
+        ub = R.begin(u);
+        ue = R.end(u);
+        vb = R.begin(v);
+        ve = R.end(v);
+      }
     }
     const int64_t du = ue - ub, dv = ve - vb;
     bool to_hll = false;
@@ -1314,8 +1417,20 @@ k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__res
           // the streamed row is u's own (this pass walks the rows of u, so
           // the search mostly hits cached lines); rows of v are not cut
           if (trim && cv) {
-            const int32_t hi = L.last_id[v];
-            if (L.last_id[u] > hi) olen = count_at_most(col, ub, du, hi);
+            if constexpr (kNear) {
+              if (last_u > last_v) {
+                if (total_ids >= kTrimWin) {
+                  TrimWindow tw;
+                  trim_window_load(col, e, total_ids, tw);
+                  olen = count_at_most_near(col, ub, (int)du, last_v, tw, total_ids);
+                } else {
+                  olen = count_at_most(col, ub, du, last_v);
+                }
+              }
+            } else {
+              const int32_t hi = L.last_id[v];
+              if (L.last_id[u] > hi) olen = count_at_most(col, ub, du, hi);
+            }
           }
           if (olen > 0) {
             word = (uint64_t)c << L.shift;
@@ -1527,10 +1642,18 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   unsigned long long *next_item = ctr.as<unsigned long long>();
   unsigned long long *n_heavy = next_item + 1;
   unsigned long long *row_stats = next_item + 4;  // 3 entries
-  // with trim_rows the rows' last ids go to nitems (int32[n], unused until
-  // the planner after the sort)
-  const PackLayout layout{shift, pack_ids ? 1 : 0, row_stats, bp.rel_bits,
-                          trim && !pack_ids ? nitems.as<int32_t>() : nullptr};
+  // with trim_rows each vertex's row start, length and last id go to tmp as
+  // one 16-byte entry (tmp is the radix sort's second buffer, unused until
+  // the sort after the edge pass; it holds them when E >= 2n); the learned
+  // router's edge pass, and graphs with fewer edges, take the rows' last ids
+  // alone from nitems (int32[n], unused until the planner after the sort)
+  const bool trim_on = trim && !pack_ids;
+  const bool router = hr && hr->net.params;
+  const bool near = trim_on && !router &&
+                    (int64_t)sizeof(uint64_t) * E >= (int64_t)sizeof(int4) * n;
+  PackLayout layout{shift, pack_ids ? 1 : 0, row_stats, bp.rel_bits,
+                    trim_on && !near ? nitems.as<int32_t>() : nullptr};
+  layout.row_info = near ? tmp.as<int4>() : nullptr;
   const unsigned long long *rows_total = csr_mode ? row_stats + 1 : nullptr;
 
   const unsigned gr = grid_for(max_runs + 1, kBlock, 16384);
@@ -1543,6 +1666,9 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
     Rows<I> R{(const I *)row_ptr};
     {
       StageTimer _timer(TC_STAGE_CLASSIFY, s);
+      if (layout.row_info)
+        hipLaunchKernelGGL(k_row_info<I>, dim3(grid_for(n, kBlock, 16384)), dim3(kBlock), 0, s,
+                           R, col, n, tmp.as<int4>());
       if (layout.last_id)
         hipLaunchKernelGGL(k_last_ids<I>, dim3(grid_for(n, kBlock, 16384)), dim3(kBlock), 0, s,
                            R, col, n, nitems.as<int32_t>());
@@ -1553,13 +1679,17 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              packed.as<uint64_t>(), graph_of, count, n_heavy, n, *hr);
         });
       } else if (hr) {
-        hipLaunchKernelGGL((k_classify_pack<I, 1, 1>), dim3(ge), dim3(kBlock), 0, s, R, col, eu,
-                           ev, E, rows_total, skip, light_max, layout, packed.as<uint64_t>(),
-                           graph_of, count, n_heavy, n, *hr);
+        const auto kfn = layout.row_info ? k_classify_pack<I, 1, 1, true>
+                                        : k_classify_pack<I, 1, 1, false>;
+        hipLaunchKernelGGL(kfn, dim3(ge), dim3(kBlock), 0, s, R, col, eu, ev, E, rows_total, skip,
+                           light_max, layout, packed.as<uint64_t>(), graph_of, count, n_heavy, n,
+                           *hr);
       } else {
-        hipLaunchKernelGGL((k_classify_pack<I, 0, 1>), dim3(ge), dim3(kBlock), 0, s, R, col, eu,
-                           ev, E, rows_total, skip, light_max, layout, packed.as<uint64_t>(),
-                           graph_of, count, n_heavy, n, HybridRoute{});
+        const auto kfn = layout.row_info ? k_classify_pack<I, 0, 1, true>
+                                        : k_classify_pack<I, 0, 1, false>;
+        hipLaunchKernelGGL(kfn, dim3(ge), dim3(kBlock), 0, s, R, col, eu, ev, E, rows_total, skip,
+                           light_max, layout, packed.as<uint64_t>(), graph_of, count, n_heavy, n,
+                           HybridRoute{});
       }
     }
     TC_HIP(hipGetLastError());
