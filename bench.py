@@ -528,7 +528,10 @@ def run_batch_extra(args, reps=10):
     build_s = time.perf_counter() - t0
     E = b.csr.n_oriented
     key = f"batch|G={b.n_graphs}|n={b.csr.n_nodes}|E={E}"
-    params, table = autotune.tune(lambda p: batch.batch_exact_count(b, p), key, reps=2)
+    # item_order applies to single graphs only (a batch's items are ordered by graph)
+    space = {k: v for k, v in autotune.SPACE.items() if k != "item_order"}
+    params, table = autotune.tune(lambda p: batch.batch_exact_count(b, p), key, reps=2,
+                                  space=space)
     g = batch.BatchCountGraph(b, params)
     _, ms = _timed(g.launch, reps)
     counts = g.counts.cpu().numpy()
@@ -721,7 +724,8 @@ def run_batch_workload(args, info):
     E = b.csr.n_oriented
     params = {"light_max": args.light_max, "chunk": args.chunk,
               "waves_per_cu": args.waves_per_cu, "unroll": args.unroll,
-              "trim_rows": args.trim_rows, "tail_units": args.tail_units}
+              "trim_rows": args.trim_rows, "tail_units": args.tail_units,
+              "block_min_bits": args.block_min_bits, "item_order": args.item_order}
     g = batch.BatchCountGraph(b, params)
     for _ in range(args.warmup):
         g.launch()
@@ -796,6 +800,12 @@ def main():
     ap.add_argument("--chunk", type=int, default=512)
     ap.add_argument("--waves-per-cu", type=int, default=16)
     ap.add_argument("--unroll", type=int, default=16)
+    ap.add_argument("--item-order", type=int, default=0, choices=(0, 1),
+                    help="1: work items handed out by the position of their first streamed "
+                         "row, one queue per XCD (single graphs)")
+    ap.add_argument("--block-min-bits", type=int, default=10,
+                    help="centers with at least 2^(b-1) forward ids are taken by whole "
+                         "workgroups (9..13)")
     ap.add_argument("--tail-units", type=int, default=0, choices=(0, 1),
                     help="1 = each streamed row's last len %% 4 ids are read in one masked "
                          "16-byte load with its last whole ids")
@@ -857,7 +867,8 @@ def main():
 
     params = {"light_max": args.light_max, "chunk": args.chunk,
               "waves_per_cu": args.waves_per_cu, "unroll": args.unroll,
-              "trim_rows": args.trim_rows, "tail_units": args.tail_units}
+              "trim_rows": args.trim_rows, "tail_units": args.tail_units,
+              "block_min_bits": args.block_min_bits, "item_order": args.item_order}
     count = torch.zeros(1, dtype=torch.int64, device="cuda")
     tune_evals = None
     if not args.no_autotune:
@@ -873,14 +884,20 @@ def main():
             count.zero_()
             gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v, count, p)
 
-        space = {"light_max": (24, 32, 36, 40, 48), "chunk": (256, 512, 1024),
-                 "block_min_bits": (9, 10, 11), "trim_rows": (0, 1), "tail_units": (0, 1)}
+        space = {"light_max": (24, 32, 36, 40, 48), "chunk": (128, 256, 512, 1024),
+                 "block_min_bits": (9, 10, 11), "trim_rows": (0, 1), "tail_units": (0, 1),
+                 "item_order": (0, 1)}
         t0 = time.perf_counter()
+        # a second descent starts from the row-ordered items, which pay only
+        # with short work items and no workgroup phase (three knobs at once)
         tuned, table = autotune.tune(run, f"exact|{args.workload}|n={csr.n_nodes}|E={E}",
                                      space=space, start=dict(params, block_min_bits=10,
-                                                             plan="sorted"))
+                                                             plan="sorted"),
+                                     extra_starts=[{"item_order": 1, "chunk": 256,
+                                                    "block_min_bits": 11}])
         params = {k: tuned[k] for k in ("light_max", "chunk", "waves_per_cu", "unroll",
-                                        "block_min_bits", "trim_rows", "tail_units")}
+                                        "block_min_bits", "trim_rows", "tail_units",
+                                        "item_order")}
         tune_evals = len(table)
         _log(f"autotuned {params} in {time.perf_counter() - t0:.1f}s ({tune_evals} evals)")
     for _ in range(args.warmup):

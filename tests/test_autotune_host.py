@@ -18,7 +18,7 @@ def _cost(best):
 
 def test_descent_finds_the_separable_minimum():
     best = {"chunk": 1024, "unroll": 8, "light_max": 16, "waves_per_cu": 8, "block_min_bits": 11,
-            "trim_rows": 1, "tail_units": 1}
+            "trim_rows": 1, "tail_units": 1, "item_order": 1}
     calls = []
 
     def measure(p):
@@ -52,7 +52,7 @@ def test_cache_round_trip(tmp_path, monkeypatch):
     monkeypatch.setenv("TCHLL_TUNE_CACHE", str(path))
     autotune.clear_memory()
     best = {"chunk": 256, "unroll": 16, "light_max": 64, "waves_per_cu": 16, "block_min_bits": 9,
-            "trim_rows": 0, "tail_units": 1}
+            "trim_rows": 0, "tail_units": 1, "item_order": 1}
     n = []
 
     def timer(p):
@@ -101,3 +101,24 @@ def test_tune_starts_from_the_given_knobs():
     assert seen[0] == dict(autotune.DEFAULTS, light_max=40, chunk=256)
     assert got["light_max"] == 36 and got["chunk"] == 256
     assert {p["chunk"] for p, _ in table} == {256}
+
+
+def test_extra_start_reaches_a_joint_optimum():
+    # a setting that pays only when three knobs move together: one knob at a
+    # time from the defaults never reaches it, a second descent started there
+    # does, and the faster end point wins
+    joint = {"item_order": 1, "chunk": 256, "block_min_bits": 11}
+
+    def timer(p):
+        if all(p[k] == v for k, v in joint.items()):
+            return 90.0
+        return 100.0 + sum(1.0 for k, v in joint.items() if p[k] == v)
+
+    autotune.clear_memory()
+    got, _ = autotune.tune(None, "joint", timer=timer, device="dev0", use_cache=False)
+    assert any(got[k] != v for k, v in joint.items())
+    got, table = autotune.tune(None, "joint", timer=timer, device="dev0", use_cache=False,
+                               extra_starts=[joint])
+    assert {k: got[k] for k in joint} == joint
+    keys = [tuple(sorted(p.items())) for p, _ in table]
+    assert len(keys) == len(set(keys))          # nothing timed twice

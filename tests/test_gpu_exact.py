@@ -160,6 +160,31 @@ def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc, unroll,
     assert _count(csr, params=params) == truth
 
 
+@pytest.mark.parametrize("graph", ["rmat12", "chunglu", "clique_mix"])
+@pytest.mark.parametrize("chunk,unroll,blk", [(1, 1, 8), (7, 2, 13), (64, 16, 10), (256, 4, 9),
+                                              (1024, 16, 11)])
+@pytest.mark.parametrize("layout", ["rows", "ids", "wide"])
+def test_item_order_does_not_change_count(device, graph, chunk, unroll, blk, layout):
+    # item_order: the wavefront phase's items sorted by the position of their
+    # first streamed row and handed out from one queue per XCD (the workgroup
+    # phase and the 128 KiB kernel keep their items); every item is counted
+    # once whatever the chunk, phase bounds and packed-word layout
+    if graph == "rmat12":
+        pairs = rmat(12, 16, seed=7)
+    elif graph == "chunglu":
+        pairs = config_pairs("chunglu", 6)
+    else:
+        n = 4400
+        clique = np.stack(np.triu_indices(n, 1), axis=1)
+        pairs = np.concatenate([clique, rmat(12, 16, seed=8).astype(np.int64) + n]).astype(np.int32)
+    truth = _truth(pairs)
+    csr = gpu_graph.build_csr_gpu(pairs)
+    params = {"chunk": chunk, "unroll": unroll, "block_min_bits": blk, "item_order": True,
+              "pack_ids": layout == "ids", "wide_offsets": layout == "wide",
+              "trim_rows": layout == "rows", "tail_units": unroll >= 4}
+    assert _count(csr, params=params) == truth, params
+
+
 @pytest.mark.parametrize("blk", [8, 10, 13])
 def test_block_phase_bounds_on_cliques(device, blk):
     # K_1400 rows of 128..1399 ids under every workgroup-phase lower bound,
@@ -293,7 +318,9 @@ def test_chunglu_full_size_matches_cpu_oracle(device):
     for params in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"wide_offsets": True},
                    {"trim_rows": True}, {"trim_rows": True, "wide_offsets": True},
                    {"trim_rows": True, "tail_units": True},
-                   {"tail_units": True, "wide_offsets": True, "plan": "atomic"}):
+                   {"tail_units": True, "wide_offsets": True, "plan": "atomic"},
+                   {"trim_rows": True, "tail_units": True, "item_order": True, "chunk": 256},
+                   {"item_order": True, "block_min_bits": 9, "pack_ids": True}):
         assert _count(csr, params=params) == truth, params
 
 
@@ -307,7 +334,8 @@ def test_rmat23_full_size_planners_agree(device):
     assert csr.n_oriented == FULL["rmat23"]["oriented_edges"]
     counts = {str(p): _count(csr, params=p)
               for p in ({}, {"pack_ids": True}, {"plan": "atomic"}, {"unroll": 4, "chunk": 64},
-                        {"wide_offsets": True}, {"trim_rows": True}, {"tail_units": True})}
+                        {"wide_offsets": True}, {"trim_rows": True}, {"tail_units": True},
+                        {"item_order": True, "chunk": 256}, {"item_order": True, "unroll": 2})}
     assert set(counts.values()) == {FULL["rmat23"]["triangles"]}, counts
 
 

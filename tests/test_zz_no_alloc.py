@@ -64,6 +64,10 @@ def test_noalloc_with_knobs_and_repeat(device):
     # rows cut by the edge pass (the last ids share scratch with the planner)
     trimmed = _pair(csr, params=_lib.ExactParams(16, 64, 8, 4, 0, 9, 0, 0, 1, 1))
     assert trimmed[0] == trimmed[1] == first[0]
+    # items handed out by row position from per-XCD queues (their keys and
+    # queue counters also come from the static scratch)
+    ordered = _pair(csr, params=_lib.ExactParams(16, 64, 8, 16, 0, 10, 0, 0, 1, 1, 1))
+    assert ordered[0] == ordered[1] == first[0]
 
 
 def test_noalloc_rejects_graphs_beyond_its_scratch(device):

@@ -90,7 +90,8 @@ class ExactParams(ctypes.Structure):
                 ("waves_per_cu", ctypes.c_int), ("unroll", ctypes.c_int),
                 ("plan", ctypes.c_int), ("block_min_bits", ctypes.c_int),
                 ("pack_ids", ctypes.c_int), ("wide_offsets", ctypes.c_int),
-                ("trim_rows", ctypes.c_int), ("tail_units", ctypes.c_int)]
+                ("trim_rows", ctypes.c_int), ("tail_units", ctypes.c_int),
+                ("item_order", ctypes.c_int)]
 
 
 class RouterNet(ctypes.Structure):

@@ -12,7 +12,9 @@ wavefronts over one joined LDS table -- and shorter ones by one wavefront) and
 the center row's largest id: fewer ids streamed where rows are ordered like
 the degrees, an extra search per heavy edge everywhere) and ``tail_units``
 (each streamed row's last ``len % 4`` ids read with its last whole ids in one
-masked 16-byte load, or one by one before its units).
+masked 16-byte load, or one by one before its units) and ``item_order`` (the
+wavefront phase's work items handed out in order of the position of their
+first streamed row, one range per XCD, or longest center first).
 
 The search is a coordinate descent from the library defaults, each candidate
 timed as the median of ``reps`` (at least 5) synchronised runs of the
@@ -27,12 +29,13 @@ import json
 import os
 import statistics
 import time
-from typing import Callable, Dict, Optional
+from typing import Callable, Dict, Optional, Sequence
 
 from . import _lib
 
 DEFAULTS = {"chunk": 512, "unroll": 16, "light_max": 32, "waves_per_cu": 16,
-            "block_min_bits": 10, "trim_rows": 0, "tail_units": 0, "plan": "sorted"}
+            "block_min_bits": 10, "trim_rows": 0, "tail_units": 0, "item_order": 0,
+            "plan": "sorted"}
 SPACE = {
     "chunk": (256, 512, 1024),
     "unroll": (8, 16),
@@ -41,6 +44,7 @@ SPACE = {
     "block_min_bits": (9, 10, 11),
     "trim_rows": (0, 1),
     "tail_units": (0, 1),
+    "item_order": (0, 1),
 }
 MIN_REPS = 5
 
@@ -117,12 +121,17 @@ def tune(run: Callable[[dict], None], key: str, space: Optional[dict] = None,
          reps: int = MIN_REPS, use_cache: bool = True,
          log: Optional[Callable[[str], None]] = None,
          timer: Optional[Callable[[dict], float]] = None, device: Optional[str] = None,
-         margin: float = 0.01, start: Optional[dict] = None):
+         margin: float = 0.01, start: Optional[dict] = None,
+         extra_starts: Sequence[dict] = ()):
     """Best knob setting for ``run`` (coordinate descent); returns ``(params, table)``.
 
     ``table`` lists every (params, ms) pair that was timed (empty on a cache
     hit).  ``start`` overrides knobs of the starting point (the library
-    defaults).  ``timer(params) -> ms`` replaces the device timing of ``run``
+    defaults).  Each of ``extra_starts`` (overrides of that starting point)
+    starts one more descent, for settings whose knobs only pay together (one
+    knob at a time cannot reach them); the fastest end point wins, and
+    settings timed by one descent are not timed again by the next.
+    ``timer(params) -> ms`` replaces the device timing of ``run``
     (``time_run``) and ``device`` the device name in the cache key; both exist
     so the search and the cache can be exercised without a GPU.
     """
@@ -146,6 +155,11 @@ def tune(run: Callable[[dict], None], key: str, space: Optional[dict] = None,
                 log(f"autotune {p} -> {seen[k]:.3f} ms")
         return seen[k]
 
-    best, _ = descend(measure, dict(DEFAULTS, **(start or {})), space, margin=margin)
+    base = dict(DEFAULTS, **(start or {}))
+    best, best_ms = descend(measure, base, space, margin=margin)
+    for extra in extra_starts:
+        cand, ms = descend(measure, dict(base, **extra), space, margin=margin)
+        if ms < best_ms:
+            best, best_ms = cand, ms
     _store(full_key, best)
     return best, table

@@ -382,6 +382,7 @@ k_scatter_others(Rows<I> R, const int32_t *__restrict__ eu, const int32_t *__res
 constexpr int kCenterBuckets = 32;
 constexpr int kBigCenterBits = 8;  // centers with d+ >= 128 get one item per queue ticket
 constexpr int kSmallBatch = 16;    // items per ticket for the shorter centers
+constexpr int kBandTicket = 4;     // items per ticket with item_order (one graph)
 constexpr int kXcds = 8;           // XCDs of an MI355X (one L2 each)
 constexpr int kXcdStride = 16;     // u64 counters between two XCD queues (128 B)
 
@@ -979,6 +980,52 @@ struct GraphAcc {
 
 // Four wavefronts per SIMD (at most 128 VGPRs): the LDS tables allow four
 // workgroups per CU, and the loads in flight need the occupancy.
+// item_order: the work items of the wavefront phase, [first, n_items), keyed
+// by the col position of their first other's row (its top 16 bits; any
+// coarser order groups the rows as well) above their index j = i - first;
+// slots from n_items - first on get a key that sorts last.  *n_keys = the
+// number of real keys (the sort reads only those).
+template <class I>
+__global__ void k_item_keys(Rows<I> R, OthersPacked others, const int2 *__restrict__ items,
+                            const int64_t *__restrict__ cptr, const int64_t *__restrict__ iptr,
+                            const int64_t *__restrict__ n_first_runs_p,
+                            const int64_t *__restrict__ n_items_p,
+                            const unsigned long long *__restrict__ row_stats, int chunk,
+                            int64_t max_items, uint64_t *__restrict__ keys,
+                            unsigned long long *__restrict__ n_keys) {
+  others.resolve();
+  const int64_t first = iptr[*n_first_runs_p], n_items = *n_items_p;
+  const int sh = max(0, bit_length_u64(row_stats[1]) - 16);
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 == 0) *n_keys = (unsigned long long)max((int64_t)0, n_items - first);
+  for (int64_t j = t0; j < max_items; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = first + j;
+    uint64_t key = ~0ull;
+    if (i < n_items) {
+      const int2 it = items[i];
+      const int64_t pos = cptr[it.x] + (int64_t)it.y * chunk;
+      const int32_t c = others.center(it.x, pos);
+      int64_t st = 0;
+      int len = 0;
+      others.row(R, pos, R.begin(c), st, len);
+      key = ((uint64_t)(st >> sh) << 32) | (uint64_t)j;
+    }
+    keys[j] = key;
+  }
+}
+
+// out[i] = items[i] below the wavefront phase, else the item of the i-th
+// sorted key.
+__global__ void k_item_gather(const int2 *__restrict__ items, const uint64_t *__restrict__ skeys,
+                              const int64_t *__restrict__ iptr,
+                              const int64_t *__restrict__ n_first_runs_p,
+                              const int64_t *__restrict__ n_items_p, int2 *__restrict__ out) {
+  const int64_t first = iptr[*n_first_runs_p], n_items = *n_items_p;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_items;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = i < first ? items[i] : items[first + (int64_t)(uint32_t)skeys[i - first]];
+}
+
 template <class I, int U, class O, bool kXcdLocal, bool kTail>
 __global__ void __launch_bounds__(kBlock, 4)
 k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__restrict__ row_stats,
@@ -987,8 +1034,8 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
         const int64_t *__restrict__ n_huge_runs_p, const int64_t *__restrict__ n_first_runs_p,
         const int64_t *__restrict__ n_big_centers_p, int chunk,
         unsigned long long *__restrict__ next_item, unsigned long long *__restrict__ next_block,
-        unsigned long long *__restrict__ xcd_next, const int32_t *__restrict__ graph_of,
-        unsigned long long *__restrict__ count) {
+        unsigned long long *__restrict__ xcd_next, int xcd_ticket,
+        const int32_t *__restrict__ graph_of, unsigned long long *__restrict__ count) {
   __shared__ int4 tbl_all[kWavesPerBlock][kTblInts / 4];
   __shared__ int seg_end_all[kWavesPerBlock][kWave];
   __shared__ __align__(16) int64_t seg_base_all[kWavesPerBlock][2 * kWave];
@@ -1015,9 +1062,9 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
 
   // Workgroup phase.  Every thread sees the same ticket, so the loop exits
   // together; the barrier at its top keeps the table and the ticket in place
-  // until all wavefronts are done with the previous item.  (The XCD-local
-  // form has no workgroup items.)
-  if (!kXcdLocal && first > huge) {
+  // until all wavefronts are done with the previous item.  (Batches have no
+  // workgroup items.)
+  if (first > huge) {
     int *tbl_blk = reinterpret_cast<int *>(&tbl_all[0][0]);
     while (true) {
       __syncthreads();
@@ -1044,21 +1091,24 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
 
   // Wavefront phase: each ticket gives a range of items [it0, it1).  By
   // default one counter hands out the long centers' items one per ticket,
-  // then kSmallBatch per ticket.  In the XCD-local form (batches) the items,
-  // ordered by graph, are split into kXcds consecutive ranges with one
-  // counter each in xcd_next (kXcdStride apart): workgroup b runs on XCD
-  // b % kXcds (round-robin dispatch), so its wavefronts start on that range
-  // and the graphs of one range share one L2; a wavefront whose range is used
-  // up moves on to the next and leaves after trying all of them.
+  // then kSmallBatch per ticket.  In the XCD-local form the items [first,
+  // n_items) -- ordered by graph (batches) or by the position of their first
+  // streamed row (item_order) -- are split into kXcds consecutive ranges with
+  // one counter each in xcd_next (kXcdStride apart), xcd_ticket items per
+  // ticket: workgroup b runs on XCD b % kXcds (round-robin dispatch), so its
+  // wavefronts start on that range and the rows its items stream share one
+  // L2; a wavefront whose range is used up moves on to the next and leaves
+  // after trying all of them.
   [[maybe_unused]] int q = (int)(blockIdx.x % kXcds), tried = 0;
   while (true) {
     int64_t it0, it1;
     unsigned long long ticket = 0;
     if constexpr (kXcdLocal) {
-      const int64_t lo = n_items * q / kXcds, hi = n_items * (q + 1) / kXcds;
+      const int64_t nw = n_items - first;
+      const int64_t lo = first + nw * q / kXcds, hi = first + nw * (q + 1) / kXcds;
       if (lane == 0) ticket = atomicAdd(&xcd_next[q * kXcdStride], 1ull);
-      it0 = lo + (int64_t)__shfl(ticket, 0) * kSmallBatch;
-      it1 = min(it0 + (int64_t)kSmallBatch, hi);
+      it0 = lo + (int64_t)__shfl(ticket, 0) * xcd_ticket;
+      it1 = min(it0 + (int64_t)xcd_ticket, hi);
       if (it0 >= hi) {
         q = (q + 1) % kXcds;
         if (++tried == kXcds) break;
@@ -1562,7 +1612,8 @@ static int effective_chunk(const tc_exact_params *params) {
 // Scratch bytes exact_sorted_plan carves for n vertices and E edges: its own
 // pieces in allocation order, then those of the radix sort, the run
 // compaction, the bucket sort and the item scan (each rounded by arena_round).
-static size_t exact_sorted_scratch_bytes(int64_t n, int64_t E, int chunk, bool csr_mode) {
+static size_t exact_sorted_scratch_bytes(int64_t n, int64_t E, int chunk, bool csr_mode,
+                                         bool item_order) {
   int key_bits = 1;
   while ((int64_t(1) << key_bits) <= n) ++key_bits;
   const int64_t max_items = n + E / chunk + 1;
@@ -1581,10 +1632,16 @@ static size_t exact_sorted_scratch_bytes(int64_t n, int64_t E, int chunk, bool c
       (int64_t)sizeof(int64_t) * (n + 1),                    // iptr
       (int64_t)sizeof(int2) * max_items,                     // items
       (int64_t)(7 * sizeof(unsigned long long)),             // ctr
+      item_order ? (int64_t)sizeof(uint64_t) * max_items : 0,      // ikeys
+      item_order ? (int64_t)sizeof(uint64_t) * max_items : 0,      // ikeys2
+      item_order ? (int64_t)sizeof(int2) * max_items : 0,          // items2
+      item_order ? (int64_t)sizeof(unsigned long long) : 0,        // n_keys
+      item_order ? (int64_t)sizeof(unsigned long long) * kXcds * kXcdStride : 0,  // xcd_ctr
   };
   size_t b = 0;
   for (const int64_t x : pieces) b += x > 0 ? arena_round((size_t)x) : 0;
   b += radix_sort_scratch_bytes(E, 64 - key_bits, 64, true);
+  if (item_order) b += radix_sort_scratch_bytes(max_items, 32, 48);
   b += compact_scratch_bytes(E);
   b += bucket_sort_scratch_bytes(n, kCenterBuckets);
   b += scan_scratch_bytes(n);
@@ -1596,8 +1653,8 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                              const uint8_t *skip, const int32_t *graph_of,
                              unsigned long long *count, int light_max, int chunk, int wpc,
                              int unroll, int blk_bits, bool pack_ids, bool wide, bool trim,
-                             bool tail_units, const BatchPlan &bp, hipStream_t s,
-                             const HybridRoute *hr) {
+                             bool tail_units, bool item_order, const BatchPlan &bp,
+                             hipStream_t s, const HybridRoute *hr) {
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;
   if (!eu) {
@@ -1633,8 +1690,17 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   TC_TRY(ctr.alloc(7 * sizeof(unsigned long long), s));
   // batches: runs ordered by graph, one item queue per XCD
   const bool xcd_local = bp.graph_bits > 0 && graph_of != nullptr;
-  Scratch xcd_ctr;
-  if (xcd_local) {
+  // item_order (one graph): the wavefront phase's items sorted by the
+  // position of their first streamed row, handed out from one queue per XCD
+  const bool band = item_order && !xcd_local && graph_of == nullptr;
+  Scratch xcd_ctr, ikeys, ikeys2, items2, n_keys;
+  if (band) {
+    TC_TRY(ikeys.alloc(sizeof(uint64_t) * max_items, s));
+    TC_TRY(ikeys2.alloc(sizeof(uint64_t) * max_items, s));
+    TC_TRY(items2.alloc(sizeof(int2) * max_items, s));
+    TC_TRY(n_keys.alloc(sizeof(unsigned long long), s));
+  }
+  if (xcd_local || band) {
     TC_TRY(xcd_ctr.alloc(sizeof(unsigned long long) * kXcds * kXcdStride, s));
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kBlock), 0, s, xcd_ctr.as<unsigned long long>(),
                        (int64_t)kXcds * kXcdStride, 0ull);
@@ -1765,10 +1831,31 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                        (const int64_t *)iptr.as<int64_t>(), n_sorted_runs, items.as<int2>());
     TC_HIP(hipGetLastError());
     const unsigned gh = (unsigned)std::max(1, num_cus() * wpc / kWavesPerBlock);
+    const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
+    const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - blk_bits + 1);
+    const int64_t *n_items_d = iptr.as<int64_t>() + max_runs;
+    const int2 *heavy_items = items.as<int2>();
+    if (band) {
+      const unsigned gi = grid_for(max_items, kBlock, 16384);
+      hipLaunchKernelGGL(k_item_keys<I>, dim3(gi), dim3(kBlock), 0, s, R,
+                         OthersPacked{(const uint64_t *)sorted, layout},
+                         (const int2 *)items.as<int2>(), (const int64_t *)run_ptr.as<int64_t>(),
+                         (const int64_t *)iptr.as<int64_t>(), n_first, n_items_d,
+                         (const unsigned long long *)row_stats, chunk, max_items,
+                         ikeys.as<uint64_t>(), n_keys.as<unsigned long long>());
+      TC_HIP(hipGetLastError());
+      uint64_t *skeys = nullptr;
+      TC_TRY(radix_sort_u64(ikeys.as<uint64_t>(), ikeys2.as<uint64_t>(), max_items, 32, 48,
+                            &skeys, s, nullptr, n_keys.as<unsigned long long>()));
+      hipLaunchKernelGGL(k_item_gather, dim3(gi), dim3(kBlock), 0, s,
+                         (const int2 *)items.as<int2>(), (const uint64_t *)skeys,
+                         (const int64_t *)iptr.as<int64_t>(), n_first, n_items_d,
+                         items2.as<int2>());
+      TC_HIP(hipGetLastError());
+      heavy_items = items2.as<int2>();
+    }
     {
       StageTimer _timer(TC_STAGE_HEAVY, s);
-      const int64_t *n_huge = starts.as<int64_t>() + (kCenterBuckets - kBigMinBits + 1);
-      const int64_t *n_first = starts.as<int64_t>() + (kCenterBuckets - blk_bits + 1);
       const auto kbig = heavy_big_kernel<I, OthersPacked>(unroll, tail_units);
       hipLaunchKernelGGL(kbig, dim3(num_cus()), dim3(kBlock), 0, s, R, col,
                          (const unsigned long long *)row_stats,
@@ -1777,17 +1864,17 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                          (const int2 *)items.as<int2>(),
                          (const int64_t *)iptr.as<int64_t>(), n_huge, chunk, next_big, graph_of,
                          count);
-      const auto kfn = xcd_local ? heavy_kernel<I, OthersPacked, true>(unroll, tail_units)
-                                 : heavy_kernel<I, OthersPacked>(unroll, tail_units);
+      const auto kfn = (xcd_local || band)
+                           ? heavy_kernel<I, OthersPacked, true>(unroll, tail_units)
+                           : heavy_kernel<I, OthersPacked>(unroll, tail_units);
       hipLaunchKernelGGL(kfn, dim3(gh), dim3(kBlock), 0, s, R,
                          col, (const unsigned long long *)row_stats,
                          (const int64_t *)run_ptr.as<int64_t>(),
-                         OthersPacked{(const uint64_t *)sorted, layout},
-                         (const int2 *)items.as<int2>(),
-                         (const int64_t *)(iptr.as<int64_t>() + max_runs),
+                         OthersPacked{(const uint64_t *)sorted, layout}, heavy_items, n_items_d,
                          (const int64_t *)iptr.as<int64_t>(), n_huge, n_first, n_big_runs, chunk,
-                         next_item, next_blk, xcd_local ? xcd_ctr.as<unsigned long long>() : nullptr,
-                         graph_of, count);
+                         next_item, next_blk,
+                         (xcd_local || band) ? xcd_ctr.as<unsigned long long>() : nullptr,
+                         band ? kBandTicket : kSmallBatch, graph_of, count);
     }
     TC_HIP(hipGetLastError());
   });
@@ -1820,7 +1907,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                              params && params->pack_ids != 0,
                              params && params->wide_offsets != 0,
                              params && params->trim_rows != 0, tail_units,
-                             bp ? *bp : BatchPlan{}, s, hr);
+                             params && params->item_order != 0, bp ? *bp : BatchPlan{}, s, hr);
 
   Scratch eu_buf;
   const bool csr_mode = eu == nullptr;
@@ -1907,7 +1994,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                          (const int2 *)items.as<int2>(), (const int64_t *)(iptr.as<int64_t>() + n),
                          (const int64_t *)iptr.as<int64_t>(), n_huge, n_first, n_big_cent, chunk,
                          ctr.as<unsigned long long>(), ctr.as<unsigned long long>() + 2, nullptr,
-                         graph_of, count);
+                         kSmallBatch, graph_of, count);
     }
     TC_HIP(hipGetLastError());
   });
@@ -1985,7 +2072,8 @@ extern "C" int tc_exact_count_noalloc(const void *row_ptr, int row_ptr_64,
   TC_TRY(ensure_init());
   if (n_edges == 0 || n_nodes == 0) return TC_OK;
   const size_t need =
-      exact_sorted_scratch_bytes(n_nodes, n_edges, effective_chunk(params), edge_u == nullptr);
+      exact_sorted_scratch_bytes(n_nodes, n_edges, effective_chunk(params), edge_u == nullptr,
+                                 params && params->item_order != 0);
   TC_CHECK(need <= kNoAllocScratchBytes, TC_ERR_INVALID,
            "%lld nodes and %lld edges need %zu bytes of scratch; tc_exact_count_noalloc holds "
            "%zu (use tc_exact_count for this graph)",

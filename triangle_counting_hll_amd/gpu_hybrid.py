@@ -83,7 +83,8 @@ def _exact_params(params: Optional[dict]):
                             int(bool(params.get("pack_ids", False))),
                             int(bool(params.get("wide_offsets", False))),
                             int(bool(params.get("trim_rows", False))),
-                            int(bool(params.get("tail_units", False))))
+                            int(bool(params.get("tail_units", False))),
+                            int(bool(params.get("item_order", False))))
 
 
 @_lib.on_stream
