@@ -383,6 +383,7 @@ constexpr int kCenterBuckets = 32;
 constexpr int kBigCenterBits = 8;  // centers with d+ >= 128 get one item per queue ticket
 constexpr int kSmallBatch = 16;    // items per ticket for the shorter centers
 constexpr int kBandTicket = 4;     // items per ticket with item_order (one graph)
+constexpr int kBatchTicket = 8;    // items per ticket of a batch's per-XCD queues
 constexpr int kXcds = 8;           // XCDs of an MI355X (one L2 each)
 constexpr int kXcdStride = 16;     // u64 counters between two XCD queues (128 B)
 
@@ -1874,7 +1875,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
                          (const int64_t *)iptr.as<int64_t>(), n_huge, n_first, n_big_runs, chunk,
                          next_item, next_blk,
                          (xcd_local || band) ? xcd_ctr.as<unsigned long long>() : nullptr,
-                         band ? kBandTicket : kSmallBatch, graph_of, count);
+                         band ? kBandTicket : kBatchTicket, graph_of, count);
     }
     TC_HIP(hipGetLastError());
   });
