@@ -1353,9 +1353,11 @@ __device__ __forceinline__ bool net_route_round(const RouterNet &net, const floa
 // trimming reads each endpoint's row from L.row_info and searches the cut
 // from the edge's own position (count_at_most_near); without it the trimming
 // reads L.last_id and searches the whole row (and untrimmed passes keep fewer
-// registers).
+// registers).  The kNear instances are held to 64 VGPRs (8 wavefronts per
+// SIMD, as the untrimmed pass) so that the search's registers do not cost
+// occupancy in this latency-bound pass.
 template <class I, int kRoute, int G, bool kNear = false>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kNear ? 8 : 1)))
 k_classify_pack(Rows<I> R, const int32_t *__restrict__ col, const int32_t *__restrict__ eu,
                 const int32_t *__restrict__ ev, int64_t E, const unsigned long long *rows_total,
                 const uint8_t *__restrict__ skip, int light_max, PackLayout L,
