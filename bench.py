@@ -244,13 +244,14 @@ def _timed(fn, reps):
     return out, (time.perf_counter() - t0) * 1e3 / reps
 
 
-def run_hybrid_extra(csr, truth, reps=3):
+def run_hybrid_extra(csr, truth, reps=3, params=None):
     """Fixed-threshold hybrid on the same graph (src/gpu_hybrid.py:357-431 rule).
 
     HLL p = 12, sketches for forward degree >= theta = gamma (the reference
     experiment passes theta = gamma, experiment/exp_gpu.py:556-557), compact
     register slots.  gamma = 1024 is the configured threshold; gamma = 128 is
-    also reported because no forward degree of this graph reaches 1024.
+    also reported because no forward degree of this graph reaches 1024.  The
+    exact part runs with ``params`` (the exact count's knobs).
     """
     import torch
 
@@ -275,7 +276,8 @@ def run_hybrid_extra(csr, truth, reps=3):
 
         def count():
             c, s, fb = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
-                                               csr.degrees, regs, has, gamma, p, slot=slot)
+                                               csr.degrees, regs, has, gamma, p, slot=slot,
+                                               params=params)
             return c, s
 
         (c, s), ms = _timed(count, reps)
@@ -285,16 +287,17 @@ def run_hybrid_extra(csr, truth, reps=3):
             "rule": "forward-degree proxy: HLL iff min(d+u, d+v) >= gamma and both sketched "
                     "(src/hybrid.py:105-112); sketches for d+ >= gamma",
             "p": p, "sketched_vertices": n_sk, "sketch_build_ms": build_ms, "count_ms": ms,
+            "exact_part_knobs": dict(params) if params else "library defaults",
             "estimate": est, "rel_err": gpu_hybrid.relative_error(est, truth),
             "n_approx": int(st[0]), "n_exact": int(st[1]), "n_clamped": int(st[2]),
             "approx_coverage": int(st[0]) / max(csr.n_oriented, 1),
         }
         del regs
-    out["undirected_gamma1024"] = run_undirected_proxy(csr, truth, 1024, p, reps)
+    out["undirected_gamma1024"] = run_undirected_proxy(csr, truth, 1024, p, reps, params)
     return out
 
 
-def run_undirected_proxy(csr, truth, gamma, p, reps):
+def run_undirected_proxy(csr, truth, gamma, p, reps, params=None):
     """The hybrid with the undirected-degree proxy: HLL iff min(D(u), D(v)) >=
     gamma, D = undirected degree (d+ plus in-degree of the oriented graph), and
     both endpoints sketched; sketches (of the forward rows, as always) for
@@ -327,7 +330,7 @@ def run_undirected_proxy(csr, truth, gamma, p, reps):
         route = ((und[csr.edge_u.long()] >= gamma) & (und[csr.edge_v.long()] >= gamma))
         c, s, fb = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v,
                                            deg, regs, has, 0, p, route=route.to(torch.uint8),
-                                           slot=slot)
+                                           slot=slot, params=params)
         return c, s
 
     (c, s), ms = _timed(count, reps)
@@ -336,7 +339,8 @@ def run_undirected_proxy(csr, truth, gamma, p, reps):
     return {"rule": "undirected-degree proxy: HLL iff min(D(u), D(v)) >= gamma (D = d+ + d-) "
                     "and both sketched; sketches for D >= gamma; support from d+",
             "p": p, "gamma": gamma, "sketched_vertices": n_sk, "sketch_build_ms": build_ms,
-            "count_ms": ms, "estimate": est, "rel_err": gpu_hybrid.relative_error(est, truth),
+            "count_ms": ms, "exact_part_knobs": dict(params) if params else "library defaults",
+            "estimate": est, "rel_err": gpu_hybrid.relative_error(est, truth),
             "n_approx": int(st[0]), "n_exact": int(st[1]), "n_clamped": int(st[2]),
             "approx_coverage": int(st[0]) / max(csr.n_oriented, 1)}
 
@@ -951,7 +955,8 @@ def main():
 
     extras = {}
     if not args.no_extras and not args.scale_down:
-        for name, fn in (("hybrid_fixed_threshold", lambda: run_hybrid_extra(csr, gpu_count)),
+        for name, fn in (("hybrid_fixed_threshold",
+                          lambda: run_hybrid_extra(csr, gpu_count, params=params)),
                          ("ingest_text", lambda: run_ingest_extra(args.workload, csr)),
                          ("batch4096", lambda: run_batch_extra(args)),
                          ("learned_router", lambda: run_router_extra()),

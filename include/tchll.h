@@ -296,6 +296,19 @@ int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
                     const uint8_t *route, int part, double *counts, int64_t *stats,
                     int64_t *fallback, double *per_edge, int hash_mode, void *stream);
 
+/* tc_hybrid_count with the exact part's tuning knobs (tc_exact_params, NULL =
+ * the library defaults, which tc_hybrid_count uses).  No reference
+ * counterpart (the reference's hybrid has no knobs): for callers that tuned
+ * the exact count on a graph and want its hybrid on the same settings.  The
+ * knobs change the exact part's speed, never its supports. */
+int tc_hybrid_count_tuned(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
+                          int64_t n_nodes, const int32_t *edge_u, const int32_t *edge_v,
+                          int64_t n_edges, const int32_t *degrees, const uint8_t *regs,
+                          const int32_t *slot, const uint8_t *has_sketch, int64_t gamma, int p,
+                          const uint8_t *route, int part, double *counts, int64_t *stats,
+                          int64_t *fallback, double *per_edge, int hash_mode,
+                          const tc_exact_params *params, void *stream);
+
 /* ---- learned exact-vs-sketch router -------------------------------------- */
 /* No reference counterpart: the reference routes by a fixed degree threshold
  * (src/hybrid.py:107-112, src/gpu_hybrid.py:80-89); these entry points let a

@@ -156,6 +156,30 @@ def test_hybrid_matches_oracle(device, gamma, theta, split):
         assert st["n_approx"] == 0 and est == float(O.exact_count_csr(rp, col))
 
 
+@pytest.mark.parametrize("split", [True, False])
+def test_hybrid_tuned_knobs_give_the_same_result(device, split):
+    # tc_hybrid_count_tuned: the exact part with the exact count's knobs
+    # (row trimming, row-ordered items, small chunks, masked tails, no
+    # workgroup phase); the supports, hence the estimate and the stats, are
+    # those of the default knobs and of the oracle
+    p = 10
+    rp, col, eu, ev, csr, regs, has, ro, ho = _hybrid_setup(p, 0)
+    ref = O.hybrid_count(rp, col, eu, ev, ro, ho, 24)
+    base = gpu_hybrid.gpu_hybrid_triangle_count(
+        csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v, csr.degrees, regs, has, 24, p,
+        split_kernels=split)
+    for params in ({"trim_rows": True, "item_order": True, "chunk": 64, "tail_units": True,
+                    "block_min_bits": 13, "light_max": 40},
+                   {"pack_ids": True, "unroll": 4}):
+        est, _, st = gpu_hybrid.gpu_hybrid_triangle_count(
+            csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v, csr.degrees, regs, has, 24, p,
+            split_kernels=split, params=params)
+        assert est == base[0], params
+        for k in ("n_approx", "n_exact", "n_clamped", "n_fallback"):
+            assert st[k] == base[2][k] == ref["stats"][k], (k, params)
+    assert math.isclose(base[0], ref["total_fsum"], rel_tol=1e-12, abs_tol=1e-9)
+
+
 def test_hybrid_known_answer_all_exact(device):
     # cpu_exp/triangle_hll_experiment.py:1078-1084: threshold 1e9 -> 4, no HLL edges
     csr = gpu_graph.build_csr_gpu(O.SELF_TEST_EDGES)

@@ -56,6 +56,8 @@ _SIGNATURES = {
     "tc_union_estimate": (I32, [P, P, I32, P, P, I64, P, I32, P]),
     "tc_hybrid_count": (I32, [P, I32, P, I64, P, P, I64, P, P, P, P, I64, I32, P, I32, P, P, P, P,
                               I32, P]),
+    "tc_hybrid_count_tuned": (I32, [P, I32, P, I64, P, P, I64, P, P, P, P, I64, I32, P, I32, P, P,
+                                    P, P, I32, P, P]),
     "tc_router_features": (I32, [P, P, P, P, I64, P, P, I32, I32, P, P, P]),
     "tc_router_n_params": (I32, []),
     "tc_mlp_forward": (I32, [P, P, P, I64, P, P]),

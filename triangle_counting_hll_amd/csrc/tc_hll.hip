@@ -510,7 +510,8 @@ static int hybrid_impl(const void *row_ptr, int row_ptr_64, const int32_t *col_i
                        const int32_t *slot, const uint8_t *has_sketch, int64_t gamma, int p,
                        const uint8_t *route, int part, double *counts, int64_t *stats,
                        int64_t *fallback, double *per_edge, int hash_mode, hipStream_t s,
-                       const RouterNet *net, uint8_t *use_out, int64_t *n_cand) {
+                       const RouterNet *net, uint8_t *use_out, int64_t *n_cand,
+                       const tc_exact_params *params = nullptr) {
   TC_CHECK(valid_hash_mode(hash_mode), TC_ERR_INVALID, "unknown hash mode %d", hash_mode);
   TC_CHECK(part == TC_PART_BOTH || part == TC_PART_EXACT || part == TC_PART_HLL, TC_ERR_INVALID,
            "unknown part %d", part);
@@ -561,7 +562,7 @@ static int hybrid_impl(const void *row_ptr, int row_ptr_64, const int32_t *col_i
         HybridRoute hr{degrees, has_sketch, gamma, route, use, counters, RouterNet{}};
         if (net) hr.net = *net;
         TC_TRY(exact_count_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, E,
-                                nullptr, nullptr, exact_sum, nullptr, s, &hr));
+                                nullptr, nullptr, exact_sum, params, s, &hr));
         if (per_edge)
           hipLaunchKernelGGL(k_exact_per_edge<I>, dim3(grid_for(E, kBlock, 16384)), dim3(kBlock),
                              0, s, R, col_idx, edge_u, edge_v, E,
@@ -623,6 +624,22 @@ extern "C" int tc_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_
   return hybrid_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges, degrees,
                      regs, slot, has_sketch, gamma, p, route, part, counts, stats, fallback,
                      per_edge, hash_mode, (hipStream_t)stream, nullptr, nullptr, nullptr);
+}
+
+extern "C" int tc_hybrid_count_tuned(const void *row_ptr, int row_ptr_64,
+                                     const int32_t *col_idx, int64_t n_nodes,
+                                     const int32_t *edge_u, const int32_t *edge_v,
+                                     int64_t n_edges, const int32_t *degrees, const uint8_t *regs,
+                                     const int32_t *slot, const uint8_t *has_sketch,
+                                     int64_t gamma, int p, const uint8_t *route, int part,
+                                     double *counts, int64_t *stats, int64_t *fallback,
+                                     double *per_edge, int hash_mode,
+                                     const tc_exact_params *params, void *stream) {
+  using namespace tc;
+  Frame _frame((hipStream_t)stream);
+  return hybrid_impl(row_ptr, row_ptr_64, col_idx, n_nodes, edge_u, edge_v, n_edges, degrees,
+                     regs, slot, has_sketch, gamma, p, route, part, counts, stats, fallback,
+                     per_edge, hash_mode, (hipStream_t)stream, nullptr, nullptr, nullptr, params);
 }
 
 extern "C" int tc_router_hybrid_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,

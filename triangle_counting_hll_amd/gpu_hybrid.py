@@ -136,8 +136,13 @@ def edge_supports(row_ptr, col_idx, edge_u, edge_v, stream=None):
 @_lib.on_stream
 def hybrid_count(row_ptr, col_idx, edge_u, edge_v, degrees, sketch_regs, has_sketch, gamma,
                  p, part=_lib.TC_PART_BOTH, counts=None, stats=None, fallback=None,
-                 route=None, slot=None, per_edge=None, hash_mode: str = "gpu", stream=None):
-    """Enqueue one hybrid pass (``tc_hybrid_count``); returns ``(counts, stats, fallback)``."""
+                 route=None, slot=None, per_edge=None, hash_mode: str = "gpu", params=None,
+                 stream=None):
+    """Enqueue one hybrid pass (``tc_hybrid_count``); returns ``(counts, stats, fallback)``.
+
+    ``params`` (the exact count's knobs, as for ``exact_count``) selects
+    ``tc_hybrid_count_tuned``: the exact part runs with those knobs.
+    """
     t = _lib.require_device()
     eu = _lib.as_device(edge_u, t.int32, "edge_u")
     ev = _lib.as_device(edge_v, t.int32, "edge_v")
@@ -162,11 +167,16 @@ def hybrid_count(row_ptr, col_idx, edge_u, edge_v, degrees, sketch_regs, has_ske
         stats = t.zeros(3, dtype=t.int64, device=dev)
     if fallback is None:
         fallback = t.zeros(1, dtype=t.int64, device=dev)
-    _lib.check(_lib.load().tc_hybrid_count(
-        _lib.ptr(rp), is64, _lib.ptr(col), n, _lib.ptr(eu), _lib.ptr(ev), eu.numel(),
-        _lib.ptr(deg), _lib.ptr(regs), _lib.ptr(sl), _lib.ptr(has), int(gamma), int(p),
-        _lib.ptr(rt), int(part), _lib.ptr(counts), _lib.ptr(stats), _lib.ptr(fallback),
-        _lib.ptr(per_edge), _lib.hash_mode(hash_mode), _lib.stream_ptr(stream)))
+    args = (_lib.ptr(rp), is64, _lib.ptr(col), n, _lib.ptr(eu), _lib.ptr(ev), eu.numel(),
+            _lib.ptr(deg), _lib.ptr(regs), _lib.ptr(sl), _lib.ptr(has), int(gamma), int(p),
+            _lib.ptr(rt), int(part), _lib.ptr(counts), _lib.ptr(stats), _lib.ptr(fallback),
+            _lib.ptr(per_edge), _lib.hash_mode(hash_mode))
+    ep = _exact_params(params)
+    if ep is None:
+        _lib.check(_lib.load().tc_hybrid_count(*args, _lib.stream_ptr(stream)))
+    else:
+        _lib.check(_lib.load().tc_hybrid_count_tuned(*args, ctypes.byref(ep),
+                                                      _lib.stream_ptr(stream)))
     return counts, stats, fallback
 
 
@@ -248,13 +258,14 @@ def gpu_exact_triangle_count(row_ptr_d, col_idx_d, edge_u_d, edge_v_d, params=No
 def gpu_hybrid_triangle_count(row_ptr_d, col_idx_d, edge_u_d, edge_v_d, degrees_d,
                               sketch_regs_d, has_sketch_d, gamma: int, p: int,
                               split_kernels: bool = True, truth: Optional[int] = None,
-                              route=None, slot=None, hash_mode: str = "gpu"):
+                              route=None, slot=None, hash_mode: str = "gpu", params=None):
     """Hybrid estimate, wall seconds and stats (src/gpu_hybrid.py:357-431).
 
     ``stats`` holds the reference keys ``n_approx, n_exact, n_clamped,
     approx_coverage, clamp_ratio, kernel_mode`` plus ``n_fallback`` and
     ``fallback_ratio`` (src/hybrid.py:152-164) and, when ``truth`` is given,
-    ``rel_err`` (src/utils.py:36-43).
+    ``rel_err`` (src/utils.py:36-43).  ``params`` (not in the reference):
+    the exact part's knobs, as for ``gpu_exact_triangle_count``.
     """
     _require_supported_block_size()
     _warn_if_float64_atomic_slow()
@@ -275,12 +286,12 @@ def gpu_hybrid_triangle_count(row_ptr_d, col_idx_d, edge_u_d, edge_v_d, degrees_
     t0 = time.perf_counter()
     if split_kernels:
         hybrid_count(rp, col, eu, ev, deg, None, has, gamma, p, _lib.TC_PART_EXACT, counts,
-                     stats_d, fb, route=route, slot=slot, hash_mode=hash_mode)
+                     stats_d, fb, route=route, slot=slot, hash_mode=hash_mode, params=params)
         hybrid_count(None, None, eu, ev, deg, regs, has, gamma, p, _lib.TC_PART_HLL, counts,
                      stats_d, None, route=route, slot=slot, hash_mode=hash_mode)
     else:
         hybrid_count(rp, col, eu, ev, deg, regs, has, gamma, p, _lib.TC_PART_BOTH, counts,
-                     stats_d, fb, route=route, slot=slot, hash_mode=hash_mode)
+                     stats_d, fb, route=route, slot=slot, hash_mode=hash_mode, params=params)
     _sync()
     elapsed = time.perf_counter() - t0
     estimate = float(counts.item())
