@@ -53,7 +53,10 @@ def test_tail_units_rows_at_the_end_of_col(device, n):
     csr = gpu_graph.build_csr_gpu(pairs)
     for params in ({"light_max": 1, "tail_units": True},
                    {"light_max": 1, "tail_units": True, "trim_rows": True},
-                   {"light_max": 1, "tail_units": True, "plan": "atomic"}):
+                   {"light_max": 1, "tail_units": True, "plan": "atomic"},
+                   {"light_max": 1, "tail_units": True, "trim_rows": True, "item_order": True,
+                    "chunk": 1},
+                   {"light_max": 1, "item_order": True, "block_min_bits": 8, "chunk": 3}):
         assert _count(csr, params=params) == math.comb(n, 3), params
 
 
@@ -348,5 +351,6 @@ def test_rgg21_full_size_matches_cpu_oracle(device):
     col = csr.col_idx.cpu().numpy()
     truth = cbind.exact_count(rp, col)
     assert truth == _truth(config_pairs("rgg21"))
-    for params in ({}, {"pack_ids": True}, {"light_max": 2}):
+    for params in ({}, {"pack_ids": True}, {"light_max": 2},
+                   {"light_max": 2, "item_order": True, "trim_rows": True, "chunk": 128}):
         assert _count(csr, params=params) == truth, params
