@@ -162,7 +162,7 @@ typedef struct tc_exact_params {
                          load); 0 = read one by one before the row's units   */
   int item_order;    /* sorted plan, one graph: 1 = the work items of the
                          wavefront phase are handed out in order of the col
-                         position of their first streamed row, in one range
+                         position of their middle streamed row, in one range
                          per XCD (items running at once stream nearby rows,
                          so each XCD's L2 serves more of them; 4 items per
                          ticket); 0 = longest center row first from one

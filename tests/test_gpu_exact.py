@@ -169,7 +169,7 @@ def test_tuning_knobs_do_not_change_count(device, light_max, chunk, wpc, unroll,
 @pytest.mark.parametrize("layout", ["rows", "ids", "wide"])
 def test_item_order_does_not_change_count(device, graph, chunk, unroll, blk, layout):
     # item_order: the wavefront phase's items sorted by the position of their
-    # first streamed row and handed out from one queue per XCD (the workgroup
+    # middle streamed row and handed out from one queue per XCD (the workgroup
     # phase and the 128 KiB kernel keep their items); every item is counted
     # once whatever the chunk, phase bounds and packed-word layout
     if graph == "rmat12":

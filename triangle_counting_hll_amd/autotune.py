@@ -14,7 +14,7 @@ the degrees, an extra search per heavy edge everywhere) and ``tail_units``
 (each streamed row's last ``len % 4`` ids read with its last whole ids in one
 masked 16-byte load, or one by one before its units) and ``item_order`` (the
 wavefront phase's work items handed out in order of the position of their
-first streamed row, one range per XCD, or longest center first).
+middle streamed row, one range per XCD, or longest center first).
 
 The search is a coordinate descent from the library defaults, each candidate
 timed as the median of ``reps`` (at least 5) synchronised runs of the

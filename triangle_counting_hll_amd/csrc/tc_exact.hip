@@ -1699,7 +1699,7 @@ static int exact_sorted_plan(const void *row_ptr, int row_ptr_64, const int32_t 
   // batches: runs ordered by graph, one item queue per XCD
   const bool xcd_local = bp.graph_bits > 0 && graph_of != nullptr;
   // item_order (one graph): the wavefront phase's items sorted by the
-  // position of their first streamed row, handed out from one queue per XCD
+  // position of their middle streamed row, handed out from one queue per XCD
   const bool band = item_order && !xcd_local && graph_of == nullptr;
   Scratch xcd_ctr, ikeys, ikeys2, items2, n_keys;
   if (band) {
@@ -2002,7 +2002,7 @@ int exact_count_impl(const void *row_ptr, int row_ptr_64, const int32_t *col, in
                          (const int2 *)items.as<int2>(), (const int64_t *)(iptr.as<int64_t>() + n),
                          (const int64_t *)iptr.as<int64_t>(), n_huge, n_first, n_big_cent, chunk,
                          ctr.as<unsigned long long>(), ctr.as<unsigned long long>() + 2, nullptr,
-                         kSmallBatch, graph_of, count);
+                         kSmallBatch /* unused: no XCD queues */, graph_of, count);
     }
     TC_HIP(hipGetLastError());
   });
