@@ -805,7 +805,7 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=16)
     ap.add_argument("--unroll", type=int, default=16)
     ap.add_argument("--item-order", type=int, default=0, choices=(0, 1),
-                    help="1: work items handed out by the position of their first streamed "
+                    help="1: work items handed out by the position of their middle streamed "
                          "row, one queue per XCD (single graphs)")
     ap.add_argument("--block-min-bits", type=int, default=10,
                     help="centers with at least 2^(b-1) forward ids are taken by whole "
