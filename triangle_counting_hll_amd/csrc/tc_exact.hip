@@ -148,8 +148,8 @@ __device__ __forceinline__ int64_t count_at_most(const int32_t *__restrict__ col
 // within a few ids of it, so the caller centres the first window on e + 1.
 constexpr int kTrimWin = 16;
 
-// The first window's start for an edge at index e (clamped inside col) and
-// its ids, read before the edge's rows are known.
+// The first window of the search for the edge at index e: its start in col
+// (centred on e + 1, clamped inside col) and its kTrimWin ids.
 struct TrimWindow {
   int64_t qs;
   int4 w[kTrimWin / 4];
@@ -1098,7 +1098,7 @@ k_heavy(Rows<I> R, const int32_t *__restrict__ col, const unsigned long long *__
   // Wavefront phase: each ticket gives a range of items [it0, it1).  By
   // default one counter hands out the long centers' items one per ticket,
   // then kSmallBatch per ticket.  In the XCD-local form the items [first,
-  // n_items) -- ordered by graph (batches) or by the position of their first
+  // n_items) -- ordered by graph (batches) or by the position of their middle
   // streamed row (item_order) -- are split into kXcds consecutive ranges with
   // one counter each in xcd_next (kXcdStride apart), xcd_ticket items per
   // ticket: workgroup b runs on XCD b % kXcds (round-robin dispatch), so its
