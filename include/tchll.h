@@ -183,7 +183,9 @@ typedef struct tc_exact_params {
  * for at most 2^24 edges outside stream capture the call synchronises the
  * stream once, after the merge pass, and returns early when no edge needs the
  * hash-probe path (small inputs are bound by launch latency); the same holds
- * for the exact part of tc_hybrid_count.
+ * for the exact part of tc_hybrid_count.  n_nodes and n_edges must be below
+ * 2^31 - 1 (the planner keeps edge positions in 32-bit fields); larger inputs
+ * return TC_ERR_INVALID before any launch.
  * Replaces src/gpu_hybrid.py:96-130 (exact_count_kernel) and :332-354
  * (gpu_exact_triangle_count). */
 int tc_exact_count(const void *row_ptr, int row_ptr_64, const int32_t *col_idx,
