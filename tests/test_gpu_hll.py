@@ -346,3 +346,14 @@ def test_chunglu_full_size_sketches_and_hybrid(device):
     # the device total is the sum of its per-edge supports (numpy's pairwise
     # sum is within ~1e-15 relative here)
     assert math.isclose(float(c.item()), float(np.sum(per_h)), rel_tol=1e-12)
+    # with the exact count's tuned knobs (the bench's hybrid lines): the same
+    # per-edge supports, total and stats
+    per_t = torch.zeros_like(per)
+    tuned = {"light_max": 40, "chunk": 128, "block_min_bits": 11, "trim_rows": True,
+             "tail_units": True, "item_order": True}
+    c_t, s_t, _ = gpu_hybrid.hybrid_count(csr.row_ptr, csr.col_idx, eu, ev, deg, regs, has,
+                                          gamma, p, slot=slot, per_edge=per_t, params=tuned)
+    torch.cuda.synchronize()
+    assert np.array_equal(per_t.cpu().numpy(), per_h)
+    assert float(c_t.item()) == float(c.item())
+    assert np.array_equal(s_t.cpu().numpy(), st)
