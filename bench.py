@@ -888,17 +888,19 @@ def main():
             count.zero_()
             gpu_hybrid.exact_count(csr.row_ptr, csr.col_idx, csr.edge_u, csr.edge_v, count, p)
 
-        space = {"light_max": (24, 32, 36, 40, 48), "chunk": (128, 256, 512, 1024),
+        space = {"light_max": (24, 32, 36, 40, 44, 48), "chunk": (128, 256, 512, 1024),
                  "block_min_bits": (9, 10, 11), "trim_rows": (0, 1), "tail_units": (0, 1),
                  "item_order": (0, 1)}
         t0 = time.perf_counter()
         # a second descent starts from the row-ordered items, which pay only
-        # with short work items and no workgroup phase (three knobs at once)
+        # with short work items and no workgroup phase (three knobs at once),
+        # and with more edges merged in the edge pass (its gain per step is
+        # below the descent's 1% margin)
         tuned, table = autotune.tune(run, f"exact|{args.workload}|n={csr.n_nodes}|E={E}",
                                      space=space, start=dict(params, block_min_bits=10,
                                                              plan="sorted"),
                                      extra_starts=[{"item_order": 1, "chunk": 256,
-                                                    "block_min_bits": 11}])
+                                                    "block_min_bits": 11, "light_max": 44}])
         params = {k: tuned[k] for k in ("light_max", "chunk", "waves_per_cu", "unroll",
                                         "block_min_bits", "trim_rows", "tail_units",
                                         "item_order")}
